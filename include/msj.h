@@ -1,0 +1,154 @@
+/*
+ * msj.h -- C ABI of the MI355X-native MiSTIC epsilon self-join.
+ *
+ * Drop-in boundary for the reference's GPU join path:
+ *   struct neighborTable* nodeLauncher5(double* data, unsigned dim,
+ *       unsigned numPoints, unsigned numRP, unsigned* pointArray, double epsilon)
+ *   (reference include/launcher.cuh:77-82, src/launcher.cu:3093-3532; called
+ *   once from main, src/main.cu:171-178).
+ *
+ * Plain C types only: no C++, HIP or torch types cross this boundary.  Every
+ * entry point returns MSJ_OK or a negative msj_status and never aborts;
+ * msj_last_error() gives the detail of the last failure on the calling thread.
+ *
+ * Data contract (as nodeLauncher5): host, row-major, n rows of d values,
+ * dimensions already reordered by decreasing standard deviation (main.cu:106-122,
+ * see msj_dim_order).  The library never writes caller memory except *out.
+ *
+ * Results: every ordered pair (p, q) with the reference's FP64 distance test
+ * ||p - q||^2 <= eps^2 (kernel.cu:833-865), including (p, p); rows and
+ * neighbour ids are the caller's row indices (0-based input order).
+ */
+#ifndef MSJ_H
+#define MSJ_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "msj_params.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MSJ_ABI_VERSION 1u
+#define MSJ_MAX_LAYERS 16
+
+typedef enum msj_status {
+    MSJ_OK = 0,
+    MSJ_E_INVAL = -1,       /* bad argument (n, d, eps, params)               */
+    MSJ_E_NOMEM = -2,       /* host or device allocation failed               */
+    MSJ_E_HIP = -3,         /* a HIP runtime call failed                      */
+    MSJ_E_OVERFLOW = -4,    /* a count exceeds what the ABI types can hold    */
+    MSJ_E_UNSUPPORTED = -5, /* valid request this build does not implement    */
+    MSJ_E_NODEV = -6        /* no usable HIP device                           */
+} msj_status;
+
+/* Storage precision of the input coordinates.  F32 inputs are widened
+ * exactly to FP64 for every decision. */
+typedef enum msj_storage { MSJ_STORAGE_F64 = 0, MSJ_STORAGE_F32 = 1 } msj_storage;
+
+/* Distance-and-compare form.  All forms return the same pair set. */
+typedef enum msj_form {
+    MSJ_FORM_PLAIN = 0,      /* FP64 chunked early-exit test on every candidate */
+    MSJ_FORM_MFMA_BF16 = 1,  /* BF16 MFMA Gram filter + FP64 band recheck       */
+    MSJ_FORM_MFMA_SPLIT = 2  /* split-BF16 MFMA Gram filter + FP64 band recheck */
+} msj_form;
+
+/* Where msj_ctx_join leaves the result arrays. */
+typedef enum msj_output {
+    MSJ_OUT_HOST = 0,   /* library-allocated host arrays in msj_result          */
+    MSJ_OUT_DEVICE = 1  /* arrays stay in HBM, owned by the context (see below) */
+} msj_output;
+
+typedef struct msj_params {
+    uint32_t struct_size;        /* sizeof(msj_params), set by msj_default_params   */
+    int32_t  device;             /* HIP device ordinal (params.cuh:40 CUDA_DEVICE)  */
+    uint32_t maxrp;              /* MAXRP: node-net layers (params.cuh:50)          */
+    uint32_t rp_per_layer;       /* RPPERLAYER, <= 32 (params.cuh:53)               */
+    uint32_t numidim;            /* NUMIDIM (params.cuh:61); used for j < min(.,d)  */
+    uint32_t seed;               /* glibc srand seed of the RP stream; 1 = unseeded */
+    double   sample_per;         /* SAMPLE_PER (params.cuh:69)                      */
+    uint64_t min_calcs_per_node; /* MIN_CALCS_PER_NODE (params.cuh:38)              */
+    uint32_t form;               /* msj_form                                        */
+    uint32_t storage;            /* msj_storage of the input                        */
+    uint32_t output;             /* msj_output                                      */
+    uint32_t verbose;            /* 1: per-layer progress on stdout                 */
+    uint32_t count_work;         /* 1: count FP64 dimensions evaluated (slower)     */
+    uint32_t reserved0;
+    uint64_t page_pairs;         /* result page capacity in pairs (0: automatic)    */
+} msj_params;
+
+typedef struct msj_stats {
+    uint32_t num_nodes;          /* final node count                                */
+    uint32_t num_layers;         /* layers built (numSplits)                        */
+    uint32_t num_candidates;     /* candidate partitions per layer                  */
+    uint32_t num_pages;          /* result pages produced                           */
+    uint64_t num_calcs;          /* candidate distance evaluations (nodes.cu:697)   */
+    uint64_t total_pairs;        /* "Total results Set Size" (launcher.cu:3513)     */
+    uint32_t chosen[MSJ_MAX_LAYERS];          /* selected candidate per layer       */
+    uint32_t nodes_per_layer[MSJ_MAX_LAYERS];
+    uint64_t calcs_per_layer[MSJ_MAX_LAYERS];
+    double   sse[MSJ_MAX_LAYERS];             /* selection statistic of the winner  */
+    uint32_t ref_num_batches;    /* batch count of the reference's plan (3203-3229) */
+    uint32_t pad0;
+    double   ref_var_calc;       /* varCalc of that plan (launcher.cu:3203)         */
+    /* device time, HIP events on the compute stream, milliseconds */
+    double   ms_index;           /* reference points, keys, node net, adjacency     */
+    double   ms_join;            /* distance-and-compare kernels                    */
+    double   ms_join_kernel;     /* the same, kernel launches only                  */
+    double   ms_output;          /* pair sort, counts, offsets                      */
+    double   ms_d2h;             /* result copy to host (MSJ_OUT_HOST)              */
+    double   ms_total;           /* host wall clock of msj_ctx_join                 */
+    uint64_t join_launches;      /* distance kernel launches (incl. retries)        */
+    uint64_t dims_evaluated;     /* count_work: sum over evaluated pairs of the
+                                    dimensions the early-exit test consumed       */
+    uint64_t band_pairs;         /* MFMA forms: pairs re-decided in FP64            */
+    double   max_score_err;      /* MFMA forms: max |filter score - FP64 score|     */
+    double   mean_score_err;     /* MFMA forms: mean of the same over scored pairs  */
+} msj_stats;
+
+typedef struct msj_result {
+    uint32_t  n;                 /* rows                                            */
+    uint32_t  pad0;
+    uint64_t  total_pairs;       /* R = sum(counts)                                 */
+    uint32_t *counts;            /* [n]    neighbours of each row, self included    */
+    uint64_t *offsets;           /* [n+1]  CSR offsets into nbrs                    */
+    uint32_t *nbrs;              /* [R]    neighbour row ids, ascending per row     */
+    uint32_t *node_order;        /* [n]    pointArray after the node-net build      */
+    uint32_t  on_device;         /* 1: the four arrays above are device pointers    */
+    uint32_t  pad1;
+    msj_stats stats;
+} msj_result;
+
+typedef struct msj_ctx msj_ctx;
+
+const char *msj_version(void);
+const char *msj_strerror(int status);
+const char *msj_last_error(void);
+void        msj_default_params(msj_params *p);
+
+/* One-shot join (replaces nodeLauncher5, launcher.cu:3093): upload, build the
+ * node net, join, return host arrays.  Free with msj_result_free. */
+int  msj_self_join(const void *data, uint32_t n, uint32_t d, double eps,
+                   const msj_params *p, msj_result *out);
+void msj_result_free(msj_result *r);
+
+/* Context API: keeps the data resident in HBM across joins. */
+int  msj_ctx_create(const msj_params *p, msj_ctx **ctx);
+void msj_ctx_destroy(msj_ctx *ctx);
+/* Uploads n x d row-major values of p->storage precision. */
+int  msj_ctx_load(msj_ctx *ctx, const void *data, uint32_t n, uint32_t d);
+/* Runs the whole join on the resident data.  With MSJ_OUT_DEVICE the result
+ * arrays are device pointers valid until the next join on, or the
+ * destruction of, this context; msj_result_free does not free them. */
+int  msj_ctx_join(msj_ctx *ctx, double eps, msj_result *out);
+
+/* utils.cu:22-53 stddev(): dimension order by decreasing population standard
+ * deviation (ties: higher index first), as main.cu:111 uses it. */
+int  msj_dim_order(const double *data, uint32_t n, uint32_t d, uint32_t *order, double *sigma);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,520 @@
+// msj_api.hip -- the C ABI (include/msj.h): argument checking, device
+// context, stage timing and result hand-over.
+//
+// Replaces nodeLauncher5 (reference launcher.cu:3093-3532) and the host helper
+// stddev (utils.cu:22-53) used by main (main.cu:111).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "msj_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace {
+
+thread_local std::string g_last_error;
+
+void set_error(const std::string &m) { g_last_error = m; }
+
+}  // namespace
+
+namespace msj {
+
+void hip_check(hipError_t e, const char *what, const char *file, int line)
+{
+    if (e == hipSuccess)
+        return;
+    (void)hipGetLastError();
+    int code = (e == hipErrorOutOfMemory) ? MSJ_E_NOMEM : MSJ_E_HIP;
+    throw Error(code, std::string(what) + " failed at " + file + ":" + std::to_string(line) + ": " +
+                          hipGetErrorString(e));
+}
+
+}  // namespace msj
+
+struct msj_ctx {
+    msj_params prm;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[8] = {};
+    uint32_t n = 0, d = 0;
+    bool f32 = false;
+    msj::DevBuf<unsigned char> staging;  // row-major upload
+    msj::DevBuf<unsigned char> xs;       // SoA, row order
+    msj::DevBuf<unsigned char> xn;       // SoA, node order
+    msj::Index ix;
+    msj::IndexBuilder builder;
+    msj::JoinRunner join;
+    msj::DevBuf<uint32_t> r_counts;
+    msj::DevBuf<uint64_t> r_offsets;
+    msj::DevBuf<uint32_t> r_nbrs;
+    uint64_t last_R = 0;
+};
+
+namespace {
+
+using msj::Error;
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev)
+    {
+        if (hipGetDevice(&prev) != hipSuccess)
+            prev = -1;
+        MSJ_HIP(hipSetDevice(dev));
+    }
+    ~DeviceGuard()
+    {
+        if (prev >= 0)
+            (void)hipSetDevice(prev);
+    }
+};
+
+void validate_params(const msj_params &p)
+{
+    if (p.struct_size != sizeof(msj_params))
+        throw Error(MSJ_E_INVAL, "msj_params.struct_size mismatch: call msj_default_params first");
+    if (p.maxrp == 0 || p.maxrp > MSJ_MAX_LAYERS)
+        throw Error(MSJ_E_INVAL, "maxrp must be in 1..16");
+    if (p.rp_per_layer == 0 || p.rp_per_layer > 32)
+        throw Error(MSJ_E_INVAL, "rp_per_layer must be in 1..32");
+    if (!(p.sample_per >= 0.0 && p.sample_per <= 1.0))
+        throw Error(MSJ_E_INVAL, "sample_per must be in [0, 1]");
+    if (p.storage != MSJ_STORAGE_F64 && p.storage != MSJ_STORAGE_F32)
+        throw Error(MSJ_E_INVAL, "unknown storage");
+    if (p.output != MSJ_OUT_HOST && p.output != MSJ_OUT_DEVICE)
+        throw Error(MSJ_E_INVAL, "unknown output mode");
+    if (p.form != MSJ_FORM_PLAIN && p.form != MSJ_FORM_MFMA_BF16 && p.form != MSJ_FORM_MFMA_SPLIT)
+        throw Error(MSJ_E_INVAL, "unknown form");
+    if (p.form != MSJ_FORM_PLAIN)
+        throw Error(MSJ_E_UNSUPPORTED, "MFMA filter forms are not built in this version");
+}
+
+template <class F>
+int guarded(F &&f)
+{
+    try {
+        f();
+        return MSJ_OK;
+    } catch (const Error &e) {
+        set_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        set_error("host allocation failed");
+        return MSJ_E_NOMEM;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return MSJ_E_HIP;
+    }
+}
+
+__global__ void k_soa(const unsigned char *__restrict__ rm, unsigned char *__restrict__ soa,
+                      uint32_t n, uint32_t d, uint32_t esz)
+{
+    size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)n * d)
+        return;
+    uint32_t j = (uint32_t)(idx / n), i = (uint32_t)(idx - (size_t)j * n);
+    if (esz == 8)
+        ((double *)soa)[idx] = ((const double *)rm)[(size_t)i * d + j];
+    else
+        ((float *)soa)[idx] = ((const float *)rm)[(size_t)i * d + j];
+}
+
+// launcher.cu:3203-3233: the reference's batch plan, kept for its stderr record.
+void reference_batch_plan(const std::vector<uint32_t> &nstart, const std::vector<uint64_t> &ncalcs,
+                          uint32_t n, double eps, uint64_t sum_calcs, uint32_t *batches,
+                          double *var_calc)
+{
+    double varCalc = sum_calcs * 1.0 / n / n;
+    unsigned batchCount = 0;
+    unsigned long long calcCounter = 0;
+    unsigned pointCounter = 0;
+    double factor1 = (1 + varCalc * CMP) * 2000 / std::sqrt((double)n);
+    double factor2 = CALC_PER_BATCH * std::pow(10, 10) * 1.0 * std::sqrt(std::pow(1 + eps, 20));
+    size_t M = ncalcs.size();
+    for (size_t i = 0; i < M; ++i) {
+        uint32_t sz = nstart[i + 1] - nstart[i];
+        for (uint32_t j = 0; j < sz; ++j) {
+            calcCounter += ncalcs[i] / sz;
+            pointCounter++;
+            if (calcCounter * factor1 >= factor2 || pointCounter >= KERNEL_BLOCKS * BLOCK_SIZE) {
+                batchCount++;
+                calcCounter = 0;
+                pointCounter = 0;
+            }
+        }
+    }
+    *batches = batchCount + 1;
+    *var_calc = varCalc;
+}
+
+void do_load(msj_ctx *c, const void *data, uint32_t n, uint32_t d)
+{
+    if (d == 0)
+        throw Error(MSJ_E_INVAL, "d must be positive");
+    if (n > 0 && !data)
+        throw Error(MSJ_E_INVAL, "data is NULL");
+    if ((uint64_t)n * d >= (1ull << 40))
+        throw Error(MSJ_E_OVERFLOW, "n*d too large");
+    DeviceGuard g(c->prm.device);
+    c->f32 = c->prm.storage == MSJ_STORAGE_F32;
+    const uint32_t esz = c->f32 ? 4 : 8;
+    const size_t bytes = (size_t)n * d * esz;
+    c->n = n;
+    c->d = d;
+    if (n == 0)
+        return;
+    c->staging.ensure(bytes);
+    c->xs.ensure(bytes);
+    MSJ_HIP(hipMemcpyAsync(c->staging.p, data, bytes, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_soa, dim3(msj::grid_for((size_t)n * d, 256)), dim3(256), 0, c->stream,
+                       c->staging.p, c->xs.p, n, d, esz);
+    MSJ_HIP(hipGetLastError());
+    MSJ_HIP(hipStreamSynchronize(c->stream));
+    c->staging.release();
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b)
+{
+    float ms = 0.f;
+    MSJ_HIP(hipEventElapsedTime(&ms, a, b));
+    return ms;
+}
+
+void do_join(msj_ctx *c, double eps, msj_result *out)
+{
+    if (!out)
+        throw Error(MSJ_E_INVAL, "out is NULL");
+    std::memset(out, 0, sizeof(*out));
+    if (!(eps > 0.0) || !std::isfinite(eps))
+        throw Error(MSJ_E_INVAL, "eps must be positive and finite");
+    if (c->d == 0)
+        throw Error(MSJ_E_INVAL, "no data loaded");
+    const auto t_start = std::chrono::steady_clock::now();
+    DeviceGuard g(c->prm.device);
+    const uint32_t n = c->n, d = c->d;
+    const bool dev_out = c->prm.output == MSJ_OUT_DEVICE;
+    hipStream_t s = c->stream;
+    msj_stats &st = out->stats;
+    out->n = n;
+    out->on_device = dev_out ? 1u : 0u;
+    if (n == 0) {
+        out->offsets = (uint64_t *)std::calloc(1, sizeof(uint64_t));
+        out->on_device = 0;
+        if (!out->offsets)
+            throw std::bad_alloc();
+        return;
+    }
+
+    // ---- index (subsystems a, b)
+    MSJ_HIP(hipEventRecord(c->ev[0], s));
+    msj::LayerStats ls;
+    c->builder.build(c->xs.p, c->f32, n, d, eps, c->prm, c->ix, ls, s);
+    const uint32_t esz = c->f32 ? 4 : 8;
+    c->xn.ensure((size_t)n * d * esz);
+    msj::gather_node_order(c->xs.p, c->xn.p, c->f32, c->ix.perm.p, n, d, s);
+    c->join.plan(c->ix, n, s);
+    MSJ_HIP(hipEventRecord(c->ev[1], s));
+
+    // ---- join (subsystem c)
+    uint64_t cap = c->prm.page_pairs;
+    if (cap == 0) {
+        size_t freeb = 0, totalb = 0;
+        MSJ_HIP(hipMemGetInfo(&freeb, &totalb));
+        uint64_t want = std::max<uint64_t>(std::max<uint64_t>(c->last_R, 128ull * n), 1ull << 22);
+        uint64_t budget = (uint64_t)(freeb / 4) / sizeof(uint64_t);
+        cap = std::max<uint64_t>(std::min(want, budget), 1);
+    }
+    float kms = 0.f;
+    uint64_t launches = 0, dims = 0;
+    const uint64_t R = c->join.run_plain(c->ix, c->xn.p, c->f32, n, d, eps, cap, s, c->ev[2],
+                                         c->ev[3], &launches, &kms,
+                                         c->prm.count_work ? &dims : nullptr);
+    st.dims_evaluated = dims;
+    c->last_R = R;
+    if (R >= 0xFFFFFFFFull * 16)
+        throw Error(MSJ_E_OVERFLOW, "result pair count too large");
+    MSJ_HIP(hipEventRecord(c->ev[4], s));
+
+    // ---- ordered results (subsystem d)
+    c->r_counts.ensure(n);
+    c->r_offsets.ensure((size_t)n + 1);
+    c->r_nbrs.ensure(std::max<uint64_t>(R, 1));
+    c->join.finish(n, R, c->r_counts.p, c->r_offsets.p, c->r_nbrs.p, s);
+    MSJ_HIP(hipEventRecord(c->ev[5], s));
+    MSJ_HIP(hipStreamSynchronize(s));
+
+    st.ms_index = elapsed(c->ev[0], c->ev[1]);
+    st.ms_join = elapsed(c->ev[1], c->ev[4]);
+    st.ms_join_kernel = kms;
+    st.ms_output = elapsed(c->ev[4], c->ev[5]);
+    st.join_launches = launches;
+    st.num_nodes = c->ix.M;
+    st.num_layers = ls.layers;
+    st.num_candidates = ls.ncand;
+    st.num_pages = 1;
+    st.num_calcs = c->ix.num_calcs;
+    st.total_pairs = R;
+    for (uint32_t l = 0; l < ls.layers; ++l) {
+        st.chosen[l] = ls.chosen[l];
+        st.nodes_per_layer[l] = ls.nodes[l];
+        st.calcs_per_layer[l] = ls.calcs[l];
+        st.sse[l] = ls.sse[l];
+    }
+    out->total_pairs = R;
+
+    // node sizes / numCalcs for the reference's batch record
+    {
+        std::vector<uint32_t> hstart((size_t)c->ix.M + 1);
+        std::vector<uint64_t> hcalcs(c->ix.M);
+        MSJ_HIP(hipMemcpy(hstart.data(), c->ix.nstart.p, hstart.size() * 4, hipMemcpyDeviceToHost));
+        MSJ_HIP(hipMemcpy(hcalcs.data(), c->ix.ncalcs.p, hcalcs.size() * 8, hipMemcpyDeviceToHost));
+        reference_batch_plan(hstart, hcalcs, n, eps, c->ix.num_calcs, &st.ref_num_batches,
+                             &st.ref_var_calc);
+    }
+
+    if (dev_out) {
+        out->counts = c->r_counts.p;
+        out->offsets = c->r_offsets.p;
+        out->nbrs = c->r_nbrs.p;
+        out->node_order = c->ix.perm.p;
+    } else {
+        MSJ_HIP(hipEventRecord(c->ev[6], s));
+        out->counts = (uint32_t *)std::malloc(sizeof(uint32_t) * n);
+        out->offsets = (uint64_t *)std::malloc(sizeof(uint64_t) * ((size_t)n + 1));
+        out->nbrs = (uint32_t *)std::malloc(sizeof(uint32_t) * std::max<uint64_t>(R, 1));
+        out->node_order = (uint32_t *)std::malloc(sizeof(uint32_t) * n);
+        if (!out->counts || !out->offsets || !out->nbrs || !out->node_order) {
+            msj_result_free(out);
+            throw std::bad_alloc();
+        }
+        MSJ_HIP(hipMemcpyAsync(out->counts, c->r_counts.p, sizeof(uint32_t) * n,
+                               hipMemcpyDeviceToHost, s));
+        MSJ_HIP(hipMemcpyAsync(out->offsets, c->r_offsets.p, sizeof(uint64_t) * ((size_t)n + 1),
+                               hipMemcpyDeviceToHost, s));
+        if (R)
+            MSJ_HIP(hipMemcpyAsync(out->nbrs, c->r_nbrs.p, sizeof(uint32_t) * R,
+                                   hipMemcpyDeviceToHost, s));
+        MSJ_HIP(hipMemcpyAsync(out->node_order, c->ix.perm.p, sizeof(uint32_t) * n,
+                               hipMemcpyDeviceToHost, s));
+        MSJ_HIP(hipEventRecord(c->ev[7], s));
+        MSJ_HIP(hipStreamSynchronize(s));
+        st.ms_d2h = elapsed(c->ev[6], c->ev[7]);
+    }
+    st.ms_total =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start)
+            .count();
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *msj_version(void) { return "msj 0.1.0 (gfx950, HIP)"; }
+
+const char *msj_strerror(int status)
+{
+    switch (status) {
+    case MSJ_OK: return "ok";
+    case MSJ_E_INVAL: return "invalid argument";
+    case MSJ_E_NOMEM: return "out of memory";
+    case MSJ_E_HIP: return "HIP runtime error";
+    case MSJ_E_OVERFLOW: return "count overflow";
+    case MSJ_E_UNSUPPORTED: return "unsupported";
+    case MSJ_E_NODEV: return "no HIP device";
+    default: return "unknown status";
+    }
+}
+
+const char *msj_last_error(void) { return g_last_error.c_str(); }
+
+void msj_default_params(msj_params *p)
+{
+    if (!p)
+        return;
+    std::memset(p, 0, sizeof(*p));
+    p->struct_size = sizeof(msj_params);
+    p->device = CUDA_DEVICE;
+    p->maxrp = MAXRP;
+    p->rp_per_layer = RPPERLAYER;
+    p->numidim = NUMIDIM;
+    p->seed = 1;
+    p->sample_per = SAMPLE_PER;
+    p->min_calcs_per_node = MIN_CALCS_PER_NODE;
+    p->form = MSJ_FORM_PLAIN;
+    p->storage = MSJ_STORAGE_F64;
+    p->output = MSJ_OUT_HOST;
+    p->verbose = 0;
+    p->count_work = 0;
+    p->page_pairs = 0;
+}
+
+int msj_ctx_create(const msj_params *p, msj_ctx **out)
+{
+    return guarded([&] {
+        if (!out)
+            throw Error(MSJ_E_INVAL, "ctx is NULL");
+        *out = nullptr;
+        msj_params prm;
+        if (p)
+            prm = *p;
+        else
+            msj_default_params(&prm);
+        validate_params(prm);
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+            (void)hipGetLastError();
+            throw Error(MSJ_E_NODEV, "no HIP device visible");
+        }
+        if (prm.device < 0 || prm.device >= ndev)
+            throw Error(MSJ_E_NODEV, "device ordinal out of range");
+        DeviceGuard g(prm.device);
+        msj_ctx *c = new msj_ctx();
+        c->prm = prm;
+        try {
+            MSJ_HIP(hipStreamCreate(&c->stream));
+            for (auto &e : c->ev)
+                MSJ_HIP(hipEventCreate(&e));
+        } catch (...) {
+            msj_ctx_destroy(c);
+            throw;
+        }
+        *out = c;
+    });
+}
+
+void msj_ctx_destroy(msj_ctx *c)
+{
+    if (!c)
+        return;
+    {
+        int prev = -1;
+        (void)hipGetDevice(&prev);
+        (void)hipSetDevice(c->prm.device);
+        if (c->stream)
+            (void)hipStreamSynchronize(c->stream);
+        for (auto &e : c->ev)
+            if (e)
+                (void)hipEventDestroy(e);
+        delete c;  // releases device buffers on this device
+        if (prev >= 0)
+            (void)hipSetDevice(prev);
+        return;
+    }
+}
+
+int msj_ctx_load(msj_ctx *c, const void *data, uint32_t n, uint32_t d)
+{
+    return guarded([&] {
+        if (!c)
+            throw Error(MSJ_E_INVAL, "ctx is NULL");
+        do_load(c, data, n, d);
+    });
+}
+
+int msj_ctx_join(msj_ctx *c, double eps, msj_result *out)
+{
+    return guarded([&] {
+        if (!c)
+            throw Error(MSJ_E_INVAL, "ctx is NULL");
+        do_join(c, eps, out);
+    });
+}
+
+int msj_self_join(const void *data, uint32_t n, uint32_t d, double eps, const msj_params *p,
+                  msj_result *out)
+{
+    msj_params prm;
+    if (p)
+        prm = *p;
+    else
+        msj_default_params(&prm);
+    prm.output = MSJ_OUT_HOST;  // the context dies with this call
+    if (out)
+        std::memset(out, 0, sizeof(*out));
+    msj_ctx *c = nullptr;
+    int rc = msj_ctx_create(&prm, &c);
+    if (rc != MSJ_OK)
+        return rc;
+    rc = msj_ctx_load(c, data, n, d);
+    if (rc == MSJ_OK)
+        rc = msj_ctx_join(c, eps, out);
+    msj_ctx_destroy(c);
+    return rc;
+}
+
+void msj_result_free(msj_result *r)
+{
+    if (!r)
+        return;
+    if (!r->on_device) {
+        std::free(r->counts);
+        std::free(r->offsets);
+        std::free(r->nbrs);
+        std::free(r->node_order);
+    }
+    r->counts = nullptr;
+    r->offsets = nullptr;
+    r->nbrs = nullptr;
+    r->node_order = nullptr;
+    r->total_pairs = 0;
+}
+
+// utils.cu:22-53 (host; sequential sums as the reference computes them)
+int msj_dim_order(const double *A, uint32_t n, uint32_t d, uint32_t *order, double *sigma)
+{
+    return guarded([&] {
+        if (!A || !order || d == 0)
+            throw Error(MSJ_E_INVAL, "bad arguments to msj_dim_order");
+        std::vector<double> dev(d);
+        auto column = [&](uint32_t i) {
+            double mean = 0.0;
+            for (uint32_t j = 0; j < n; ++j)
+                mean = mean + A[(size_t)d * j + i];
+            mean = mean / n;
+            double devmean = 0.0;
+            for (uint32_t j = 0; j < n; ++j) {
+                double t = A[(size_t)d * j + i] - mean;
+                double sq = t * t;
+                devmean = devmean + sq;
+            }
+            devmean = devmean / n;
+            dev[i] = std::sqrt(devmean);
+        };
+        // columns are independent; each column's sums stay sequential
+        unsigned nt = std::max(1u, std::min<unsigned>(std::thread::hardware_concurrency(), d));
+        std::vector<std::thread> pool;
+        for (unsigned t = 0; t < nt; ++t)
+            pool.emplace_back([&, t] {
+                for (uint32_t i = t; i < d; i += nt)
+                    column(i);
+            });
+        for (auto &th : pool)
+            th.join();
+        auto rkey = [](double x) {
+            uint64_t u;
+            std::memcpy(&u, &x, 8);
+            return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+        };
+        std::vector<uint32_t> asc(d);
+        for (uint32_t i = 0; i < d; ++i)
+            asc[i] = i;
+        std::stable_sort(asc.begin(), asc.end(),
+                         [&](uint32_t a, uint32_t b) { return rkey(dev[a]) < rkey(dev[b]); });
+        for (uint32_t i = 0; i < d; ++i) {
+            order[i] = asc[d - 1 - i];
+            if (sigma)
+                sigma[i] = dev[asc[d - 1 - i]];
+        }
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,686 @@
+// msj_index.hip -- node-net construction on the GPU (subsystems a and b).
+//
+// Reference: nodes.cu:5-356 buildNodeNet, 358-449 initNodes, 452-633
+// splitNodes, 677-695 updateNodeCalcs, 719-747 updateNeighbors;
+// utils.cu:22-137 stddev/createRPArray; kernel.cu:880-909 binningKernel.
+//
+// Per layer, everything except the 64*d glibc rand() draws and the 64-entry
+// sigma sort runs on the device:
+//   1. sample distances to 64 random candidates and their population sigma
+//      (the host FP64 operation order: no fused multiply-add), max corner;
+//   2. one kernel computes all 32 reference-point keys and min(6,d) axis keys
+//      per point (FP64, fma accumulation as nvcc contracts binningKernel);
+//   3. the 38 candidate partitions are evaluated in ONE radix sort of
+//      (candidate, parent node, key) composites; run lengths in sorted order
+//      are the would-be node sizes in the reference's node order, and the
+//      sum of squared deviations is accumulated sequentially per candidate,
+//      in that order, so it is bit-identical to nodes.cu:194-199;
+//   4. the winner is applied with a stable (parent, key) sort carrying the
+//      point ids (points stay ascending within a node, as thrust's stable
+//      sorts leave them);
+//   5. adjacency of the new layer comes from the parent layer's adjacency:
+//      for each child and each parent neighbour Q, the children of Q whose
+//      key is within +-1 (binary search in Q's sorted child keys), or all of
+//      them when either side stopped splitting.  Count pass, prefix scan,
+//      write pass; the neighbour lists come out in the reference's order.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "msj_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace msj {
+
+// ------------------------------------------------------- glibc TYPE_3 rand
+void GlibcRand::seed(uint32_t s)
+{
+    if (s == 0)
+        s = 1;
+    int32_t word = (int32_t)s;
+    r[0] = word;
+    for (int i = 1; i < 31; i++) {
+        long hi = word / 127773;
+        long lo = word % 127773;
+        long w = 16807 * lo - 2836 * hi;
+        if (w < 0)
+            w += 2147483647;
+        word = (int32_t)w;
+        r[i] = word;
+    }
+    f = 3;
+    b = 0;
+    for (int i = 0; i < 310; i++)
+        (void)next();
+}
+
+int32_t GlibcRand::next()
+{
+    uint32_t val = (uint32_t)r[f] + (uint32_t)r[b];
+    r[f] = (int32_t)val;
+    int32_t res = (int32_t)(val >> 1);
+    if (++f >= 31) {
+        f = 0;
+        ++b;
+    } else if (++b >= 31) {
+        b = 0;
+    }
+    return res;
+}
+
+namespace {
+
+// x86-64 host conversions used by the reference's host code.
+inline uint32_t host_cvt_u32(double q)  // (unsigned)q: cvttsd2si 64-bit, low half
+{
+    if (!(q > -9223372036854775808.0 && q < 9223372036854775808.0))
+        return 0u;
+    return (uint32_t)(uint64_t)(int64_t)q;
+}
+inline int32_t host_cvt_i32(double q)  // (int)q: cvttsd2si 32-bit
+{
+    if (!(q > -2147483649.0 && q < 2147483648.0))
+        return (int32_t)0x80000000u;
+    return (int32_t)q;
+}
+inline uint64_t radix_key_f64(double x)
+{
+    uint64_t u;
+    std::memcpy(&u, &x, 8);
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+
+// Device twins of the conversions.
+__device__ inline uint32_t dev_cvt_u32_sat(double q)  // cvt.rzi.u32.f64
+{
+    if (!(q > 0.0))
+        return 0u;
+    if (q >= 4294967295.0)
+        return 0xFFFFFFFFu;
+    return (uint32_t)q;
+}
+__device__ inline uint32_t dev_cvt_u32_x86(double q)
+{
+    if (!(q > -9223372036854775808.0 && q < 9223372036854775808.0))
+        return 0u;
+    return (uint32_t)(uint64_t)(int64_t)q;
+}
+
+constexpr int KEY_RP = 32;  // register accumulators of the key kernel
+
+__global__ __launch_bounds__(256) void k_iota(uint32_t *p, uint32_t n)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = i;
+}
+
+template <class T>
+__global__ void k_fill(T *p, size_t n, T v)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = v;
+}
+
+// utils.cu:84-89: distmat[i*test_rp + j] = sqrt(sum (rp - x)^2), host order.
+template <class T>
+__global__ __launch_bounds__(256) void k_rp_distmat(const T *__restrict__ xs, uint32_t n,
+                                                    uint32_t d, uint32_t S,
+                                                    const double *__restrict__ trp,
+                                                    uint32_t ntest, double *__restrict__ distmat)
+{
+    size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)S * ntest)
+        return;
+    uint32_t i = (uint32_t)(idx / ntest), j = (uint32_t)(idx % ntest);
+    double dist = 0;
+    for (uint32_t k = 0; k < d; ++k) {
+        double diff = trp[(size_t)j * d + k] - (double)xs[(size_t)k * n + i];
+        double sq = diff * diff;
+        dist = dist + sq;
+    }
+    distmat[idx] = sqrt(dist);
+}
+
+// utils.cu:29-41 per column: sequential mean, then sequential sum of squares.
+__global__ void k_col_sigma(const double *__restrict__ A, uint32_t rows, uint32_t cols,
+                            double *__restrict__ sigma)
+{
+    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cols)
+        return;
+    double mean = 0.0;
+    for (uint32_t r = 0; r < rows; ++r)
+        mean = mean + A[(size_t)r * cols + c];
+    mean = mean / (double)rows;
+    double dev = 0.0;
+    for (uint32_t r = 0; r < rows; ++r) {
+        double t = A[(size_t)r * cols + c] - mean;
+        double sq = t * t;
+        dev = dev + sq;
+    }
+    dev = dev / (double)rows;
+    sigma[c] = sqrt(dev);
+}
+
+// utils.cu:105-118: max corner over the sample rows, starting from 0.
+template <class T>
+__global__ void k_col_max(const T *__restrict__ xs, uint32_t n, uint32_t d, uint32_t S,
+                          double *__restrict__ maxc)
+{
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= d)
+        return;
+    double m = 0;
+    for (uint32_t i = 0; i < S; ++i) {
+        double v = (double)xs[(size_t)j * n + i];
+        if (m < v)
+            m = v;
+    }
+    maxc[j] = m;
+}
+
+// kernel.cu:880-909 (32 RP keys) + nodes.cu:128-135 (axis keys).  One thread
+// per point reads its column once; RP coordinates are wave-uniform (scalar
+// loads of rpT[j][0..31]).
+template <class T>
+__global__ __launch_bounds__(256) void k_keys(const T *__restrict__ xs, uint32_t n, uint32_t d,
+                                              const double *__restrict__ rpT, uint32_t nrp,
+                                              uint32_t naxis, double eps,
+                                              uint32_t *__restrict__ keys)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    double acc[KEY_RP];
+#pragma unroll
+    for (int t = 0; t < KEY_RP; ++t)
+        acc[t] = 0.0;
+    for (uint32_t j = 0; j < d; ++j) {
+        const double x = (double)xs[(size_t)j * n + i];
+        const double *r = rpT + (size_t)j * KEY_RP;
+#pragma unroll
+        for (int t = 0; t < KEY_RP; ++t) {
+            double df = x - r[t];
+            acc[t] = fma(df, df, acc[t]);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < KEY_RP; ++t)
+        if ((uint32_t)t < nrp)
+            keys[(size_t)t * n + i] = dev_cvt_u32_sat(floor(sqrt(acc[t]) / eps));
+    for (uint32_t a = 0; a < naxis; ++a) {
+        double x = (double)xs[(size_t)a * n + i];
+        double q = ceil((x + 0.0000001) / eps);
+        keys[(size_t)(nrp + a) * n + i] = dev_cvt_u32_x86(q);
+    }
+}
+
+// Composite (candidate | parent node | key) for candidates [t0, t0+tcount).
+__global__ __launch_bounds__(256) void k_cand_keys(const uint32_t *__restrict__ keys,
+                                                   const uint32_t *__restrict__ perm,
+                                                   const uint32_t *__restrict__ pnode,
+                                                   const uint8_t *__restrict__ nsplit, uint32_t n,
+                                                   uint32_t t0, uint32_t tcount, int kb, int nb,
+                                                   uint64_t *__restrict__ ck)
+{
+    size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)tcount * n)
+        return;
+    uint32_t t = (uint32_t)(idx / n), i = (uint32_t)(idx - (size_t)t * n);
+    uint32_t node = pnode[i];
+    uint64_t k = nsplit[node] ? keys[(size_t)(t0 + t) * n + perm[i]] : 0u;
+    ck[idx] = ((uint64_t)t << (nb + kb)) | ((uint64_t)node << kb) | k;
+}
+
+__global__ __launch_bounds__(256) void k_run_flags(const uint64_t *__restrict__ ck, size_t m,
+                                                   uint32_t *__restrict__ flags)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m)
+        flags[i] = (i == 0 || ck[i] != ck[i - 1]) ? 1u : 0u;
+}
+
+// starts[r] = first position of run r; starts[#runs] = m.
+__global__ __launch_bounds__(256) void k_run_starts(const uint32_t *__restrict__ flags,
+                                                    const uint32_t *__restrict__ runidx, size_t m,
+                                                    uint32_t *__restrict__ starts)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m && flags[i])
+        starts[runidx[i]] = (uint32_t)i;
+    if (i == 0)
+        starts[runidx[m]] = (uint32_t)m;
+}
+
+__global__ void k_runfirst(const uint32_t *__restrict__ runidx, uint32_t n, uint32_t tcount,
+                           uint32_t *__restrict__ runfirst)
+{
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t <= tcount)
+        runfirst[t] = runidx[(size_t)t * n];
+}
+
+// nodes.cu:194-199: sum over nodes, in node order, of (size - N/#nodes)^2.
+__global__ void k_cand_sse(const uint32_t *__restrict__ runfirst,
+                           const uint32_t *__restrict__ starts, uint32_t tcount, uint32_t n,
+                           double *__restrict__ sse, uint32_t *__restrict__ cnt)
+{
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= tcount)
+        return;
+    uint32_t r0 = runfirst[t], r1 = runfirst[t + 1];
+    double avg = (double)n * 1.0 / (double)(r1 - r0);
+    double s = 0;
+    for (uint32_t r = r0; r < r1; ++r) {
+        double df = (double)(starts[r + 1] - starts[r]) - avg;
+        double sq = df * df;
+        s = s + sq;
+    }
+    sse[t] = s;
+    cnt[t] = r1 - r0;
+}
+
+__global__ __launch_bounds__(256) void k_apply_keys(const uint32_t *__restrict__ key_t,
+                                                    const uint32_t *__restrict__ perm,
+                                                    const uint32_t *__restrict__ pnode,
+                                                    const uint8_t *__restrict__ nsplit, uint32_t n,
+                                                    int kb, uint64_t *__restrict__ ck,
+                                                    uint32_t *__restrict__ vals)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    uint32_t node = pnode[i];
+    uint32_t row = perm[i];
+    uint64_t k = nsplit[node] ? key_t[row] : 0u;
+    ck[i] = ((uint64_t)node << kb) | k;
+    vals[i] = row;
+}
+
+// Children of the new layer: start, parent, key (UINT_MAX when the parent
+// stopped splitting, as splitNodes stores it, nodes.cu:482).
+__global__ __launch_bounds__(256) void k_new_nodes(const uint64_t *__restrict__ ck,
+                                                   const uint32_t *__restrict__ flags,
+                                                   const uint32_t *__restrict__ runidx, uint32_t n,
+                                                   int kb, const uint8_t *__restrict__ nsplit,
+                                                   uint32_t *__restrict__ cstart,
+                                                   uint32_t *__restrict__ cparent,
+                                                   uint32_t *__restrict__ ckey,
+                                                   uint32_t *__restrict__ pnode_new)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    uint32_t c = runidx[i] + flags[i] - 1u;
+    pnode_new[i] = c;
+    if (flags[i]) {
+        uint64_t v = ck[i];
+        uint32_t parent = (uint32_t)(v >> kb);
+        uint64_t kmask = kb ? ((kb >= 64) ? ~0ull : ((1ull << kb) - 1)) : 0ull;
+        cstart[c] = i;
+        cparent[c] = parent;
+        ckey[c] = nsplit[parent] ? (uint32_t)(v & kmask) : 0xFFFFFFFFu;
+    }
+    if (i == 0)
+        cstart[runidx[n]] = n;
+}
+
+__global__ __launch_bounds__(256) void k_cfirst(const uint32_t *__restrict__ cparent, uint32_t mnew,
+                                                uint32_t mold, uint32_t *__restrict__ cfirst)
+{
+    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < mnew && (c == 0 || cparent[c] != cparent[c - 1]))
+        cfirst[cparent[c]] = c;
+    if (c == 0)
+        cfirst[mold] = mnew;
+}
+
+struct AdjView {
+    const uint32_t *cparent, *ckey, *cstart, *cfirst;
+    const uint8_t *osplit;
+    const uint32_t *oaoff, *oadj;
+};
+
+__device__ inline uint32_t lower_bound_u32(const uint32_t *a, uint32_t lo, uint32_t hi, uint32_t v)
+{
+    while (lo < hi) {
+        uint32_t mid = lo + (hi - lo) / 2;
+        if (a[mid] < v)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Neighbours of child c in the reference's order (nodes.cu:557-578 siblings,
+// then nodes.cu:719-747 for every other parent neighbour, in list order).
+template <class F>
+__device__ inline void visit_nbrs(const AdjView &A, uint32_t c, F &&emit)
+{
+    const uint32_t P = A.cparent[c];
+    const uint32_t k = A.ckey[c];
+    const bool s = A.osplit[P] != 0;
+    const uint32_t pf = A.cfirst[P], pl = A.cfirst[P + 1];
+    for (uint32_t a = A.oaoff[P]; a < A.oaoff[P + 1]; ++a) {
+        const uint32_t Q = A.oadj[a];
+        if (Q == P) {
+            emit(c);
+            if (s) {
+                if (c > pf && A.ckey[c - 1] + 1u == k)
+                    emit(c - 1);
+                if (c + 1 < pl && A.ckey[c + 1] - 1u == k)
+                    emit(c + 1);
+            }
+            continue;
+        }
+        const uint32_t q0 = A.cfirst[Q], q1 = A.cfirst[Q + 1];
+        if (!s || !A.osplit[Q]) {
+            for (uint32_t v = q0; v < q1; ++v)
+                emit(v);
+            continue;
+        }
+        // keys k-1, k, k+1 modulo 2^32; Q's child keys ascend
+        uint32_t hit[3];
+        int nh = 0;
+        const uint32_t want[3] = {k - 1u, k, k + 1u};
+        for (int w = 0; w < 3; ++w) {
+            uint32_t pos = lower_bound_u32(A.ckey, q0, q1, want[w]);
+            if (pos < q1 && A.ckey[pos] == want[w])
+                hit[nh++] = pos;
+        }
+        for (int x = 1; x < nh; ++x)  // emit in child order
+            for (int y = x; y > 0 && hit[y - 1] > hit[y]; --y) {
+                uint32_t tmp = hit[y];
+                hit[y] = hit[y - 1];
+                hit[y - 1] = tmp;
+            }
+        for (int x = 0; x < nh; ++x)
+            emit(hit[x]);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_adj_count(AdjView A, uint32_t mnew,
+                                                   uint32_t *__restrict__ deg,
+                                                   uint64_t *__restrict__ sumsz,
+                                                   uint64_t *__restrict__ ncalcs)
+{
+    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= mnew)
+        return;
+    uint32_t cnt = 0;
+    uint64_t sz = 0;
+    visit_nbrs(A, c, [&](uint32_t v) {
+        ++cnt;
+        sz += A.cstart[v + 1] - A.cstart[v];
+    });
+    deg[c] = cnt;
+    sumsz[c] = sz;
+    ncalcs[c] = sz * (uint64_t)(A.cstart[c + 1] - A.cstart[c]);  // nodes.cu:691
+}
+
+__global__ __launch_bounds__(256) void k_adj_write(AdjView A, uint32_t mnew,
+                                                   const uint32_t *__restrict__ aoff,
+                                                   uint32_t *__restrict__ adj)
+{
+    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= mnew)
+        return;
+    uint32_t w = aoff[c];
+    visit_nbrs(A, c, [&](uint32_t v) { adj[w++] = v; });
+}
+
+__global__ __launch_bounds__(256) void k_nsplit(const uint64_t *__restrict__ ncalcs, uint32_t m,
+                                                uint64_t minc, uint8_t *__restrict__ nsplit)
+{
+    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < m)
+        nsplit[c] = ncalcs[c] >= minc ? 1 : 0;  // nodes.cu:476
+}
+
+__global__ void k_root(uint8_t *nsplit, uint32_t *aoff, uint32_t *adj)
+{
+    if (threadIdx.x == 0) {
+        nsplit[0] = 1;
+        aoff[0] = 0;
+        aoff[1] = 1;
+        adj[0] = 0;
+    }
+}
+
+template <class T>
+void swap_buf(DevBuf<T> &a, DevBuf<T> &b)
+{
+    std::swap(a.p, b.p);
+    std::swap(a.cap, b.cap);
+}
+
+template <class T>
+T d2h_scalar(const T *dptr, hipStream_t s)
+{
+    T v;
+    MSJ_HIP(hipMemcpyAsync(&v, dptr, sizeof(T), hipMemcpyDeviceToHost, s));
+    MSJ_HIP(hipStreamSynchronize(s));
+    return v;
+}
+
+#define LAUNCH(kern, grid, block, ...)                                                   \
+    do {                                                                                 \
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, s, __VA_ARGS__);            \
+        MSJ_HIP(hipGetLastError());                                                      \
+    } while (0)
+
+template <class T>
+void build_impl(IndexBuilder &B, const T *xs, uint32_t n, uint32_t d, double eps,
+                const msj_params &p, Index &out, LayerStats &st, hipStream_t s)
+{
+    const uint32_t nrp = p.rp_per_layer;
+    const uint32_t naxis = std::min(p.numidim, d);
+    const uint32_t ncand = nrp + naxis;
+    const uint32_t layers = std::min<uint32_t>(p.maxrp, MSJ_MAX_LAYERS);
+    const uint32_t ntest = nrp * 2;
+    // (unsigned)(numPoints * SAMPLE_PER), utils.cu:68
+    const uint32_t S = host_cvt_u32((double)(unsigned long long)n * p.sample_per);
+    st.ncand = ncand;
+    st.layers = layers;
+
+    GlibcRand rng;
+    rng.seed(p.seed);
+
+    B.keys.ensure((size_t)ncand * n);
+    B.test_rp.ensure((size_t)ntest * d);
+    B.distmat.ensure((size_t)std::max<uint32_t>(S, 1) * ntest);
+    B.sigma.ensure(ntest);
+    B.maxc.ensure(d);
+    B.rpT.ensure((size_t)d * KEY_RP);
+    B.u32scalar.ensure(1);
+    B.u64scalar.ensure(1);
+    out.perm.ensure(n);
+    B.vals.ensure(n);
+    B.pnode.ensure(n);
+    B.pnode2.ensure(n);
+    B.nsplit.ensure(1);
+    B.aoff1.ensure(2);
+    B.adj1.ensure(1);
+
+    LAUNCH(k_iota, grid_for(n, 256), 256, out.perm.p, n);
+    LAUNCH(k_fill<uint32_t>, grid_for(n, 256), 256, B.pnode.p, (size_t)n, 0u);
+    LAUNCH(k_root, 1, 64, B.nsplit.p, B.aoff1.p, B.adj1.p);
+    uint32_t mold = 1;
+
+    std::vector<double> h_trp((size_t)ntest * d), h_sig(ntest), h_max(d), h_rp((size_t)nrp * d),
+        h_rpT((size_t)d * KEY_RP), h_sse;
+    std::vector<uint32_t> order(ntest);
+
+    for (uint32_t l = 0; l < layers; ++l) {
+        // ---- (1) reference points, utils.cu:66-137
+        for (size_t i = 0; i < h_trp.size(); ++i)
+            h_trp[i] = (double)rng.next() / (double)2147483647;
+        MSJ_HIP(hipMemcpyAsync(B.test_rp.p, h_trp.data(), h_trp.size() * sizeof(double),
+                               hipMemcpyHostToDevice, s));
+        if (S > 0)
+            LAUNCH(k_rp_distmat<T>, grid_for((size_t)S * ntest, 256), 256, xs, n, d, S,
+                   B.test_rp.p, ntest, B.distmat.p);
+        LAUNCH(k_col_sigma, 1, 64 * ((ntest + 63) / 64), B.distmat.p, S, ntest, B.sigma.p);
+        LAUNCH(k_col_max<T>, grid_for(d, 64), 64, xs, n, d, S, B.maxc.p);
+        MSJ_HIP(hipMemcpyAsync(h_sig.data(), B.sigma.p, ntest * sizeof(double),
+                               hipMemcpyDeviceToHost, s));
+        MSJ_HIP(hipMemcpyAsync(h_max.data(), B.maxc.p, d * sizeof(double), hipMemcpyDeviceToHost,
+                               s));
+        MSJ_HIP(hipStreamSynchronize(s));
+        // thrust::sort_by_key (stable, IEEE radix order) ascending, reversed
+        for (uint32_t i = 0; i < ntest; ++i)
+            order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+            return radix_key_f64(h_sig[a]) < radix_key_f64(h_sig[b]);
+        });
+        std::reverse(order.begin(), order.end());
+        for (uint32_t i = 0; i < nrp / 2; ++i)
+            for (uint32_t j = 0; j < d; ++j)
+                h_rp[(size_t)i * d + j] = h_trp[(size_t)order[i] * d + j];
+        // box points: max corner, then corner patterns (utils.cu:104-136)
+        for (uint32_t j = 0; j < d; ++j)
+            h_trp[j] = h_max[j];
+        const uint32_t step = host_cvt_u32(std::pow(2.0, (double)d) / ntest);
+        for (uint32_t i = 1; i < ntest; ++i)
+            for (uint32_t j = 0; j < d; ++j) {
+                uint32_t div = (uint32_t)host_cvt_i32(std::pow(2.0, (double)j));
+                uint32_t bit = ((i * step) / div) % 2u;
+                h_trp[(size_t)i * d + j] = (double)bit * h_trp[j];
+            }
+        for (uint32_t i = 0; i < nrp / 2 + nrp % 2; ++i)
+            for (uint32_t j = 0; j < d; ++j)
+                h_rp[(size_t)(nrp / 2 + i) * d + j] = h_trp[(size_t)i * d + j];
+        std::fill(h_rpT.begin(), h_rpT.end(), 0.0);
+        for (uint32_t t = 0; t < nrp; ++t)
+            for (uint32_t j = 0; j < d; ++j)
+                h_rpT[(size_t)j * KEY_RP + t] = h_rp[(size_t)t * d + j];
+        MSJ_HIP(hipMemcpyAsync(B.rpT.p, h_rpT.data(), h_rpT.size() * sizeof(double),
+                               hipMemcpyHostToDevice, s));
+
+        // ---- (2) keys of every candidate partition
+        LAUNCH(k_keys<T>, grid_for(n, 256), 256, xs, n, d, B.rpT.p, nrp, naxis, eps, B.keys.p);
+        reduce_max_u32(B.keys.p, (size_t)ncand * n, B.u32scalar.p, s);
+        const uint32_t kmax = d2h_scalar(B.u32scalar.p, s);
+        const int kb = bits_for(kmax);
+        const int nb = bits_for(mold - 1);
+
+        // ---- (3) candidate node sizes and SSE
+        int tb_all = bits_for(ncand - 1);
+        uint32_t group = ncand;
+        if (tb_all + nb + kb > 64) {
+            int room = 64 - nb - kb;  // >= 0 since nb, kb <= 32
+            group = room >= 31 ? ncand : std::max<uint32_t>(1u, std::min<uint32_t>(ncand, 1u << room));
+        }
+        h_sse.assign(ncand, 0.0);
+        B.sse.ensure(ncand);
+        B.cnt_nodes.ensure(ncand);
+        B.runfirst.ensure((size_t)ncand + 1);
+        for (uint32_t t0 = 0; t0 < ncand; t0 += group) {
+            const uint32_t tc = std::min(group, ncand - t0);
+            const size_t m = (size_t)tc * n;
+            const int tb = bits_for(tc - 1);
+            B.ck.ensure(m);
+            B.flags.ensure(m);
+            B.runidx.ensure(m + 1);
+            B.starts.ensure(m + 1);
+            LAUNCH(k_cand_keys, grid_for(m, 256), 256, B.keys.p, out.perm.p, B.pnode.p,
+                   B.nsplit.p, n, t0, tc, kb, nb, B.ck.p);
+            B.sorter.sort(B.ck.p, nullptr, m, tb + nb + kb, s);
+            LAUNCH(k_run_flags, grid_for(m, 256), 256, B.ck.p, m, B.flags.p);
+            B.scan.excl_u32_to_u32(B.flags.p, B.runidx.p, m, s);
+            LAUNCH(k_run_starts, grid_for(m, 256), 256, B.flags.p, B.runidx.p, m, B.starts.p);
+            LAUNCH(k_runfirst, 1, 64 * ((tc + 1 + 63) / 64), B.runidx.p, n, tc, B.runfirst.p);
+            LAUNCH(k_cand_sse, 1, 64 * ((tc + 63) / 64), B.runfirst.p, B.starts.p, tc, n,
+                   B.sse.p + t0, B.cnt_nodes.p + t0);
+        }
+        MSJ_HIP(hipMemcpyAsync(h_sse.data(), B.sse.p, ncand * sizeof(double),
+                               hipMemcpyDeviceToHost, s));
+        MSJ_HIP(hipStreamSynchronize(s));
+        uint32_t best = 0;  // strict '<': the lowest index wins a tie
+        for (uint32_t t = 1; t < ncand; ++t)
+            if (h_sse[t] < h_sse[best])
+                best = t;
+
+        // ---- (4) apply the winner
+        B.ck.ensure(n);
+        B.flags.ensure(n);
+        B.runidx.ensure((size_t)n + 1);
+        LAUNCH(k_apply_keys, grid_for(n, 256), 256, B.keys.p + (size_t)best * n, out.perm.p,
+               B.pnode.p, B.nsplit.p, n, kb, B.ck.p, B.vals.p);
+        B.sorter.sort(B.ck.p, B.vals.p, n, nb + kb, s);
+        LAUNCH(k_run_flags, grid_for(n, 256), 256, B.ck.p, (size_t)n, B.flags.p);
+        B.scan.excl_u32_to_u32(B.flags.p, B.runidx.p, n, s);
+        const uint32_t mnew = d2h_scalar(B.runidx.p + n, s);
+        B.cstart.ensure((size_t)mnew + 1);
+        B.cparent.ensure(mnew);
+        B.ckey.ensure(mnew);
+        B.cfirst.ensure((size_t)mold + 1);
+        LAUNCH(k_new_nodes, grid_for(n, 256), 256, B.ck.p, B.flags.p, B.runidx.p, n, kb,
+               B.nsplit.p, B.cstart.p, B.cparent.p, B.ckey.p, B.pnode2.p);
+        LAUNCH(k_cfirst, grid_for(mnew, 256), 256, B.cparent.p, mnew, mold, B.cfirst.p);
+
+        // ---- (5) adjacency of the new layer
+        B.deg.ensure(mnew);
+        B.sumsz2.ensure(mnew);
+        B.ncalcs2.ensure(mnew);
+        B.aoff2.ensure((size_t)mnew + 1);
+        AdjView A{B.cparent.p, B.ckey.p, B.cstart.p, B.cfirst.p, B.nsplit.p, B.aoff1.p, B.adj1.p};
+        LAUNCH(k_adj_count, grid_for(mnew, 256), 256, A, mnew, B.deg.p, B.sumsz2.p, B.ncalcs2.p);
+        {
+            // degree prefix in 64 bits first, to detect > 2^32 adjacency entries
+            B.tmp64.ensure((size_t)mnew + 1);
+            B.scan.excl_u32_to_u64(B.deg.p, B.tmp64.p, mnew, s);
+            const uint64_t edges = d2h_scalar(B.tmp64.p + mnew, s);
+            if (edges >= 0xFFFFFFFFull)
+                throw Error(MSJ_E_OVERFLOW, "node adjacency exceeds 2^32 entries");
+            B.scan.excl_u32_to_u32(B.deg.p, B.aoff2.p, mnew, s);
+            B.adj2.ensure(std::max<uint64_t>(edges, 1));
+            LAUNCH(k_adj_write, grid_for(mnew, 256), 256, A, mnew, B.aoff2.p, B.adj2.p);
+            out.edges = (uint32_t)edges;
+        }
+        B.nsplit2.ensure(mnew);
+        LAUNCH(k_nsplit, grid_for(mnew, 256), 256, B.ncalcs2.p, mnew, p.min_calcs_per_node,
+               B.nsplit2.p);
+        reduce_sum_u64(B.ncalcs2.p, mnew, B.u64scalar.p, s);
+        const uint64_t calcs = d2h_scalar(B.u64scalar.p, s);
+
+        swap_buf(out.perm, B.vals);
+        swap_buf(B.pnode, B.pnode2);
+        swap_buf(B.nsplit, B.nsplit2);
+        swap_buf(B.aoff1, B.aoff2);
+        swap_buf(B.adj1, B.adj2);
+        mold = mnew;
+        st.chosen[l] = best;
+        st.sse[l] = h_sse[best];
+        st.nodes[l] = mnew;
+        st.calcs[l] = calcs;
+        out.num_calcs = calcs;
+        if (p.verbose)
+            printf("Layer %u Selecting RP %u with Nodes: %u and calcs: %llu and stdev %f\n", l,
+                   best, mnew, (unsigned long long)calcs, h_sse[best]);
+    }
+    out.M = mold;
+    swap_buf(out.nstart, B.cstart);
+    swap_buf(out.aoff, B.aoff1);
+    swap_buf(out.adj, B.adj1);
+    swap_buf(out.sumsz, B.sumsz2);
+    swap_buf(out.ncalcs, B.ncalcs2);
+}
+
+}  // namespace
+
+void IndexBuilder::build(const void *xs, bool f32, uint32_t n, uint32_t d, double eps,
+                         const msj_params &p, Index &out, LayerStats &st, hipStream_t s)
+{
+    if (f32)
+        build_impl<float>(*this, (const float *)xs, n, d, eps, p, out, st, s);
+    else
+        build_impl<double>(*this, (const double *)xs, n, d, eps, p, out, st, s);
+}
+
+}  // namespace msj

@@ -1,0 +1,187 @@
+// msj_internal.h -- shared declarations of the HIP implementation.
+//
+// Everything here is internal to libmsj.so; the public boundary is
+// include/msj.h.  Device code in this library is compiled with
+// -ffp-contract=off: every fused multiply-add in a parity-relevant
+// expression is written as fma().
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+#include "msj.h"
+
+namespace msj {
+
+// Internal failure carrying an msj_status; converted to a return code at the
+// C ABI.
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char *what, const char *file, int line);
+#define MSJ_HIP(x) ::msj::hip_check((x), #x, __FILE__, __LINE__)
+
+// Device allocation that only grows; released on destruction.
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t cap = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void ensure(size_t n)
+    {
+        if (n <= cap)
+            return;
+        release();
+        hipError_t e = hipMalloc((void **)&p, n * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            throw Error(MSJ_E_NOMEM, std::string("hipMalloc of ") + std::to_string(n * sizeof(T)) +
+                                         " bytes failed: " + hipGetErrorString(e));
+        }
+        cap = n;
+    }
+    void release()
+    {
+        if (p)
+            (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+constexpr int WAVE = 64;
+
+// ---------------------------------------------------------------- primitives
+// Exclusive prefix sum: out[i] = sum(in[0..i)), out[n] = total.  out must hold
+// n+1 elements.  Scratch is grown on demand.
+struct Scanner {
+    DevBuf<uint64_t> part[4];
+    void excl_u32_to_u32(const uint32_t *in, uint32_t *out, size_t n, hipStream_t s);
+    void excl_u32_to_u64(const uint32_t *in, uint64_t *out, size_t n, hipStream_t s);
+    void excl_u64_to_u64(const uint64_t *in, uint64_t *out, size_t n, hipStream_t s);
+};
+
+// Stable LSD radix sort of 64-bit keys (optionally carrying 32-bit values)
+// on the low `bits` bits, 8 bits per pass.
+struct RadixSorter {
+    DevBuf<uint64_t> k_alt;
+    DevBuf<uint32_t> v_alt;
+    DevBuf<uint32_t> hist;
+    DevBuf<uint64_t> hist_scan;
+    Scanner scan;
+    void sort(uint64_t *keys, uint32_t *vals, size_t n, int bits, hipStream_t s);
+};
+
+// Sum of a u64 array into *dst (device), *dst cleared first.
+void reduce_sum_u64(const uint64_t *in, size_t n, unsigned long long *dst, hipStream_t s);
+// Max of a u32 array into *dst (device), *dst cleared first.
+void reduce_max_u32(const uint32_t *in, size_t n, uint32_t *dst, hipStream_t s);
+
+inline unsigned grid_for(size_t n, unsigned block)
+{
+    size_t g = (n + block - 1) / block;
+    return (unsigned)(g ? g : 1);
+}
+
+inline int bits_for(uint64_t maxval)  // bits needed to represent 0..maxval
+{
+    int b = 0;
+    while (b < 64 && (maxval >> b) != 0)
+        ++b;
+    return b;
+}
+
+// ------------------------------------------------------------- node net
+// Final node net in HBM (the reference's std::vector<Node>, nodes.cuh:9-38,
+// flattened to CSR).  Nodes are in the reference's node order; points within
+// a node ascend by row id.
+struct Index {
+    uint32_t M = 0;              // nodes
+    uint32_t edges = 0;          // adjacency entries (self included)
+    uint64_t num_calcs = 0;      // sum of numCalcs
+    DevBuf<uint32_t> perm;       // [n]   node-order position -> row (pointArray)
+    DevBuf<uint32_t> nstart;     // [M+1] node boundaries
+    DevBuf<uint32_t> aoff;       // [M+1] neighbour-node CSR offsets
+    DevBuf<uint32_t> adj;        // [edges]
+    DevBuf<uint64_t> sumsz;      // [M]   points in the neighbour nodes
+    DevBuf<uint64_t> ncalcs;     // [M]   numCalcs = size * sumsz
+};
+
+struct LayerStats {
+    uint32_t layers = 0, ncand = 0;
+    uint32_t chosen[MSJ_MAX_LAYERS] = {};
+    uint32_t nodes[MSJ_MAX_LAYERS] = {};
+    uint64_t calcs[MSJ_MAX_LAYERS] = {};
+    double sse[MSJ_MAX_LAYERS] = {};
+};
+
+// glibc random_r TYPE_3 stream (the generator behind the reference's rand(),
+// utils.cu:77); re-seeded at the start of every join.
+struct GlibcRand {
+    int32_t r[31];
+    int f = 3, b = 0;
+    void seed(uint32_t s);
+    int32_t next();
+};
+
+struct IndexBuilder {
+    // per-layer scratch
+    DevBuf<uint32_t> keys;       // [ncand*n] candidate keys, row order
+    DevBuf<double> test_rp, distmat, sigma, maxc, rpT;
+    DevBuf<uint64_t> ck;         // composite keys
+    DevBuf<uint32_t> flags, runidx, starts, runfirst, cnt_nodes;
+    DevBuf<double> sse;
+    DevBuf<uint32_t> vals, pnode, pnode2, perm2;
+    DevBuf<uint8_t> nsplit, nsplit2;
+    DevBuf<uint32_t> cparent, ckey, cstart, cfirst, deg, aoff2, adj2, aoff1, adj1;
+    DevBuf<uint64_t> sumsz2, ncalcs2, tmp64;
+    DevBuf<uint32_t> u32scalar;
+    DevBuf<unsigned long long> u64scalar;
+    Scanner scan;
+    RadixSorter sorter;
+    // Builds the node net of `xs` (SoA, row order, storage T) into `out`.
+    void build(const void *xs, bool f32, uint32_t n, uint32_t d, double eps, const msj_params &p,
+               Index &out, LayerStats &st, hipStream_t s);
+};
+
+// ------------------------------------------------------------------- join
+struct JoinItem {  // one wave: up to 64 queries of node x against list[cbeg, cend)
+    uint32_t x, qbeg, cbeg, cend;
+};
+
+struct JoinRunner {
+    DevBuf<uint32_t> itcnt, itoff, seglen;
+    DevBuf<JoinItem> items;
+    DevBuf<unsigned long long> counter, work_ctr;
+    DevBuf<uint64_t> pairs;      // (row << qbits | neighbour) keys
+    uint32_t nitems = 0;
+    Scanner scan;
+    RadixSorter sorter;
+    // Plans the work items of the final node net.
+    void plan(const Index &ix, uint32_t n, hipStream_t s);
+    // Runs the plain FP64 distance kernels; returns the number of pairs R.
+    // Grows `pairs` and reruns when R exceeds the capacity.
+    // With dims_evaluated, counting kernels also sum the FP64 dimensions consumed.
+    uint64_t run_plain(const Index &ix, const void *xn, bool f32, uint32_t n, uint32_t d,
+                       double eps, uint64_t cap_hint, hipStream_t s, hipEvent_t k0, hipEvent_t k1,
+                       uint64_t *launches, float *kernel_ms, uint64_t *dims_evaluated);
+    // Sorts the R pairs and writes counts[n], offsets[n+1], nbrs[R].
+    void finish(uint32_t n, uint64_t R, uint32_t *counts, uint64_t *offsets, uint32_t *nbrs,
+                hipStream_t s);
+};
+
+// Permutes SoA columns into node order: xn[j*n+i] = xs[j*n+perm[i]].
+void gather_node_order(const void *xs, void *xn, bool f32, const uint32_t *perm, uint32_t n,
+                       uint32_t d, hipStream_t s);
+
+}  // namespace msj

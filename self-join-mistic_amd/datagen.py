@@ -1,0 +1,129 @@
+"""Seeded synthetic inputs for the benchmark configurations and the tests.
+
+All generators use numpy's PCG64 (numpy.random.default_rng), so a seed gives
+the same FP64 values on every machine.  Data seeds are 1000 + config number,
+calibration-sample seeds 2000 + config number.
+
+  c1  uniform [0,1)^2, 65,536 points, eps = 0.01
+  c2  8-Gaussian mixture + 10% uniform noise, d = 18, min-max normalised
+  c3  correlated Gaussian, covariance Q diag(i^-1.2) Q^T, d = 90, normalised
+  c4  1,000 isotropic clusters (sigma 0.05, Zipf(1.1) sizes) clipped to [0,1],
+      d = 384, stored as FP32
+"""
+import numpy as np
+
+
+def uniform(n, d, seed):
+    return np.random.default_rng(seed).random((n, d))
+
+
+def _minmax(x):
+    lo = x.min(axis=0)
+    hi = x.max(axis=0)
+    span = np.where(hi > lo, hi - lo, 1.0)
+    return (x - lo) / span
+
+
+def mixture(n, d, seed, k=8, noise=0.1, sig_lo=0.02, sig_hi=0.15):
+    """c2 generator: k equal-weight Gaussians with per-(component, dim) sigma,
+    plus `noise` fraction of uniform points, then per-dimension min-max."""
+    rng = np.random.default_rng(seed)
+    means = rng.random((k, d))
+    sigmas = rng.uniform(sig_lo, sig_hi, size=(k, d))
+    is_noise = rng.random(n) < noise
+    comp = rng.integers(0, k, size=n)
+    x = means[comp] + sigmas[comp] * rng.standard_normal((n, d))
+    nn = int(is_noise.sum())
+    x[is_noise] = rng.random((nn, d))
+    return _minmax(x)
+
+
+def correlated(n, d, seed):
+    """c3 generator: x = Q diag(sqrt(lambda)) z, lambda_i = i^-1.2, Q from the QR
+    of a Gaussian matrix with diag(R) > 0, then per-dimension min-max."""
+    rng = np.random.default_rng(seed)
+    g = rng.standard_normal((d, d))
+    q, r = np.linalg.qr(g)
+    q = q * np.sign(np.diag(r))[None, :]
+    lam = np.arange(1, d + 1, dtype=np.float64) ** -1.2
+    z = rng.standard_normal((n, d))
+    x = (z * np.sqrt(lam)[None, :]) @ q.T
+    return _minmax(x)
+
+
+def clusters(n, d, seed, ncl=1000, zipf=1.1, sigma=0.05):
+    """c4 generator: Zipf-sized isotropic clusters clipped to [0,1], FP32."""
+    rng = np.random.default_rng(seed)
+    centres = rng.random((ncl, d))
+    w = np.arange(1, ncl + 1, dtype=np.float64) ** -zipf
+    w /= w.sum()
+    comp = rng.choice(ncl, size=n, p=w)
+    out = np.empty((n, d), dtype=np.float32)
+    step = 65536
+    for s in range(0, n, step):
+        e = min(n, s + step)
+        blk = centres[comp[s:e]] + sigma * rng.standard_normal((e - s, d))
+        out[s:e] = np.clip(blk, 0.0, 1.0).astype(np.float32)
+    return out
+
+
+def reorder_dims(x, order):
+    """main.cu:117-122: column j of the result is column order[j] of x."""
+    return np.ascontiguousarray(x[:, np.asarray(order, dtype=np.int64)])
+
+
+def calibrate_eps(x, target, sample=10000, seed=2000, device=None, k0=1024):
+    """eps such that the mean neighbour count (self excluded) over a seeded
+    sample of `sample` query rows is `target`: eps^2 is the (target*sample)-th
+    smallest of the pooled query-to-all squared distances (self excluded).
+
+    Each query contributes its k smallest distances; a query whose k-th value
+    does not exceed the current estimate is recomputed with 8x larger k until
+    no query can hide a smaller value.  Uses torch on `device` when given
+    (FP64 throughout), numpy otherwise."""
+    n = x.shape[0]
+    rng = np.random.default_rng(seed)
+    s = min(sample, n)
+    qidx = np.sort(rng.choice(n, size=s, replace=False))
+    want = target * s
+    if device is not None:
+        import torch
+        X = torch.as_tensor(np.ascontiguousarray(x, dtype=np.float64), device=device)
+
+        def smallest(rows, k):
+            out, kth = [], []
+            chunk = max(1, int(1e9 // (8 * n)))
+            for a in range(0, len(rows), chunk):
+                q = X[torch.as_tensor(rows[a:a + chunk], device=device)]
+                d2 = torch.cdist(q, X, compute_mode="donot_use_mm_for_euclid_dist") ** 2
+                v, _ = torch.topk(d2, k + 1, dim=1, largest=False)
+                out.append(v[:, 1:])
+                kth.append(v[:, -1])
+                del d2
+            return torch.cat(out), torch.cat(kth)
+
+        k = min(k0, n - 1)
+        main, kth = smallest(qidx, k)
+        kq = np.full(s, k)
+        extra = []
+        while True:
+            pool = torch.cat([main.reshape(-1)] + extra)
+            e2 = torch.kthvalue(pool, want).values.item()
+            sat = np.nonzero((kth.cpu().numpy() <= e2) & (kq < n - 1))[0]
+            if len(sat) == 0:
+                return float(np.sqrt(e2))
+            knew = int(min(n - 1, kq[sat].max() * 8))
+            v, kk = smallest(qidx[sat], knew)
+            idx = torch.as_tensor(sat, device=device)
+            main[idx] = float("inf")
+            kth[idx] = kk
+            kq[sat] = knew
+            extra.append(v.reshape(-1))
+    X = np.ascontiguousarray(x, dtype=np.float64)
+    pool = []
+    for i in qidx:
+        d2 = ((X - X[i]) ** 2).sum(axis=1)
+        d2[i] = np.inf
+        pool.append(np.sort(d2)[:n - 1])
+    pool = np.concatenate(pool)
+    return float(np.sqrt(np.partition(pool, want - 1)[want - 1]))

@@ -1,0 +1,268 @@
+"""ctypes binding of libmsj.so (include/msj.h).
+
+Mirrors the reference's join entry point nodeLauncher5 (reference
+src/launcher.cu:3093, include/launcher.cuh:77-82) and its host helper stddev
+(src/utils.cu:22-53).  Every call goes to the HIP library; there is no CPU
+fallback: if libmsj.so is missing or fails to load, the calls raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmsj.so")
+
+MAX_LAYERS = 16
+OK, E_INVAL, E_NOMEM, E_HIP, E_OVERFLOW, E_UNSUPPORTED, E_NODEV = 0, -1, -2, -3, -4, -5, -6
+STORAGE_F64, STORAGE_F32 = 0, 1
+FORM_PLAIN, FORM_MFMA_BF16, FORM_MFMA_SPLIT = 0, 1, 2
+OUT_HOST, OUT_DEVICE = 0, 1
+
+# Every symbol include/msj.h declares.
+EXPORTS = (
+    "msj_version", "msj_strerror", "msj_last_error", "msj_default_params",
+    "msj_self_join", "msj_result_free", "msj_ctx_create", "msj_ctx_destroy",
+    "msj_ctx_load", "msj_ctx_join", "msj_dim_order",
+)
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("struct_size", C.c_uint32),
+        ("device", C.c_int32),
+        ("maxrp", C.c_uint32),
+        ("rp_per_layer", C.c_uint32),
+        ("numidim", C.c_uint32),
+        ("seed", C.c_uint32),
+        ("sample_per", C.c_double),
+        ("min_calcs_per_node", C.c_uint64),
+        ("form", C.c_uint32),
+        ("storage", C.c_uint32),
+        ("output", C.c_uint32),
+        ("verbose", C.c_uint32),
+        ("count_work", C.c_uint32),
+        ("reserved0", C.c_uint32),
+        ("page_pairs", C.c_uint64),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("num_nodes", C.c_uint32),
+        ("num_layers", C.c_uint32),
+        ("num_candidates", C.c_uint32),
+        ("num_pages", C.c_uint32),
+        ("num_calcs", C.c_uint64),
+        ("total_pairs", C.c_uint64),
+        ("chosen", C.c_uint32 * MAX_LAYERS),
+        ("nodes_per_layer", C.c_uint32 * MAX_LAYERS),
+        ("calcs_per_layer", C.c_uint64 * MAX_LAYERS),
+        ("sse", C.c_double * MAX_LAYERS),
+        ("ref_num_batches", C.c_uint32),
+        ("pad0", C.c_uint32),
+        ("ref_var_calc", C.c_double),
+        ("ms_index", C.c_double),
+        ("ms_join", C.c_double),
+        ("ms_join_kernel", C.c_double),
+        ("ms_output", C.c_double),
+        ("ms_d2h", C.c_double),
+        ("ms_total", C.c_double),
+        ("join_launches", C.c_uint64),
+        ("dims_evaluated", C.c_uint64),
+        ("band_pairs", C.c_uint64),
+        ("max_score_err", C.c_double),
+        ("mean_score_err", C.c_double),
+    ]
+
+
+class Result(C.Structure):
+    _fields_ = [
+        ("n", C.c_uint32),
+        ("pad0", C.c_uint32),
+        ("total_pairs", C.c_uint64),
+        ("counts", C.POINTER(C.c_uint32)),
+        ("offsets", C.POINTER(C.c_uint64)),
+        ("nbrs", C.POINTER(C.c_uint32)),
+        ("node_order", C.POINTER(C.c_uint32)),
+        ("on_device", C.c_uint32),
+        ("pad1", C.c_uint32),
+        ("stats", Stats),
+    ]
+
+
+class MsjError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("msj error %d (%s): %s" % (code, _strerror(code), msg))
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Loads libmsj.so; raises if it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libmsj.so is not built (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        L.msj_version.restype = C.c_char_p
+        L.msj_strerror.restype = C.c_char_p
+        L.msj_strerror.argtypes = [C.c_int]
+        L.msj_last_error.restype = C.c_char_p
+        L.msj_default_params.argtypes = [C.POINTER(Params)]
+        L.msj_self_join.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_double,
+                                    C.POINTER(Params), C.POINTER(Result)]
+        L.msj_result_free.argtypes = [C.POINTER(Result)]
+        L.msj_ctx_create.argtypes = [C.POINTER(Params), C.POINTER(C.c_void_p)]
+        L.msj_ctx_destroy.argtypes = [C.c_void_p]
+        L.msj_ctx_load.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]
+        L.msj_ctx_join.argtypes = [C.c_void_p, C.c_double, C.POINTER(Result)]
+        L.msj_dim_order.argtypes = [C.POINTER(C.c_double), C.c_uint32, C.c_uint32,
+                                    C.POINTER(C.c_uint32), C.POINTER(C.c_double)]
+        _lib = L
+    return _lib
+
+
+def _strerror(code):
+    try:
+        return lib().msj_strerror(code).decode()
+    except Exception:  # noqa: BLE001 - message only
+        return "?"
+
+
+def _check(rc):
+    if rc != OK:
+        raise MsjError(rc, lib().msj_last_error().decode())
+
+
+def version():
+    return lib().msj_version().decode()
+
+
+def default_params(**kw):
+    p = Params()
+    lib().msj_default_params(C.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise AttributeError("msj_params has no field %r" % k)
+        setattr(p, k, v)
+    return p
+
+
+def dim_order(data):
+    """utils.cu:22-53: dims by decreasing population sigma (ties: higher index first)."""
+    data = np.ascontiguousarray(data, dtype=np.float64)
+    n, d = data.shape
+    order = np.zeros(d, dtype=np.uint32)
+    sigma = np.zeros(d, dtype=np.float64)
+    _check(lib().msj_dim_order(data.ctypes.data_as(C.POINTER(C.c_double)), n, d,
+                               order.ctypes.data_as(C.POINTER(C.c_uint32)),
+                               sigma.ctypes.data_as(C.POINTER(C.c_double))))
+    return order, sigma
+
+
+class JoinResult:
+    """Host copy of an msj_result.  `stats` is a dict of msj_stats."""
+
+    def __init__(self, r, copy_arrays=True):
+        self.n = r.n
+        self.total_pairs = r.total_pairs
+        st = r.stats
+        L = st.num_layers
+        self.stats = {
+            "num_nodes": st.num_nodes, "num_layers": L, "num_candidates": st.num_candidates,
+            "num_pages": st.num_pages, "num_calcs": st.num_calcs, "total_pairs": st.total_pairs,
+            "chosen": list(st.chosen)[:L], "nodes_per_layer": list(st.nodes_per_layer)[:L],
+            "calcs_per_layer": list(st.calcs_per_layer)[:L], "sse": list(st.sse)[:L],
+            "ref_num_batches": st.ref_num_batches, "ref_var_calc": st.ref_var_calc,
+            "ms_index": st.ms_index, "ms_join": st.ms_join, "ms_join_kernel": st.ms_join_kernel,
+            "ms_output": st.ms_output, "ms_d2h": st.ms_d2h, "ms_total": st.ms_total,
+            "join_launches": st.join_launches, "dims_evaluated": st.dims_evaluated,
+            "band_pairs": st.band_pairs,
+            "max_score_err": st.max_score_err, "mean_score_err": st.mean_score_err,
+        }
+        self.on_device = bool(r.on_device)
+        if copy_arrays and not r.on_device:
+            n, R = r.n, r.total_pairs
+
+            def arr(ptr, cnt, dt):
+                if not ptr or cnt == 0:
+                    return np.zeros(cnt if ptr else 0, dtype=dt)
+                return np.ctypeslib.as_array(ptr, shape=(cnt,)).copy()
+
+            self.counts = arr(r.counts, n, np.uint32)
+            self.offsets = arr(r.offsets, n + 1, np.uint64)
+            self.nbrs = arr(r.nbrs, R, np.uint32)
+            self.node_order = arr(r.node_order, n, np.uint32)
+
+    def pairs(self):
+        rows = np.repeat(np.arange(self.n, dtype=np.uint32), self.counts.astype(np.int64))
+        return np.stack([rows, self.nbrs], axis=1)
+
+
+def _as_input(data, storage):
+    dt = np.float32 if storage == STORAGE_F32 else np.float64
+    data = np.ascontiguousarray(data, dtype=dt)
+    if data.ndim != 2:
+        raise ValueError("data must be 2-D (n, d)")
+    return data
+
+
+def self_join(data, eps, **kw):
+    """One-shot join: the nodeLauncher5 replacement (launcher.cu:3093-3532)."""
+    p = default_params(**kw)
+    data = _as_input(data, p.storage)
+    n, d = data.shape
+    r = Result()
+    rc = lib().msj_self_join(data.ctypes.data_as(C.c_void_p), n, d, float(eps), C.byref(p),
+                             C.byref(r))
+    try:
+        _check(rc)
+        return JoinResult(r)
+    finally:
+        lib().msj_result_free(C.byref(r))
+
+
+class Context:
+    """Data resident in HBM across joins (msj_ctx_*)."""
+
+    def __init__(self, **kw):
+        self.params = default_params(**kw)
+        h = C.c_void_p()
+        _check(lib().msj_ctx_create(C.byref(self.params), C.byref(h)))
+        self.h = h
+        self.n = self.d = 0
+
+    def load(self, data):
+        data = _as_input(data, self.params.storage)
+        n, d = data.shape
+        _check(lib().msj_ctx_load(self.h, data.ctypes.data_as(C.c_void_p), n, d))
+        self.n, self.d = n, d
+
+    def join(self, eps, copy_arrays=True):
+        r = Result()
+        rc = lib().msj_ctx_join(self.h, float(eps), C.byref(r))
+        try:
+            _check(rc)
+            return JoinResult(r, copy_arrays=copy_arrays)
+        finally:
+            lib().msj_result_free(C.byref(r))
+
+    def close(self):
+        if self.h:
+            lib().msj_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass
