@@ -78,7 +78,7 @@ def cpu_baseline(x, eps, num_calcs, gpu_rows, seconds, threads):
     # parity on the sampled rows (full neighbour lists on both sides)
     pos = np.arange(0, x.shape[0], every, dtype=np.int64)
     rows = r.node_order[pos]
-    counts_g, offs_g, nbrs_g = gpu_rows
+    counts_g, offs_g, nbrs_g, _ = gpu_rows
     mism = 0
     for row in rows:
         a0, a1 = int(r.offsets[row]), int(r.offsets[row + 1])
@@ -105,6 +105,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--eps", type=float, default=None, help="override the calibrated eps")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="recompute eps from the seeded 10,000-query sample, print it, exit")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-threads", type=int, default=16)
@@ -119,16 +121,19 @@ def main():
     x = make_data(cfg)
     if args.eps is not None:
         eps = args.eps
-    elif "eps" in cfg:
+    elif "eps" in cfg and not args.calibrate:
         eps = cfg["eps"]
     else:
         import torch
         log("[bench] calibrating eps on the GPU (10,000 seeded queries)")
         eps = datagen.calibrate_eps(x, cfg["target"], sample=10000, seed=cfg["cal_seed"],
-                                    device="cuda:0")
+                                    device="cuda:0", log=lambda m: log("[bench] " + m))
         torch.cuda.empty_cache()
     eps_hex = struct.pack(">d", eps).hex()
     log("[bench] eps = %r (0x%s)" % (eps, eps_hex))
+    if args.calibrate:
+        print(json.dumps({"config": args.config, "eps": eps, "eps_hex": eps_hex}), flush=True)
+        return
 
     # counting pass (untimed): algorithmic FP64 work of the distance kernel
     with msj.Context(output=msj.OUT_HOST, count_work=1) as cc:

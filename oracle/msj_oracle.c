@@ -818,8 +818,12 @@ int or_self_join(const double *data, uint32_t n, uint32_t d, double eps, const o
     out->node_start[nn] = pos;
     out->adj_off[nn] = apos;
     out->adj = XMALLOC(uint32_t, apos);
-    for (uint32_t x = 0; x < nn; x++)
+    out->node_keys = XMALLOC(uint32_t, (size_t)nn * out->num_layers);
+    for (uint32_t x = 0; x < nn; x++) {
         memcpy(out->adj + out->adj_off[x], nodes.v[x].nbr, sizeof(uint32_t) * nodes.v[x].nnbr);
+        memcpy(out->node_keys + (size_t)x * out->num_layers, nodes.v[x].keys,
+               sizeof(uint32_t) * out->num_layers);
+    }
     ovec_free(&nodes);
     out->t_construct = omp_get_wtime() - t0;
 
@@ -940,5 +944,6 @@ void or_result_free(or_result *r)
     free(r->node_start);
     free(r->adj_off);
     free(r->adj);
+    free(r->node_keys);
     memset(r, 0, sizeof(*r));
 }

@@ -56,6 +56,8 @@ typedef struct or_result {
     uint32_t *node_start;        /* [num_nodes+1] node boundaries in node order     */
     uint32_t *adj_off;           /* [num_nodes+1] neighbour-node CSR                */
     uint32_t *adj;               /* [adj_off[num_nodes]] neighbour node ids         */
+    uint32_t *node_keys;         /* [num_nodes * num_layers] binNumbers per node;
+                                    0xFFFFFFFF from the layer a node stopped splitting */
     uint32_t chosen[OR_MAXL];    /* selected candidate per layer                    */
     double   sse[OR_MAXL];       /* its sum of squared node-size deviations         */
     uint32_t nodes_per_layer[OR_MAXL];

@@ -49,6 +49,7 @@ class Result(C.Structure):
         ("node_start", C.POINTER(C.c_uint32)),
         ("adj_off", C.POINTER(C.c_uint32)),
         ("adj", C.POINTER(C.c_uint32)),
+        ("node_keys", C.POINTER(C.c_uint32)),
         ("chosen", C.c_uint32 * MAXL),
         ("sse", C.c_double * MAXL),
         ("nodes_per_layer", C.c_uint32 * MAXL),
@@ -172,6 +173,8 @@ class JoinResult:
         na = int(self.adj_off[-1]) if len(self.adj_off) else 0
         self.adj = arr(r.adj, na, np.uint32)
         L = r.num_layers
+        self.node_keys = arr(r.node_keys, r.num_nodes * L if r.node_keys else 0,
+                             np.uint32).reshape(-1, L) if L else np.zeros((0, 0), np.uint32)
         self.chosen = list(r.chosen)[:L]
         self.sse = list(r.sse)[:L]
         self.nodes_per_layer = list(r.nodes_per_layer)[:L]

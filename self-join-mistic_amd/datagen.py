@@ -72,58 +72,63 @@ def reorder_dims(x, order):
     return np.ascontiguousarray(x[:, np.asarray(order, dtype=np.int64)])
 
 
-def calibrate_eps(x, target, sample=10000, seed=2000, device=None, k0=1024):
+def calibrate_eps(x, target, sample=10000, seed=2000, device=None, log=None):
     """eps such that the mean neighbour count (self excluded) over a seeded
     sample of `sample` query rows is `target`: eps^2 is the (target*sample)-th
     smallest of the pooled query-to-all squared distances (self excluded).
 
-    Each query contributes its k smallest distances; a query whose k-th value
-    does not exceed the current estimate is recomputed with 8x larger k until
-    no query can hide a smaller value.  Uses torch on `device` when given
-    (FP64 throughout), numpy otherwise."""
+    On a torch `device` the squared distances come from an FP64 GEMM,
+    |q|^2 + |x|^2 - 2 q.x (clamped at 0), and the selection is an exact radix
+    select over their IEEE bit patterns: three streaming passes, each a
+    histogram of the next 22/21/21 bits among the values sharing the prefix
+    found so far; nothing of size sample*n is stored.  Without a device,
+    numpy computes the pool directly (small inputs only)."""
     n = x.shape[0]
     rng = np.random.default_rng(seed)
     s = min(sample, n)
     qidx = np.sort(rng.choice(n, size=s, replace=False))
     want = target * s
-    if device is not None:
-        import torch
-        X = torch.as_tensor(np.ascontiguousarray(x, dtype=np.float64), device=device)
+    if device is None:
+        X = np.ascontiguousarray(x, dtype=np.float64)
+        pool = []
+        for i in qidx:
+            d2 = ((X - X[i]) ** 2).sum(axis=1)
+            d2[i] = np.inf
+            pool.append(d2)
+        pool = np.concatenate(pool)
+        return float(np.sqrt(np.partition(pool, want - 1)[want - 1]))
 
-        def smallest(rows, k):
-            out, kth = [], []
-            chunk = max(1, int(1e9 // (8 * n)))
-            for a in range(0, len(rows), chunk):
-                q = X[torch.as_tensor(rows[a:a + chunk], device=device)]
-                d2 = torch.cdist(q, X, compute_mode="donot_use_mm_for_euclid_dist") ** 2
-                v, _ = torch.topk(d2, k + 1, dim=1, largest=False)
-                out.append(v[:, 1:])
-                kth.append(v[:, -1])
-                del d2
-            return torch.cat(out), torch.cat(kth)
-
-        k = min(k0, n - 1)
-        main, kth = smallest(qidx, k)
-        kq = np.full(s, k)
-        extra = []
-        while True:
-            pool = torch.cat([main.reshape(-1)] + extra)
-            e2 = torch.kthvalue(pool, want).values.item()
-            sat = np.nonzero((kth.cpu().numpy() <= e2) & (kq < n - 1))[0]
-            if len(sat) == 0:
-                return float(np.sqrt(e2))
-            knew = int(min(n - 1, kq[sat].max() * 8))
-            v, kk = smallest(qidx[sat], knew)
-            idx = torch.as_tensor(sat, device=device)
-            main[idx] = float("inf")
-            kth[idx] = kk
-            kq[sat] = knew
-            extra.append(v.reshape(-1))
-    X = np.ascontiguousarray(x, dtype=np.float64)
-    pool = []
-    for i in qidx:
-        d2 = ((X - X[i]) ** 2).sum(axis=1)
-        d2[i] = np.inf
-        pool.append(np.sort(d2)[:n - 1])
-    pool = np.concatenate(pool)
-    return float(np.sqrt(np.partition(pool, want - 1)[want - 1]))
+    import torch
+    X = torch.as_tensor(np.ascontiguousarray(x, dtype=np.float64), device=device)
+    x2 = (X * X).sum(dim=1)
+    Q = torch.as_tensor(qidx, device=device)
+    chunk = max(1, int(4e9 // (8 * n)))
+    inf_bits = 0x7FF0000000000000
+    prefix, used, remaining = 0, 0, want
+    for width in (22, 21, 21):
+        shift = 64 - used - width
+        hist = torch.zeros(1 << width, dtype=torch.int64, device=device)
+        for a in range(0, s, chunk):
+            rows = Q[a:a + chunk]
+            q = X[rows]
+            d2 = torch.addmm(x2[None, :], q, X.t(), beta=1.0, alpha=-2.0)
+            d2 += x2[rows][:, None]
+            d2.clamp_(min=0.0)
+            d2[torch.arange(len(rows), device=device), rows] = float("inf")  # self
+            b = d2.view(torch.int64)
+            if used == 0:
+                sel = b[b < inf_bits]
+            else:
+                sel = b[(b >> (64 - used)) == prefix]
+            hist += torch.bincount((sel >> shift) & ((1 << width) - 1), minlength=1 << width)
+            del d2, b, sel
+        cum = torch.cumsum(hist, 0)
+        digit = int(torch.searchsorted(cum, torch.tensor([remaining], device=device)).item())
+        if digit > 0:
+            remaining -= int(cum[digit - 1].item())
+        prefix = (prefix << width) | digit
+        used += width
+        if log:
+            log("calibration pass %d/3 done" % (used // 21))
+    e2 = float(np.array([prefix], dtype=np.int64).view(np.float64)[0])
+    return float(np.sqrt(e2))
