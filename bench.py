@@ -27,7 +27,9 @@ import argparse
 import json
 import os
 import struct
+import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -44,10 +46,37 @@ FP64_PEAK_TFLOPS = 78.6
 CONFIGS = {
     "c1": dict(n=65536, d=2, kind="uniform", seed=1001, eps=0.01,
                workload="plain FP64 join, 65,536 x 2 uniform [0,1), eps 0.01"),
+    # eps_hex: `python bench.py --calibrate` on an MI355X (10,000 seeded queries)
     "c2": dict(n=2_000_000, d=18, kind="mixture", seed=1002, target=64, cal_seed=2002,
+               eps_hex="3fc1a4c7913e006e",
                workload="plain FP64 join, 2,000,000 x 18 Gaussian mixture + noise, "
                         "mean 64 neighbours"),
 }
+
+# Runs in a child process: torch brings its own HIP runtime, which must not
+# share a process with libmsj.so's.
+_CALIBRATE = r"""
+import sys, json
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+import datagen
+x = np.load(sys.argv[2])
+eps = datagen.calibrate_eps(x, int(sys.argv[3]), sample=10000, seed=int(sys.argv[4]),
+                            device="cuda:0", log=lambda m: print("[bench] " + m, file=sys.stderr,
+                                                                   flush=True))
+print(json.dumps({"eps": eps}))
+"""
+
+
+def calibrate(x, cfg):
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "x.npy")
+        np.save(path, x)
+        out = subprocess.run([sys.executable, "-c", _CALIBRATE,
+                              os.path.join(ROOT, "self-join-mistic_amd"), path,
+                              str(cfg["target"]), str(cfg["cal_seed"])],
+                             stdout=subprocess.PIPE, check=True, text=True).stdout
+    return json.loads(out.strip().splitlines()[-1])["eps"]
 
 
 def log(*a):
@@ -123,12 +152,11 @@ def main():
         eps = args.eps
     elif "eps" in cfg and not args.calibrate:
         eps = cfg["eps"]
+    elif "eps_hex" in cfg and not args.calibrate:
+        eps = struct.unpack(">d", bytes.fromhex(cfg["eps_hex"]))[0]
     else:
-        import torch
         log("[bench] calibrating eps on the GPU (10,000 seeded queries)")
-        eps = datagen.calibrate_eps(x, cfg["target"], sample=10000, seed=cfg["cal_seed"],
-                                    device="cuda:0", log=lambda m: log("[bench] " + m))
-        torch.cuda.empty_cache()
+        eps = calibrate(x, cfg)
     eps_hex = struct.pack(">d", eps).hex()
     log("[bench] eps = %r (0x%s)" % (eps, eps_hex))
     if args.calibrate:
@@ -144,20 +172,19 @@ def main():
     log("[bench] counting pass: R=%d nodes=%d candidates=%d dims=%d" % (
         ref_run.total_pairs, ref_run.stats["num_nodes"], ref_run.stats["num_calcs"], dims_eval))
 
-    import torch
+    # msj_ctx_join returns after its stream has drained: the clock below
+    # brackets exactly the K joins.
     with msj.Context(output=msj.OUT_DEVICE) as ctx:
         ctx.load(x)
         for w in range(args.warmup):
             r = ctx.join(eps, copy_arrays=False)
             log("[bench] warmup %d: %.1f ms" % (w, r.stats["ms_total"]))
-        torch.cuda.synchronize()
         kms, stage = [], []
         t0 = time.perf_counter()
         for k in range(args.steps):
             r = ctx.join(eps, copy_arrays=False)
             kms.append(r.stats["ms_join_kernel"])
             stage.append((r.stats["ms_index"], r.stats["ms_join"], r.stats["ms_output"]))
-        torch.cuda.synchronize()
         wall = time.perf_counter() - t0
     ms_step = wall * 1e3 / args.steps
     st = r.stats

@@ -48,7 +48,7 @@ struct msj_ctx {
     bool f32 = false;
     msj::DevBuf<unsigned char> staging;  // row-major upload
     msj::DevBuf<unsigned char> xs;       // SoA, row order
-    msj::DevBuf<unsigned char> xn;       // SoA, node order
+    msj::DevBuf<double> xr;              // FP64 rows, node order
     msj::Index ix;
     msj::IndexBuilder builder;
     msj::JoinRunner join;
@@ -218,9 +218,8 @@ void do_join(msj_ctx *c, double eps, msj_result *out)
     MSJ_HIP(hipEventRecord(c->ev[0], s));
     msj::LayerStats ls;
     c->builder.build(c->xs.p, c->f32, n, d, eps, c->prm, c->ix, ls, s);
-    const uint32_t esz = c->f32 ? 4 : 8;
-    c->xn.ensure((size_t)n * d * esz);
-    msj::gather_node_order(c->xs.p, c->xn.p, c->f32, c->ix.perm.p, n, d, s);
+    c->xr.ensure((size_t)n * msj::padded_dim(d));
+    msj::gather_node_rows(c->xs.p, c->xr.p, c->f32, c->ix.perm.p, n, d, s);
     c->join.plan(c->ix, n, s);
     MSJ_HIP(hipEventRecord(c->ev[1], s));
 
@@ -235,7 +234,7 @@ void do_join(msj_ctx *c, double eps, msj_result *out)
     }
     float kms = 0.f;
     uint64_t launches = 0, dims = 0;
-    const uint64_t R = c->join.run_plain(c->ix, c->xn.p, c->f32, n, d, eps, cap, s, c->ev[2],
+    const uint64_t R = c->join.run_plain(c->ix, c->xr.p, n, d, eps, cap, s, c->ev[2],
                                          c->ev[3], &launches, &kms,
                                          c->prm.count_work ? &dims : nullptr);
     st.dims_evaluated = dims;

@@ -146,18 +146,44 @@ __global__ __launch_bounds__(256) void k_rp_distmat(const T *__restrict__ xs, ui
 }
 
 // utils.cu:29-41 per column: sequential mean, then sequential sum of squares.
+// The additions stay in row order; loads are issued eight rows ahead so the
+// chain is not serialised on memory latency.
 __global__ void k_col_sigma(const double *__restrict__ A, uint32_t rows, uint32_t cols,
                             double *__restrict__ sigma)
 {
     uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= cols)
         return;
+    constexpr uint32_t U = 8;
     double mean = 0.0;
-    for (uint32_t r = 0; r < rows; ++r)
+    uint32_t r = 0;
+    for (; r + U <= rows; r += U) {
+        double v[U];
+#pragma unroll
+        for (uint32_t k = 0; k < U; ++k)
+            v[k] = A[(size_t)(r + k) * cols + c];
+#pragma unroll
+        for (uint32_t k = 0; k < U; ++k)
+            mean = mean + v[k];
+    }
+    for (; r < rows; ++r)
         mean = mean + A[(size_t)r * cols + c];
     mean = mean / (double)rows;
     double dev = 0.0;
-    for (uint32_t r = 0; r < rows; ++r) {
+    r = 0;
+    for (; r + U <= rows; r += U) {
+        double v[U];
+#pragma unroll
+        for (uint32_t k = 0; k < U; ++k)
+            v[k] = A[(size_t)(r + k) * cols + c];
+#pragma unroll
+        for (uint32_t k = 0; k < U; ++k) {
+            double t = v[k] - mean;
+            double sq = t * t;
+            dev = dev + sq;
+        }
+    }
+    for (; r < rows; ++r) {
         double t = A[(size_t)r * cols + c] - mean;
         double sq = t * t;
         dev = dev + sq;
@@ -166,21 +192,35 @@ __global__ void k_col_sigma(const double *__restrict__ A, uint32_t rows, uint32_
     sigma[c] = sqrt(dev);
 }
 
-// utils.cu:105-118: max corner over the sample rows, starting from 0.
+// utils.cu:105-118: max corner over the sample rows, starting from 0.  A
+// maximum does not depend on the order it is taken in: one block per column.
 template <class T>
-__global__ void k_col_max(const T *__restrict__ xs, uint32_t n, uint32_t d, uint32_t S,
-                          double *__restrict__ maxc)
+__global__ __launch_bounds__(256) void k_col_max(const T *__restrict__ xs, uint32_t n, uint32_t d,
+                                                 uint32_t S, double *__restrict__ maxc)
 {
-    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= d)
-        return;
+    __shared__ double part[256 / WAVE];
+    const uint32_t j = blockIdx.x;
     double m = 0;
-    for (uint32_t i = 0; i < S; ++i) {
+    for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) {
         double v = (double)xs[(size_t)j * n + i];
         if (m < v)
             m = v;
     }
-    maxc[j] = m;
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) {
+        double v = __shfl_down(m, o, WAVE);
+        if (m < v)
+            m = v;
+    }
+    if ((threadIdx.x & (WAVE - 1)) == 0)
+        part[threadIdx.x / WAVE] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 256 / WAVE; ++w)
+            if (m < part[w])
+                m = part[w];
+        maxc[j] = m;
+    }
 }
 
 // kernel.cu:880-909 (32 RP keys) + nodes.cu:128-135 (axis keys).  One thread
@@ -265,6 +305,7 @@ __global__ void k_runfirst(const uint32_t *__restrict__ runidx, uint32_t n, uint
 }
 
 // nodes.cu:194-199: sum over nodes, in node order, of (size - N/#nodes)^2.
+// Sequential in node order (loads issued eight runs ahead).
 __global__ void k_cand_sse(const uint32_t *__restrict__ runfirst,
                            const uint32_t *__restrict__ starts, uint32_t tcount, uint32_t n,
                            double *__restrict__ sse, uint32_t *__restrict__ cnt)
@@ -272,10 +313,24 @@ __global__ void k_cand_sse(const uint32_t *__restrict__ runfirst,
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= tcount)
         return;
-    uint32_t r0 = runfirst[t], r1 = runfirst[t + 1];
-    double avg = (double)n * 1.0 / (double)(r1 - r0);
+    constexpr uint32_t U = 8;
+    const uint32_t r0 = runfirst[t], r1 = runfirst[t + 1];
+    const double avg = (double)n * 1.0 / (double)(r1 - r0);
     double s = 0;
-    for (uint32_t r = r0; r < r1; ++r) {
+    uint32_t r = r0;
+    for (; r + U <= r1; r += U) {
+        uint32_t st[U + 1];
+#pragma unroll
+        for (uint32_t k = 0; k <= U; ++k)
+            st[k] = starts[r + k];
+#pragma unroll
+        for (uint32_t k = 0; k < U; ++k) {
+            double df = (double)(st[k + 1] - st[k]) - avg;
+            double sq = df * df;
+            s = s + sq;
+        }
+    }
+    for (; r < r1; ++r) {
         double df = (double)(starts[r + 1] - starts[r]) - avg;
         double sq = df * df;
         s = s + sq;
@@ -526,7 +581,7 @@ void build_impl(IndexBuilder &B, const T *xs, uint32_t n, uint32_t d, double eps
             LAUNCH(k_rp_distmat<T>, grid_for((size_t)S * ntest, 256), 256, xs, n, d, S,
                    B.test_rp.p, ntest, B.distmat.p);
         LAUNCH(k_col_sigma, 1, 64 * ((ntest + 63) / 64), B.distmat.p, S, ntest, B.sigma.p);
-        LAUNCH(k_col_max<T>, grid_for(d, 64), 64, xs, n, d, S, B.maxc.p);
+        LAUNCH(k_col_max<T>, d, 256, xs, n, d, S, B.maxc.p);
         MSJ_HIP(hipMemcpyAsync(h_sig.data(), B.sigma.p, ntest * sizeof(double),
                                hipMemcpyDeviceToHost, s));
         MSJ_HIP(hipMemcpyAsync(h_max.data(), B.maxc.p, d * sizeof(double), hipMemcpyDeviceToHost,

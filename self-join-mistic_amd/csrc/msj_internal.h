@@ -155,33 +155,41 @@ struct IndexBuilder {
 };
 
 // ------------------------------------------------------------------- join
-struct JoinItem {  // one wave: up to 64 queries of node x against list[cbeg, cend)
-    uint32_t x, qbeg, cbeg, cend;
+// One wave: queries [qbeg, qend) of node x (offsets inside the node) against
+// entries [cbeg, cend) of x's upper candidate list.
+struct JoinItem {
+    uint32_t x, qbeg, qend, cbeg, cend;
 };
 
 struct JoinRunner {
-    DevBuf<uint32_t> itcnt, itoff, seglen;
+    DevBuf<uint32_t> itcnt, itoff, uoff, uadj;
+    DevBuf<uint64_t> ulen, tot64;
     DevBuf<JoinItem> items;
     DevBuf<unsigned long long> counter, work_ctr;
     DevBuf<uint64_t> pairs;      // (row << qbits | neighbour) keys
     uint32_t nitems = 0;
     Scanner scan;
     RadixSorter sorter;
-    // Plans the work items of the final node net.
+    // Upper neighbour lists and work items of the final node net.
     void plan(const Index &ix, uint32_t n, hipStream_t s);
-    // Runs the plain FP64 distance kernels; returns the number of pairs R.
-    // Grows `pairs` and reruns when R exceeds the capacity.
-    // With dims_evaluated, counting kernels also sum the FP64 dimensions consumed.
-    uint64_t run_plain(const Index &ix, const void *xn, bool f32, uint32_t n, uint32_t d,
-                       double eps, uint64_t cap_hint, hipStream_t s, hipEvent_t k0, hipEvent_t k1,
+    // Runs the plain FP64 distance kernel on the node-ordered row-major FP64
+    // copy xr; returns the number of pairs R.  Grows `pairs` and reruns when
+    // R exceeds the capacity.  With dims_evaluated, a counting build of the
+    // kernel also sums the FP64 dimensions the early-exit test consumed.
+    uint64_t run_plain(const Index &ix, const double *xr, uint32_t n, uint32_t d, double eps,
+                       uint64_t cap_hint, hipStream_t s, hipEvent_t k0, hipEvent_t k1,
                        uint64_t *launches, float *kernel_ms, uint64_t *dims_evaluated);
     // Sorts the R pairs and writes counts[n], offsets[n+1], nbrs[R].
     void finish(uint32_t n, uint64_t R, uint32_t *counts, uint64_t *offsets, uint32_t *nbrs,
                 hipStream_t s);
 };
 
-// Permutes SoA columns into node order: xn[j*n+i] = xs[j*n+perm[i]].
-void gather_node_order(const void *xs, void *xn, bool f32, const uint32_t *perm, uint32_t n,
-                       uint32_t d, hipStream_t s);
+// Row width of the node-ordered FP64 copy: d rounded up to a multiple of 4.
+inline uint32_t padded_dim(uint32_t d) { return (d + 3u) & ~3u; }
+
+// xr[i*ds + j] = FP64 value of column j of node-order row i (row perm[i]),
+// ds = padded_dim(d), zero padding.
+void gather_node_rows(const void *xs, double *xr, bool f32, const uint32_t *perm, uint32_t n,
+                      uint32_t d, hipStream_t s);
 
 }  // namespace msj

@@ -4,23 +4,31 @@
 // Reference: kernel.cu:992-1046 nodeByPoint5 and kernel.cu:833-865
 // cachedDistanceCheck (ILP = 4); launcher.cu:3418-3506 batch loop.
 //
-// Work decomposition.  A work item is one wave: up to 64 query points of one
-// node X against a contiguous segment of X's candidate list (the points of
-// X's neighbour nodes, concatenated in adjacency order).  When X has fewer
-// than 64 points the wave's lanes form 64/|X| groups that stride the segment
-// together, so small nodes still fill the wave.  Query coordinates of the
-// first DR dimensions live in registers; candidate coordinates are read
-// column-wise from the node-ordered SoA copy (lanes of a group read the same
-// address).
+// Symmetry.  The test is bit-symmetric: q - c = -(c - q) exactly, so both
+// orders of a pair give the same sum and the same decision.  Each unordered
+// candidate pair is therefore evaluated once -- node X against the nodes
+// Y >= X of its neighbour list, and inside X only pairs p <= c -- and an
+// accepted pair is emitted in both orders (the self pair once).
+//
+// Work decomposition.  A work item is one wave: a range of X's points (the
+// queries) against a range of X's upper candidate list (the points of the
+// neighbour nodes Y >= X, concatenated).  The wave walks that range 64
+// candidates at a time, one candidate per lane with its first DR
+// coordinates in VGPRs, and loops over the queries; a query's coordinates
+// are wave-uniform, so they are scalar loads into SGPRs and reach every lane
+// with no broadcast instruction.  The FP64 data is a node-ordered row-major
+// copy, so a query row is one contiguous scalar load.
 //
 // Decision: exactly the reference's sequence per 4-dimension chunk,
 //   s = fma(d0, d0, s); s = s + d1*d1; s = s + d2*d2; s = s + d3*d3;
-//   reject as soon as s > eps^2,
-// with this file compiled without floating-point contraction.
+//   reject once s > eps^2,
+// with this file compiled without floating-point contraction.  The wave
+// leaves the chunk loop as soon as no lane is still undecided.
 //
-// Emission: one atomic per wave per step (ballot + popcount prefix), not per
-// pair.  Keys are (row << qbits | neighbour row); one radix sort then yields
-// every row's neighbours ascending, the counts and the CSR offsets.
+// Emission: one atomic per wave per accepting query (ballot + popcount
+// prefix), not per pair.  Keys are (row << qbits | neighbour row); one radix
+// sort then yields every row's neighbours ascending, the counts and the
+// CSR offsets.
 #include <algorithm>
 
 #include "msj_internal.h"
@@ -31,113 +39,135 @@ namespace msj {
 
 namespace {
 
-constexpr uint32_t QB = 64;                  // queries per work item
-constexpr uint64_t EVALS_PER_ITEM = 262144;  // pair evaluations per work item
+constexpr uint32_t QMAX = 256;               // queries per work item
+constexpr uint64_t EVALS_PER_ITEM = 262144;  // lane evaluations per work item
 
-// `dims` (counting builds only) accumulates the dimensions consumed before
-// the decision: min(d, 4 * chunks evaluated).
-template <class T, int DR, bool COUNT>
-__device__ inline bool a3_accept(const double (&qc)[DR], const T *__restrict__ xn, uint32_t n,
-                                 uint32_t d, uint32_t p, uint32_t c, double eps2, uint32_t &dims)
-{
-    double s = 0.0;
-#pragma unroll
-    for (int l = 0; l < DR; l += 4) {
-        if (l < (int)d) {
-            if (COUNT)
-                dims += min(4u, d - (uint32_t)l);
-            double t0 = qc[l] - (double)xn[(size_t)l * n + c];
-            s = fma(t0, t0, s);
-#pragma unroll
-            for (int j = 1; j < 4; ++j) {
-                if (l + j < (int)d) {
-                    double t = qc[l + j] - (double)xn[(size_t)(l + j) * n + c];
-                    double sq = t * t;
-                    s = s + sq;
-                }
-            }
-            if (s > eps2)
-                return false;
-        }
-    }
-    for (uint32_t l = DR; l < d; l += 4) {
-        if (COUNT)
-            dims += min(4u, d - l);
-        double t0 = (double)xn[(size_t)l * n + p] - (double)xn[(size_t)l * n + c];
-        s = fma(t0, t0, s);
-        for (uint32_t j = 1; j < 4 && l + j < d; ++j) {
-            double t = (double)xn[(size_t)(l + j) * n + p] - (double)xn[(size_t)(l + j) * n + c];
-            double sq = t * t;
-            s = s + sq;
-        }
-        if (s > eps2)
-            return false;
-    }
-    return true;
-}
+// Rows of xr are ds = roundup(d, 4) doubles, zero-padded: a padded
+// coordinate contributes s + (0 - 0)^2 = s exactly, so the chunk loop needs
+// no per-dimension guard.  DS is ds when ds <= 32 (the whole row lives in
+// registers: candidate in VGPRs, query in SGPRs); otherwise DS = 32 and the
+// remaining chunks are read from memory.
+// Wave-wide vote on a lane predicate, kept in SGPRs (no VALU round trip).
+__device__ inline uint64_t wave_vote(bool v) { return __builtin_amdgcn_ballot_w64(v); }
 
-template <class T, int DR, bool COUNT>
-__global__ __launch_bounds__(256) void k_join_plain(
-    const T *__restrict__ xn, uint32_t n, uint32_t d, double eps2,
-    const uint32_t *__restrict__ nstart, const uint32_t *__restrict__ aoff,
-    const uint32_t *__restrict__ adj, const uint32_t *__restrict__ perm,
+template <int DS, bool TAIL, bool COUNT>
+__global__ __launch_bounds__(256) void k_join_sym(
+    const double *__restrict__ xr, uint32_t d, uint32_t ds, double eps2,
+    const uint32_t *__restrict__ nstart, const uint32_t *__restrict__ uoff,
+    const uint32_t *__restrict__ uadj, const uint32_t *__restrict__ perm,
     const JoinItem *__restrict__ items, uint32_t nitems, int qbits,
     unsigned long long *__restrict__ out, unsigned long long *__restrict__ counter,
     unsigned long long cap, unsigned long long *__restrict__ work)
 {
     const uint32_t lane = threadIdx.x & (WAVE - 1);
-    const uint32_t it = blockIdx.x * (blockDim.x / WAVE) + (threadIdx.x / WAVE);
+    // wave-uniform by construction; readfirstlane lets the compiler keep the
+    // item, the query index and the query row in SGPRs (scalar loads)
+    const uint32_t it = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / WAVE) +
+                                                       (threadIdx.x / WAVE));
     if (it >= nitems)
-        return;  // wave-uniform
+        return;
     const JoinItem w = items[it];
-    const uint32_t x0 = nstart[w.x];
-    const uint32_t xsz = nstart[w.x + 1] - x0;
-    const uint32_t qn = min(QB, xsz - w.qbeg);
-    const uint32_t G = QB / qn;
-    const uint32_t qi = lane % qn, g = lane / qn;
-    const bool lane_on = g < G;
-    const uint32_t p = x0 + w.qbeg + (lane_on ? qi : 0u);
-
-    double qc[DR];
-#pragma unroll
-    for (int j = 0; j < DR; ++j)
-        qc[j] = (j < (int)d) ? (double)xn[(size_t)j * n + p] : 0.0;
-    const unsigned long long pkey = (unsigned long long)perm[p] << qbits;
+    const uint32_t X = w.x;
+    const uint32_t x0 = nstart[X];
     const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (WAVE - lane));
+    const uint32_t stride = TAIL ? ds : (uint32_t)DS;
 
-    uint32_t a = aoff[w.x];
+    // per-lane cursor into X's upper neighbour list
+    uint32_t a = uoff[X];
+    uint32_t Y = uadj[a];
     uint32_t ybase = 0;
-    uint32_t ystart = nstart[adj[a]];
-    uint32_t ysz = nstart[adj[a] + 1] - ystart;
-    const uint32_t iters = (w.cend - w.cbeg + G - 1) / G;
-    uint32_t dims = 0;
-    for (uint32_t itr = 0; itr < iters; ++itr) {
-        const uint32_t c = w.cbeg + g + itr * G;
-        const bool valid = lane_on && c < w.cend;
-        bool acc = false;
-        uint32_t pos = 0;
-        if (valid) {
-            while (c >= ybase + ysz) {  // advance to the neighbour node holding c
+    uint32_t ystart = nstart[Y];
+    uint32_t ysz = nstart[Y + 1] - ystart;
+    uint64_t dims = 0;
+
+    for (uint32_t cs = w.cbeg; cs < w.cend; cs += WAVE) {
+        const uint32_t ci = cs + lane;
+        const bool cvalid = ci < w.cend;
+        uint32_t c = 0;
+        if (cvalid) {
+            while (ci >= ybase + ysz) {
                 ybase += ysz;
                 ++a;
-                const uint32_t Y = adj[a];
+                Y = uadj[a];
                 ystart = nstart[Y];
                 ysz = nstart[Y + 1] - ystart;
             }
-            pos = ystart + (c - ybase);
-            acc = a3_accept<T, DR, COUNT>(qc, xn, n, d, p, pos, eps2, dims);
+            c = ystart + (ci - ybase);
         }
-        const uint64_t m = __ballot(acc);
-        if (m) {
-            const uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1u;
-            unsigned long long base = 0;
-            if (lane == leader)
-                base = atomicAdd(counter, (unsigned long long)__popcll(m));
-            base = __shfl(base, (int)leader, WAVE);
-            if (acc) {
-                const unsigned long long idx = base + (unsigned long long)__popcll(m & lt);
-                if (idx < cap)
-                    out[idx] = pkey | (unsigned long long)perm[pos];
+        const bool self_node = cvalid && Y == X;
+        const double *crow = xr + (size_t)c * stride;  // c = 0 for idle lanes
+        double cc[DS];
+#pragma unroll
+        for (int k = 0; k < DS; ++k)
+            cc[k] = crow[k];
+        const unsigned long long corig = perm[c];
+
+        for (uint32_t p = x0 + w.qbeg; p < x0 + w.qend; ++p) {
+            bool alive = cvalid && (!self_node || p <= c);
+            if (!wave_vote(alive))
+                continue;
+            const double *qrow = xr + (size_t)p * stride;  // wave-uniform
+            double qc[DS];
+#pragma unroll
+            for (int k = 0; k < DS; ++k)
+                qc[k] = qrow[k];
+            double s = 0.0;
+#pragma unroll
+            for (int l = 0; l < DS; l += 4) {
+                if (l > 0 && !wave_vote(alive))
+                    break;
+                if (COUNT && alive && (uint32_t)l < d)
+                    dims += min(4u, d - (uint32_t)l);
+                double t0 = qc[l] - cc[l];
+                s = fma(t0, t0, s);
+                double t1 = qc[l + 1] - cc[l + 1];
+                double q1 = t1 * t1;
+                s = s + q1;
+                double t2 = qc[l + 2] - cc[l + 2];
+                double q2 = t2 * t2;
+                s = s + q2;
+                double t3 = qc[l + 3] - cc[l + 3];
+                double q3 = t3 * t3;
+                s = s + q3;
+                alive = alive && !(s > eps2);
+            }
+            if (TAIL) {
+                for (uint32_t l = DS; l < ds && wave_vote(alive); l += 4) {
+                    if (COUNT && alive && l < d)
+                        dims += min(4u, d - l);
+                    double t0 = qrow[l] - crow[l];
+                    s = fma(t0, t0, s);
+                    double t1 = qrow[l + 1] - crow[l + 1];
+                    double q1 = t1 * t1;
+                    s = s + q1;
+                    double t2 = qrow[l + 2] - crow[l + 2];
+                    double q2 = t2 * t2;
+                    s = s + q2;
+                    double t3 = qrow[l + 3] - crow[l + 3];
+                    double q3 = t3 * t3;
+                    s = s + q3;
+                    alive = alive && !(s > eps2);
+                }
+            }
+            const uint64_t m1 = wave_vote(alive);
+            if (m1) {
+                const bool two = alive && c != p;
+                const uint64_t m2 = wave_vote(two);
+                const unsigned long long tot =
+                    (unsigned long long)(__popcll(m1) + __popcll(m2));
+                unsigned long long base = 0;
+                if (lane == 0)
+                    base = atomicAdd(counter, tot);
+                base = __shfl(base, 0, WAVE);
+                if (alive) {
+                    const unsigned long long porig = perm[p];
+                    unsigned long long idx =
+                        base + (unsigned long long)(__popcll(m1 & lt) + __popcll(m2 & lt));
+                    if (idx < cap)
+                        out[idx] = (porig << qbits) | corig;
+                    if (two && idx + 1 < cap)
+                        out[idx + 1] = (corig << qbits) | porig;
+                }
             }
         }
     }
@@ -151,39 +181,78 @@ __global__ __launch_bounds__(256) void k_join_plain(
     }
 }
 
-// Work items per node: ceil(|X|/64) query chunks x segments of the
-// candidate list sized so one item is about EVALS_PER_ITEM evaluations.
-__global__ __launch_bounds__(256) void k_item_count(const uint32_t *__restrict__ nstart,
-                                                    const uint64_t *__restrict__ sumsz, uint32_t M,
-                                                    uint32_t *__restrict__ cnt,
-                                                    uint32_t *__restrict__ seglen)
+// Upper neighbour lists: entries Y >= X of X's adjacency, and their sizes.
+__global__ __launch_bounds__(256) void k_upper_count(const uint32_t *__restrict__ nstart,
+                                                     const uint32_t *__restrict__ aoff,
+                                                     const uint32_t *__restrict__ adj, uint32_t M,
+                                                     uint32_t *__restrict__ cnt,
+                                                     uint64_t *__restrict__ len)
 {
     uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= M)
         return;
-    uint32_t sz = nstart[x + 1] - nstart[x];
-    uint64_t L = sumsz[x];
-    uint32_t qn = min(QB, sz);
-    uint64_t sl = EVALS_PER_ITEM / qn;
-    uint64_t nseg = (L + sl - 1) / sl;
-    if (nseg == 0)
-        nseg = 1;
-    uint64_t nq = (sz + QB - 1) / QB;
-    cnt[x] = (uint32_t)(nq * nseg);
-    seglen[x] = (uint32_t)sl;
+    uint32_t k = 0;
+    uint64_t L = 0;
+    for (uint32_t e = aoff[x]; e < aoff[x + 1]; ++e) {
+        uint32_t y = adj[e];
+        if (y >= x) {
+            ++k;
+            L += nstart[y + 1] - nstart[y];
+        }
+    }
+    cnt[x] = k;
+    len[x] = L;
+}
+
+__global__ __launch_bounds__(256) void k_upper_write(const uint32_t *__restrict__ aoff,
+                                                     const uint32_t *__restrict__ adj, uint32_t M,
+                                                     const uint32_t *__restrict__ uoff,
+                                                     uint32_t *__restrict__ uadj)
+{
+    uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= M)
+        return;
+    uint32_t o = uoff[x];
+    for (uint32_t e = aoff[x]; e < aoff[x + 1]; ++e)
+        if (adj[e] >= x)
+            uadj[o++] = adj[e];
+}
+
+// Items per node: query ranges of up to QMAX points x candidate ranges sized
+// so one item is about EVALS_PER_ITEM lane evaluations (multiples of 64).
+__device__ inline void item_shape(uint32_t sz, uint64_t L, uint32_t *qper, uint32_t *nq,
+                                  uint32_t *cper, uint32_t *nc)
+{
+    *qper = min(QMAX, sz);
+    *nq = (sz + *qper - 1) / *qper;
+    uint64_t cp = EVALS_PER_ITEM / *qper;
+    cp = (cp + WAVE - 1) / WAVE * WAVE;
+    *cper = (uint32_t)cp;
+    uint64_t k = (L + cp - 1) / cp;
+    *nc = (uint32_t)(k ? k : 1);
+}
+
+__global__ __launch_bounds__(256) void k_item_count(const uint32_t *__restrict__ nstart,
+                                                    const uint64_t *__restrict__ ulen, uint32_t M,
+                                                    uint32_t *__restrict__ cnt)
+{
+    uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= M)
+        return;
+    uint32_t qper, nq, cper, nc;
+    item_shape(nstart[x + 1] - nstart[x], ulen[x], &qper, &nq, &cper, &nc);
+    cnt[x] = nq * nc;
 }
 
 __global__ __launch_bounds__(256) void k_item_fill(const uint32_t *__restrict__ nstart,
-                                                   const uint64_t *__restrict__ sumsz,
-                                                   const uint32_t *__restrict__ seglen,
+                                                   const uint64_t *__restrict__ ulen,
                                                    const uint32_t *__restrict__ itoff, uint32_t M,
                                                    uint32_t nitems, JoinItem *__restrict__ items)
 {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nitems)
         return;
-    // node x with itoff[x] <= k < itoff[x+1]
-    uint32_t lo = 0, hi = M;
+    uint32_t lo = 0, hi = M;  // node x with itoff[x] <= k < itoff[x+1]
     while (hi - lo > 1) {
         uint32_t mid = lo + (hi - lo) / 2;
         if (itoff[mid] <= k)
@@ -191,32 +260,35 @@ __global__ __launch_bounds__(256) void k_item_fill(const uint32_t *__restrict__ 
         else
             hi = mid;
     }
-    uint32_t x = lo;
-    uint32_t local = k - itoff[x];
-    uint64_t L = sumsz[x];
-    uint32_t sl = seglen[x];
-    uint32_t nseg = (uint32_t)((L + sl - 1) / sl);
-    if (nseg == 0)
-        nseg = 1;
-    uint32_t q = local / nseg, sg = local % nseg;
+    const uint32_t x = lo;
+    const uint32_t sz = nstart[x + 1] - nstart[x];
+    const uint64_t L = ulen[x];
+    uint32_t qper, nq, cper, nc;
+    item_shape(sz, L, &qper, &nq, &cper, &nc);
+    const uint32_t local = k - itoff[x];
+    const uint32_t qi = local / nc, ck = local % nc;
     JoinItem w;
     w.x = x;
-    w.qbeg = q * QB;
-    w.cbeg = (uint32_t)std::min<uint64_t>((uint64_t)sg * sl, L);
-    w.cend = (uint32_t)std::min<uint64_t>((uint64_t)(sg + 1) * sl, L);
+    w.qbeg = qi * qper;
+    w.qend = min(sz, (qi + 1) * qper);
+    w.cbeg = (uint32_t)std::min<uint64_t>((uint64_t)ck * cper, L);
+    w.cend = (uint32_t)std::min<uint64_t>((uint64_t)(ck + 1) * cper, L);
     items[k] = w;
 }
 
+// xr[i*ds + j] = xs[j*n + perm[i]] widened to FP64 (exact for FP32 inputs),
+// zero for the padding columns d <= j < ds.
 template <class T>
-__global__ __launch_bounds__(256) void k_gather(const T *__restrict__ xs, T *__restrict__ xn,
-                                                const uint32_t *__restrict__ perm, uint32_t n,
-                                                uint32_t d)
+__global__ __launch_bounds__(256) void k_gather_rows(const T *__restrict__ xs,
+                                                     double *__restrict__ xr,
+                                                     const uint32_t *__restrict__ perm, uint32_t n,
+                                                     uint32_t d, uint32_t ds)
 {
     size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (size_t)n * d)
+    if (idx >= (size_t)n * ds)
         return;
-    uint32_t j = (uint32_t)(idx / n), i = (uint32_t)(idx - (size_t)j * n);
-    xn[idx] = xs[(size_t)j * n + perm[i]];
+    uint32_t i = (uint32_t)(idx / ds), j = (uint32_t)(idx - (size_t)i * ds);
+    xr[idx] = j < d ? (double)xs[(size_t)j * n + perm[i]] : 0.0;
 }
 
 __global__ __launch_bounds__(256) void k_rows_count(const uint64_t *__restrict__ pairs, size_t R,
@@ -262,83 +334,92 @@ __global__ void k_zero(T *p, size_t n)
         MSJ_HIP(hipGetLastError());                                                      \
     } while (0)
 
-template <class T, int DR>
-void launch_plain(const Index &ix, const void *xn, uint32_t n, uint32_t d, double eps2,
-                  const JoinItem *items, uint32_t nitems, int qbits, unsigned long long *out,
-                  unsigned long long *counter, unsigned long long cap, unsigned long long *work,
-                  hipStream_t s)
+template <int DS, bool TAIL>
+void launch_sym(const JoinRunner &J, const Index &ix, const double *xr, uint32_t d, uint32_t ds,
+                double eps2, int qbits, unsigned long long *out, unsigned long long cap,
+                unsigned long long *work, hipStream_t s)
 {
-    const unsigned waves_per_block = 4;
+    const unsigned wpb = 4;  // waves per block
     if (work)
-        LAUNCH((k_join_plain<T, DR, true>), grid_for(nitems, waves_per_block),
-               WAVE * waves_per_block, (const T *)xn, n, d, eps2, ix.nstart.p, ix.aoff.p, ix.adj.p,
-               ix.perm.p, items, nitems, qbits, out, counter, cap, work);
+        LAUNCH((k_join_sym<DS, TAIL, true>), grid_for(J.nitems, wpb), WAVE * wpb, xr, d, ds, eps2,
+               ix.nstart.p, J.uoff.p, J.uadj.p, ix.perm.p, J.items.p, J.nitems, qbits, out,
+               J.counter.p, cap, work);
     else
-        LAUNCH((k_join_plain<T, DR, false>), grid_for(nitems, waves_per_block),
-               WAVE * waves_per_block, (const T *)xn, n, d, eps2, ix.nstart.p, ix.aoff.p, ix.adj.p,
-               ix.perm.p, items, nitems, qbits, out, counter, cap, work);
+        LAUNCH((k_join_sym<DS, TAIL, false>), grid_for(J.nitems, wpb), WAVE * wpb, xr, d, ds,
+               eps2, ix.nstart.p, J.uoff.p, J.uadj.p, ix.perm.p, J.items.p, J.nitems, qbits, out,
+               J.counter.p, cap, work);
 }
 
-template <class T>
-void dispatch_plain(const Index &ix, const void *xn, uint32_t n, uint32_t d, double eps2,
-                    const JoinItem *items, uint32_t nitems, int qbits, unsigned long long *out,
-                    unsigned long long *counter, unsigned long long cap, unsigned long long *work,
-                    hipStream_t s)
+void dispatch_sym(const JoinRunner &J, const Index &ix, const double *xr, uint32_t d, double eps2,
+                  int qbits, unsigned long long *out, unsigned long long cap,
+                  unsigned long long *work, hipStream_t s)
 {
-#define MSJ_DR(v)                                                                        \
-    if (d <= v) {                                                                        \
-        launch_plain<T, v>(ix, xn, n, d, eps2, items, nitems, qbits, out, counter, cap, work, \
-                           s);                                                           \
-        return;                                                                          \
+    const uint32_t ds = padded_dim(d);
+    switch (ds) {
+#define MSJ_DS(v)                                                                   \
+    case v:                                                                         \
+        launch_sym<v, false>(J, ix, xr, d, ds, eps2, qbits, out, cap, work, s);     \
+        return;
+        MSJ_DS(4)
+        MSJ_DS(8)
+        MSJ_DS(12)
+        MSJ_DS(16)
+        MSJ_DS(20)
+        MSJ_DS(24)
+        MSJ_DS(28)
+        MSJ_DS(32)
+#undef MSJ_DS
+    default:
+        launch_sym<32, true>(J, ix, xr, d, ds, eps2, qbits, out, cap, work, s);
     }
-    MSJ_DR(4)
-    MSJ_DR(8)
-    MSJ_DR(12)
-    MSJ_DR(16)
-    MSJ_DR(20)
-    MSJ_DR(24)
-    MSJ_DR(32)
-    MSJ_DR(48)
-#undef MSJ_DR
-    launch_plain<T, 64>(ix, xn, n, d, eps2, items, nitems, qbits, out, counter, cap, work, s);
 }
 
 }  // namespace
 
-void gather_node_order(const void *xs, void *xn, bool f32, const uint32_t *perm, uint32_t n,
-                       uint32_t d, hipStream_t s)
+void gather_node_rows(const void *xs, double *xr, bool f32, const uint32_t *perm, uint32_t n,
+                      uint32_t d, hipStream_t s)
 {
-    size_t m = (size_t)n * d;
+    const uint32_t ds = padded_dim(d);
+    size_t m = (size_t)n * ds;
     if (f32)
-        LAUNCH(k_gather<float>, grid_for(m, 256), 256, (const float *)xs, (float *)xn, perm, n, d);
+        LAUNCH(k_gather_rows<float>, grid_for(m, 256), 256, (const float *)xs, xr, perm, n, d, ds);
     else
-        LAUNCH(k_gather<double>, grid_for(m, 256), 256, (const double *)xs, (double *)xn, perm, n,
-               d);
+        LAUNCH(k_gather_rows<double>, grid_for(m, 256), 256, (const double *)xs, xr, perm, n, d,
+               ds);
 }
 
 void JoinRunner::plan(const Index &ix, uint32_t n, hipStream_t s)
 {
     const uint32_t M = ix.M;
     itcnt.ensure(M);
-    seglen.ensure(M);
     itoff.ensure((size_t)M + 1);
-    LAUNCH(k_item_count, grid_for(M, 256), 256, ix.nstart.p, ix.sumsz.p, M, itcnt.p, seglen.p);
-    DevBuf<uint64_t> tot;
-    tot.ensure((size_t)M + 1);
-    scan.excl_u32_to_u64(itcnt.p, tot.p, M, s);
+    ulen.ensure(M);
+    uoff.ensure((size_t)M + 1);
+    tot64.ensure((size_t)M + 1);
+    LAUNCH(k_upper_count, grid_for(M, 256), 256, ix.nstart.p, ix.aoff.p, ix.adj.p, M, itcnt.p,
+           ulen.p);
+    scan.excl_u32_to_u32(itcnt.p, uoff.p, M, s);
+    uint32_t ue = 0;
+    MSJ_HIP(hipMemcpyAsync(&ue, uoff.p + M, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    MSJ_HIP(hipStreamSynchronize(s));
+    uadj.ensure(std::max<uint32_t>(ue, 1));
+    LAUNCH(k_upper_write, grid_for(M, 256), 256, ix.aoff.p, ix.adj.p, M, uoff.p, uadj.p);
+
+    LAUNCH(k_item_count, grid_for(M, 256), 256, ix.nstart.p, ulen.p, M, itcnt.p);
+    scan.excl_u32_to_u64(itcnt.p, tot64.p, M, s);
     uint64_t total = 0;
-    MSJ_HIP(hipMemcpyAsync(&total, tot.p + M, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    MSJ_HIP(hipMemcpyAsync(&total, tot64.p + M, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
     MSJ_HIP(hipStreamSynchronize(s));
     if (total >= 0xFFFFFFFFull)
         throw Error(MSJ_E_OVERFLOW, "join work items exceed 2^32");
     nitems = (uint32_t)total;
     scan.excl_u32_to_u32(itcnt.p, itoff.p, M, s);
     items.ensure(std::max<uint32_t>(nitems, 1));
-    LAUNCH(k_item_fill, grid_for(nitems, 256), 256, ix.nstart.p, ix.sumsz.p, seglen.p, itoff.p, M,
-           nitems, items.p);
+    LAUNCH(k_item_fill, grid_for(nitems, 256), 256, ix.nstart.p, ulen.p, itoff.p, M, nitems,
+           items.p);
 }
 
-uint64_t JoinRunner::run_plain(const Index &ix, const void *xn, bool f32, uint32_t n, uint32_t d,
+uint64_t JoinRunner::run_plain(const Index &ix, const double *xr, uint32_t n, uint32_t d,
                                double eps, uint64_t cap_hint, hipStream_t s, hipEvent_t k0,
                                hipEvent_t k1, uint64_t *launches, float *kernel_ms,
                                uint64_t *dims_evaluated)
@@ -360,13 +441,8 @@ uint64_t JoinRunner::run_plain(const Index &ix, const void *xn, bool f32, uint32
             LAUNCH(k_zero<unsigned long long>, 1, 64, work, (size_t)1);
         MSJ_HIP(hipEventRecord(k0, s));
         if (nitems) {
-            if (f32)
-                dispatch_plain<float>(ix, xn, n, d, eps2, items.p, nitems, qbits,
-                                      (unsigned long long *)pairs.p, counter.p, pairs.cap, work, s);
-            else
-                dispatch_plain<double>(ix, xn, n, d, eps2, items.p, nitems, qbits,
-                                       (unsigned long long *)pairs.p, counter.p, pairs.cap, work,
-                                       s);
+            dispatch_sym(*this, ix, xr, d, eps2, qbits, (unsigned long long *)pairs.p, pairs.cap,
+                         work, s);
             ++*launches;
         }
         MSJ_HIP(hipEventRecord(k1, s));
