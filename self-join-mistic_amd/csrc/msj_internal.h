@@ -171,7 +171,7 @@ struct JoinRunner {
     Scanner scan;
     RadixSorter sorter;
     // Upper neighbour lists and work items of the final node net.
-    void plan(const Index &ix, uint32_t n, hipStream_t s);
+    void plan(const Index &ix, uint32_t n, uint32_t d, hipStream_t s);
     // Runs the plain FP64 distance kernel on the node-ordered row-major FP64
     // copy xr; returns the number of pairs R.  Grows `pairs` and reruns when
     // R exceeds the capacity.  With dims_evaluated, a counting build of the

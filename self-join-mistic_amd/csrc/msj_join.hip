@@ -39,17 +39,35 @@ namespace msj {
 
 namespace {
 
-constexpr uint32_t QMAX = 256;               // queries per work item
 constexpr uint64_t EVALS_PER_ITEM = 262144;  // lane evaluations per work item
+constexpr uint32_t WPB = 4;                  // waves per block
+constexpr uint32_t LDS_WAVE_BYTES = 8192;    // staged query rows per wave
+constexpr uint32_t HITS = 256;               // per-wave LDS buffer of result keys
 
-// Rows of xr are ds = roundup(d, 4) doubles, zero-padded: a padded
-// coordinate contributes s + (0 - 0)^2 = s exactly, so the chunk loop needs
-// no per-dimension guard.  DS is ds when ds <= 32 (the whole row lives in
-// registers: candidate in VGPRs, query in SGPRs); otherwise DS = 32 and the
-// remaining chunks are read from memory.
+// Queries per work item for a row width DS (in-register dimensions): as many
+// FP64 rows as fit LDS_WAVE_BYTES, at most 64.
+__host__ __device__ constexpr uint32_t qmax_for(uint32_t DS)
+{
+    return (LDS_WAVE_BYTES / (8u * DS)) < 64u ? (LDS_WAVE_BYTES / (8u * DS)) : 64u;
+}
+
 // Wave-wide vote on a lane predicate, kept in SGPRs (no VALU round trip).
 __device__ inline uint64_t wave_vote(bool v) { return __builtin_amdgcn_ballot_w64(v); }
 
+// Rows of xr are ds = roundup(d, 4) doubles, zero-padded: a padded
+// coordinate contributes s + (0 - 0)^2 = s exactly, so the chunk loop needs
+// no per-dimension guard.  DS is ds when ds <= 32 (candidate rows live in
+// VGPRs); otherwise DS = 32 and the remaining chunks come from memory.
+//
+// A work item's query rows are contiguous in xr (node order); the wave
+// copies them into its LDS slice once, then every query of the inner loop
+// is a wave-uniform LDS address (a broadcast read) instead of a scalar load
+// that would miss the small scalar cache.
+//
+// Result keys go to a per-wave LDS buffer; the wave reserves global space
+// with one atomic per flush (at most HITS keys) and copies the buffer out
+// with coalesced stores.  A global atomic per accepting step would
+// serialise on the single counter.
 template <int DS, bool TAIL, bool COUNT>
 __global__ __launch_bounds__(256) void k_join_sym(
     const double *__restrict__ xr, uint32_t d, uint32_t ds, double eps2,
@@ -59,18 +77,51 @@ __global__ __launch_bounds__(256) void k_join_sym(
     unsigned long long *__restrict__ out, unsigned long long *__restrict__ counter,
     unsigned long long cap, unsigned long long *__restrict__ work)
 {
+    constexpr uint32_t QM = qmax_for(DS);
+    __shared__ double qsh[WPB][QM * DS];
+    __shared__ unsigned long long hsh[WPB][HITS];
     const uint32_t lane = threadIdx.x & (WAVE - 1);
+    const uint32_t wv = threadIdx.x / WAVE;
     // wave-uniform by construction; readfirstlane lets the compiler keep the
-    // item, the query index and the query row in SGPRs (scalar loads)
-    const uint32_t it = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / WAVE) +
-                                                       (threadIdx.x / WAVE));
-    if (it >= nitems)
-        return;
-    const JoinItem w = items[it];
+    // item and the loop counters in SGPRs
+    const uint32_t it = __builtin_amdgcn_readfirstlane(blockIdx.x * WPB + wv);
+    const bool have = it < nitems;
+    JoinItem w = {0, 0, 0, 0, 0};
+    if (have)
+        w = items[it];
     const uint32_t X = w.x;
-    const uint32_t x0 = nstart[X];
-    const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (WAVE - lane));
+    const uint32_t x0 = have ? nstart[X] : 0u;
     const uint32_t stride = TAIL ? ds : (uint32_t)DS;
+    const uint32_t qn = w.qend - w.qbeg;
+    // stage the item's query rows (first DS dimensions) in LDS
+    double *qs = qsh[wv];
+    for (uint32_t e = lane; e < qn * DS; e += WAVE) {
+        const uint32_t r = e / DS, k = e - r * DS;
+        qs[e] = xr[(size_t)(x0 + w.qbeg + r) * stride + k];
+    }
+    __syncthreads();
+    if (!have)
+        return;
+    const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (WAVE - lane));
+    unsigned long long *hb = hsh[wv];
+    uint32_t fill = 0;  // wave-uniform
+    // copies the buffered keys out: one atomic for the whole buffer
+    auto flush = [&]() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        unsigned long long base = 0;
+        if (lane == 0)
+            base = atomicAdd(counter, (unsigned long long)fill);
+        base = __shfl(base, 0, WAVE);
+        for (uint32_t e = lane; e < fill; e += WAVE)
+            if (base + e < cap)
+                out[base + e] = hb[e];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        fill = 0;
+    };
 
     // per-lane cursor into X's upper neighbour list
     uint32_t a = uoff[X];
@@ -102,15 +153,12 @@ __global__ __launch_bounds__(256) void k_join_sym(
             cc[k] = crow[k];
         const unsigned long long corig = perm[c];
 
-        for (uint32_t p = x0 + w.qbeg; p < x0 + w.qend; ++p) {
+        for (uint32_t qi = 0; qi < qn; ++qi) {
+            const uint32_t p = x0 + w.qbeg + qi;
             bool alive = cvalid && (!self_node || p <= c);
             if (!wave_vote(alive))
                 continue;
-            const double *qrow = xr + (size_t)p * stride;  // wave-uniform
-            double qc[DS];
-#pragma unroll
-            for (int k = 0; k < DS; ++k)
-                qc[k] = qrow[k];
+            const double *qrow = qs + qi * DS;  // wave-uniform LDS address
             double s = 0.0;
 #pragma unroll
             for (int l = 0; l < DS; l += 4) {
@@ -118,32 +166,35 @@ __global__ __launch_bounds__(256) void k_join_sym(
                     break;
                 if (COUNT && alive && (uint32_t)l < d)
                     dims += min(4u, d - (uint32_t)l);
-                double t0 = qc[l] - cc[l];
+                const double2 qa = *(const double2 *)(qrow + l);
+                const double2 qb = *(const double2 *)(qrow + l + 2);
+                double t0 = qa.x - cc[l];
                 s = fma(t0, t0, s);
-                double t1 = qc[l + 1] - cc[l + 1];
+                double t1 = qa.y - cc[l + 1];
                 double q1 = t1 * t1;
                 s = s + q1;
-                double t2 = qc[l + 2] - cc[l + 2];
+                double t2 = qb.x - cc[l + 2];
                 double q2 = t2 * t2;
                 s = s + q2;
-                double t3 = qc[l + 3] - cc[l + 3];
+                double t3 = qb.y - cc[l + 3];
                 double q3 = t3 * t3;
                 s = s + q3;
                 alive = alive && !(s > eps2);
             }
             if (TAIL) {
+                const double *grow = xr + (size_t)p * stride;
                 for (uint32_t l = DS; l < ds && wave_vote(alive); l += 4) {
                     if (COUNT && alive && l < d)
                         dims += min(4u, d - l);
-                    double t0 = qrow[l] - crow[l];
+                    double t0 = grow[l] - crow[l];
                     s = fma(t0, t0, s);
-                    double t1 = qrow[l + 1] - crow[l + 1];
+                    double t1 = grow[l + 1] - crow[l + 1];
                     double q1 = t1 * t1;
                     s = s + q1;
-                    double t2 = qrow[l + 2] - crow[l + 2];
+                    double t2 = grow[l + 2] - crow[l + 2];
                     double q2 = t2 * t2;
                     s = s + q2;
-                    double t3 = qrow[l + 3] - crow[l + 3];
+                    double t3 = grow[l + 3] - crow[l + 3];
                     double q3 = t3 * t3;
                     s = s + q3;
                     alive = alive && !(s > eps2);
@@ -153,24 +204,22 @@ __global__ __launch_bounds__(256) void k_join_sym(
             if (m1) {
                 const bool two = alive && c != p;
                 const uint64_t m2 = wave_vote(two);
-                const unsigned long long tot =
-                    (unsigned long long)(__popcll(m1) + __popcll(m2));
-                unsigned long long base = 0;
-                if (lane == 0)
-                    base = atomicAdd(counter, tot);
-                base = __shfl(base, 0, WAVE);
+                const uint32_t tot = (uint32_t)(__popcll(m1) + __popcll(m2));
+                if (fill + tot > HITS)
+                    flush();
                 if (alive) {
                     const unsigned long long porig = perm[p];
-                    unsigned long long idx =
-                        base + (unsigned long long)(__popcll(m1 & lt) + __popcll(m2 & lt));
-                    if (idx < cap)
-                        out[idx] = (porig << qbits) | corig;
-                    if (two && idx + 1 < cap)
-                        out[idx + 1] = (corig << qbits) | porig;
+                    const uint32_t idx = fill + (uint32_t)(__popcll(m1 & lt) + __popcll(m2 & lt));
+                    hb[idx] = (porig << qbits) | corig;
+                    if (two)
+                        hb[idx + 1] = (corig << qbits) | porig;
                 }
+                fill += tot;
             }
         }
     }
+    if (fill)
+        flush();
     if (COUNT) {
         unsigned long long v = dims;
 #pragma unroll
@@ -218,12 +267,12 @@ __global__ __launch_bounds__(256) void k_upper_write(const uint32_t *__restrict_
             uadj[o++] = adj[e];
 }
 
-// Items per node: query ranges of up to QMAX points x candidate ranges sized
+// Items per node: query ranges of up to qmax points x candidate ranges sized
 // so one item is about EVALS_PER_ITEM lane evaluations (multiples of 64).
-__device__ inline void item_shape(uint32_t sz, uint64_t L, uint32_t *qper, uint32_t *nq,
-                                  uint32_t *cper, uint32_t *nc)
+__device__ inline void item_shape(uint32_t sz, uint64_t L, uint32_t qmax, uint32_t *qper,
+                                  uint32_t *nq, uint32_t *cper, uint32_t *nc)
 {
-    *qper = min(QMAX, sz);
+    *qper = min(qmax, sz);
     *nq = (sz + *qper - 1) / *qper;
     uint64_t cp = EVALS_PER_ITEM / *qper;
     cp = (cp + WAVE - 1) / WAVE * WAVE;
@@ -234,20 +283,21 @@ __device__ inline void item_shape(uint32_t sz, uint64_t L, uint32_t *qper, uint3
 
 __global__ __launch_bounds__(256) void k_item_count(const uint32_t *__restrict__ nstart,
                                                     const uint64_t *__restrict__ ulen, uint32_t M,
-                                                    uint32_t *__restrict__ cnt)
+                                                    uint32_t qmax, uint32_t *__restrict__ cnt)
 {
     uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= M)
         return;
     uint32_t qper, nq, cper, nc;
-    item_shape(nstart[x + 1] - nstart[x], ulen[x], &qper, &nq, &cper, &nc);
+    item_shape(nstart[x + 1] - nstart[x], ulen[x], qmax, &qper, &nq, &cper, &nc);
     cnt[x] = nq * nc;
 }
 
 __global__ __launch_bounds__(256) void k_item_fill(const uint32_t *__restrict__ nstart,
                                                    const uint64_t *__restrict__ ulen,
                                                    const uint32_t *__restrict__ itoff, uint32_t M,
-                                                   uint32_t nitems, JoinItem *__restrict__ items)
+                                                   uint32_t qmax, uint32_t nitems,
+                                                   JoinItem *__restrict__ items)
 {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nitems)
@@ -264,7 +314,7 @@ __global__ __launch_bounds__(256) void k_item_fill(const uint32_t *__restrict__ 
     const uint32_t sz = nstart[x + 1] - nstart[x];
     const uint64_t L = ulen[x];
     uint32_t qper, nq, cper, nc;
-    item_shape(sz, L, &qper, &nq, &cper, &nc);
+    item_shape(sz, L, qmax, &qper, &nq, &cper, &nc);
     const uint32_t local = k - itoff[x];
     const uint32_t qi = local / nc, ck = local % nc;
     JoinItem w;
@@ -339,7 +389,7 @@ void launch_sym(const JoinRunner &J, const Index &ix, const double *xr, uint32_t
                 double eps2, int qbits, unsigned long long *out, unsigned long long cap,
                 unsigned long long *work, hipStream_t s)
 {
-    const unsigned wpb = 4;  // waves per block
+    const unsigned wpb = WPB;
     if (work)
         LAUNCH((k_join_sym<DS, TAIL, true>), grid_for(J.nitems, wpb), WAVE * wpb, xr, d, ds, eps2,
                ix.nstart.p, J.uoff.p, J.uadj.p, ix.perm.p, J.items.p, J.nitems, qbits, out,
@@ -388,9 +438,11 @@ void gather_node_rows(const void *xs, double *xr, bool f32, const uint32_t *perm
                ds);
 }
 
-void JoinRunner::plan(const Index &ix, uint32_t n, hipStream_t s)
+void JoinRunner::plan(const Index &ix, uint32_t n, uint32_t d, hipStream_t s)
 {
     const uint32_t M = ix.M;
+    const uint32_t ds = padded_dim(d);
+    const uint32_t qmax = qmax_for(ds <= 32 ? ds : 32u);
     itcnt.ensure(M);
     itoff.ensure((size_t)M + 1);
     ulen.ensure(M);
@@ -405,7 +457,7 @@ void JoinRunner::plan(const Index &ix, uint32_t n, hipStream_t s)
     uadj.ensure(std::max<uint32_t>(ue, 1));
     LAUNCH(k_upper_write, grid_for(M, 256), 256, ix.aoff.p, ix.adj.p, M, uoff.p, uadj.p);
 
-    LAUNCH(k_item_count, grid_for(M, 256), 256, ix.nstart.p, ulen.p, M, itcnt.p);
+    LAUNCH(k_item_count, grid_for(M, 256), 256, ix.nstart.p, ulen.p, M, qmax, itcnt.p);
     scan.excl_u32_to_u64(itcnt.p, tot64.p, M, s);
     uint64_t total = 0;
     MSJ_HIP(hipMemcpyAsync(&total, tot64.p + M, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
@@ -415,7 +467,7 @@ void JoinRunner::plan(const Index &ix, uint32_t n, hipStream_t s)
     nitems = (uint32_t)total;
     scan.excl_u32_to_u32(itcnt.p, itoff.p, M, s);
     items.ensure(std::max<uint32_t>(nitems, 1));
-    LAUNCH(k_item_fill, grid_for(nitems, 256), 256, ix.nstart.p, ulen.p, itoff.p, M, nitems,
+    LAUNCH(k_item_fill, grid_for(nitems, 256), 256, ix.nstart.p, ulen.p, itoff.p, M, qmax, nitems,
            items.p);
 }
 
