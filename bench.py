@@ -79,6 +79,16 @@ def calibrate(x, cfg):
     return json.loads(out.strip().splitlines()[-1])["eps"]
 
 
+def traffic_bytes(config):
+    """HBM bytes per launch of the distance kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload, if any."""
+    path = os.path.join(ROOT, "profiles", "k_join_sym_2Mx18_plain_traffic.json")
+    if config != "c2" or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)["traffic_bytes_per_launch"]
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -210,8 +220,9 @@ def main():
         "ms_join": float(np.mean([s[1] for s in stage])),
         "ms_output": float(np.mean([s[2] for s in stage])),
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "k_join_plain (FP64 vector ALU; FP64 peak)",
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                     "traffic": traffic_bytes(args.config),
+                     "kernel": "k_join_sym (FP64 vector ALU; MI355X FP64 peak)",
                      "kernel_ms": k_ms, "algorithmic_flops": flops},
     }
     if not args.no_cpu_baseline:

@@ -218,8 +218,9 @@ void do_join(msj_ctx *c, double eps, msj_result *out)
     MSJ_HIP(hipEventRecord(c->ev[0], s));
     msj::LayerStats ls;
     c->builder.build(c->xs.p, c->f32, n, d, eps, c->prm, c->ix, ls, s);
+    c->join.order(c->ix, c->xs.p, c->f32, n, d, s);
     c->xr.ensure((size_t)n * msj::padded_dim(d));
-    msj::gather_node_rows(c->xs.p, c->xr.p, c->f32, c->ix.perm.p, n, d, s);
+    msj::gather_node_rows(c->xs.p, c->xr.p, c->f32, c->join.jperm.p, n, d, s);
     c->join.plan(c->ix, n, d, s);
     MSJ_HIP(hipEventRecord(c->ev[1], s));
 

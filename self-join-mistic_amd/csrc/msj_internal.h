@@ -167,9 +167,15 @@ struct JoinRunner {
     DevBuf<JoinItem> items;
     DevBuf<unsigned long long> counter, work_ctr;
     DevBuf<uint64_t> pairs;      // (row << qbits | neighbour) keys
+    DevBuf<uint32_t> jperm;      // join order: node order, Morton order inside a node
+    DevBuf<uint64_t> jkeys;
+    DevBuf<double> mm;
     uint32_t nitems = 0;
     Scanner scan;
     RadixSorter sorter;
+    // Join order of the points: nodes as in ix, points inside a node sorted
+    // along a Morton curve of the first three columns (jperm).
+    void order(const Index &ix, const void *xs, bool f32, uint32_t n, uint32_t d, hipStream_t s);
     // Upper neighbour lists and work items of the final node net.
     void plan(const Index &ix, uint32_t n, uint32_t d, hipStream_t s);
     // Runs the plain FP64 distance kernel on the node-ordered row-major FP64

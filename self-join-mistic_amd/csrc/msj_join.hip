@@ -43,6 +43,24 @@ constexpr uint64_t EVALS_PER_ITEM = 262144;  // lane evaluations per work item
 constexpr uint32_t WPB = 4;                  // waves per block
 constexpr uint32_t LDS_WAVE_BYTES = 8192;    // staged query rows per wave
 constexpr uint32_t HITS = 256;               // per-wave LDS buffer of result keys
+__device__ inline double wave_min(double v)
+{
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) {
+        double t = __shfl_xor(v, o, WAVE);
+        v = t < v ? t : v;
+    }
+    return v;
+}
+__device__ inline double wave_max(double v)
+{
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) {
+        double t = __shfl_xor(v, o, WAVE);
+        v = t > v ? t : v;
+    }
+    return v;
+}
 
 // Queries per work item for a row width DS (in-register dimensions): as many
 // FP64 rows as fit LDS_WAVE_BYTES, at most 64.
@@ -103,6 +121,7 @@ __global__ __launch_bounds__(256) void k_join_sym(
     if (!have)
         return;
     const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (WAVE - lane));
+    const uint64_t qall = qn >= WAVE ? ~0ull : ((1ull << qn) - 1ull);
     unsigned long long *hb = hsh[wv];
     uint32_t fill = 0;  // wave-uniform
     // copies the buffered keys out: one atomic for the whole buffer
@@ -153,7 +172,10 @@ __global__ __launch_bounds__(256) void k_join_sym(
             cc[k] = crow[k];
         const unsigned long long corig = perm[c];
 
-        for (uint32_t qi = 0; qi < qn; ++qi) {
+        uint64_t qmask = qall;
+        while (qmask) {
+            const uint32_t qi = (uint32_t)__builtin_ctzll(qmask);
+            qmask &= qmask - 1ull;
             const uint32_t p = x0 + w.qbeg + qi;
             bool alive = cvalid && (!self_node || p <= c);
             if (!wave_vote(alive))
@@ -370,6 +392,81 @@ __global__ __launch_bounds__(256) void k_rows_nbrs(const uint64_t *__restrict__ 
         nbrs[i] = (uint32_t)(pairs[i] & qmask);
 }
 
+// Per-column minimum and maximum of the first columns (one block per column).
+template <class T>
+__global__ __launch_bounds__(256) void k_col_range(const T *__restrict__ xs, uint32_t n,
+                                                   double *__restrict__ mm)
+{
+    __shared__ double lo_s[256 / WAVE], hi_s[256 / WAVE];
+    const uint32_t j = blockIdx.x;
+    double lo = __builtin_inf(), hi = -__builtin_inf();
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        double v = (double)xs[(size_t)j * n + i];
+        lo = v < lo ? v : lo;
+        hi = v > hi ? v : hi;
+    }
+    lo = wave_min(lo);
+    hi = wave_max(hi);
+    if ((threadIdx.x & (WAVE - 1)) == 0) {
+        lo_s[threadIdx.x / WAVE] = lo;
+        hi_s[threadIdx.x / WAVE] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 256 / WAVE; ++w) {
+            lo = lo_s[w] < lo ? lo_s[w] : lo;
+            hi = hi_s[w] > hi ? hi_s[w] : hi;
+        }
+        mm[2 * j] = lo;
+        mm[2 * j + 1] = hi;
+    }
+}
+
+__device__ inline uint32_t spread3(uint32_t v)  // 10 bits -> every third bit
+{
+    v &= 0x3FFu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+// Key (node, Morton code of the first min(3, d) columns quantised to 10
+// bits) for every node-order position: sorting it orders points inside each
+// node along a space-filling curve.  Values: the node-order row ids.
+template <class T>
+__global__ __launch_bounds__(256) void k_morton_keys(const T *__restrict__ xs, uint32_t n,
+                                                     uint32_t km, const double *__restrict__ mm,
+                                                     const uint32_t *__restrict__ nstart,
+                                                     uint32_t M, const uint32_t *__restrict__ perm,
+                                                     uint64_t *__restrict__ keys,
+                                                     uint32_t *__restrict__ vals)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    uint32_t lo = 0, hi = M;  // node holding position i
+    while (hi - lo > 1) {
+        uint32_t mid = lo + (hi - lo) / 2;
+        if (nstart[mid] <= i)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const uint32_t row = perm[i];
+    uint32_t code = 0;
+    for (uint32_t k = 0; k < km; ++k) {
+        const double a = mm[2 * k], b = mm[2 * k + 1];
+        const double v = (double)xs[(size_t)k * n + row];
+        double q = b > a ? (v - a) / (b - a) * 1023.0 : 0.0;
+        q = q > 0.0 ? (q < 1023.0 ? q : 1023.0) : 0.0;  // also maps NaN to 0
+        code |= spread3((uint32_t)q) << k;
+    }
+    keys[i] = ((uint64_t)lo << 30) | code;
+    vals[i] = row;
+}
+
 template <class T>
 __global__ void k_zero(T *p, size_t n)
 {
@@ -392,11 +489,11 @@ void launch_sym(const JoinRunner &J, const Index &ix, const double *xr, uint32_t
     const unsigned wpb = WPB;
     if (work)
         LAUNCH((k_join_sym<DS, TAIL, true>), grid_for(J.nitems, wpb), WAVE * wpb, xr, d, ds, eps2,
-               ix.nstart.p, J.uoff.p, J.uadj.p, ix.perm.p, J.items.p, J.nitems, qbits, out,
+               ix.nstart.p, J.uoff.p, J.uadj.p, J.jperm.p, J.items.p, J.nitems, qbits, out,
                J.counter.p, cap, work);
     else
         LAUNCH((k_join_sym<DS, TAIL, false>), grid_for(J.nitems, wpb), WAVE * wpb, xr, d, ds,
-               eps2, ix.nstart.p, J.uoff.p, J.uadj.p, ix.perm.p, J.items.p, J.nitems, qbits, out,
+               eps2, ix.nstart.p, J.uoff.p, J.uadj.p, J.jperm.p, J.items.p, J.nitems, qbits, out,
                J.counter.p, cap, work);
 }
 
@@ -408,7 +505,7 @@ void dispatch_sym(const JoinRunner &J, const Index &ix, const double *xr, uint32
     switch (ds) {
 #define MSJ_DS(v)                                                                   \
     case v:                                                                         \
-        launch_sym<v, false>(J, ix, xr, d, ds, eps2, qbits, out, cap, work, s);     \
+        launch_sym<v, false>(J, ix, xr, d, ds, eps2, qbits, out, cap, work, s);       \
         return;
         MSJ_DS(4)
         MSJ_DS(8)
@@ -436,6 +533,25 @@ void gather_node_rows(const void *xs, double *xr, bool f32, const uint32_t *perm
     else
         LAUNCH(k_gather_rows<double>, grid_for(m, 256), 256, (const double *)xs, xr, perm, n, d,
                ds);
+}
+
+void JoinRunner::order(const Index &ix, const void *xs, bool f32, uint32_t n, uint32_t d,
+                       hipStream_t s)
+{
+    const uint32_t km = d < 3 ? d : 3u;
+    mm.ensure(2 * 3);
+    jperm.ensure(n);
+    jkeys.ensure(n);
+    if (f32) {
+        LAUNCH(k_col_range<float>, km, 256, (const float *)xs, n, mm.p);
+        LAUNCH(k_morton_keys<float>, grid_for(n, 256), 256, (const float *)xs, n, km, mm.p,
+               ix.nstart.p, ix.M, ix.perm.p, jkeys.p, jperm.p);
+    } else {
+        LAUNCH(k_col_range<double>, km, 256, (const double *)xs, n, mm.p);
+        LAUNCH(k_morton_keys<double>, grid_for(n, 256), 256, (const double *)xs, n, km, mm.p,
+               ix.nstart.p, ix.M, ix.perm.p, jkeys.p, jperm.p);
+    }
+    sorter.sort(jkeys.p, jperm.p, n, bits_for(ix.M ? ix.M - 1 : 0) + 30, s);
 }
 
 void JoinRunner::plan(const Index &ix, uint32_t n, uint32_t d, hipStream_t s)
