@@ -42,6 +42,8 @@ import msj  # noqa: E402
 
 METRIC = "self-join time & distance evals/s at fixed ε: FP64 vs MFMA-filter, 1 MI355X"
 FP64_PEAK_TFLOPS = 78.6
+BF16_PEAK_TFLOPS = 2500.0  # dense MFMA
+FORMS = {"plain": 0, "bf16": 1, "split": 2}
 
 CONFIGS = {
     "c1": dict(n=65536, d=2, kind="uniform", seed=1001, eps=0.01,
@@ -51,6 +53,13 @@ CONFIGS = {
                eps_hex="3fc1a4c7913e006e",
                workload="plain FP64 join, 2,000,000 x 18 Gaussian mixture + noise, "
                         "mean 64 neighbours"),
+    "c3": dict(n=515_345, d=90, kind="correlated", seed=1003, target=32, cal_seed=2003,
+               eps_hex="3fe8aa41cdaa80fa",
+               workload="FP64 join, 515,345 x 90 correlated Gaussian, mean 32 neighbours"),
+    "c4": dict(n=5_000_000, d=384, kind="clusters", seed=1004, target=16, cal_seed=2004,
+               cal_sample=2000, cal_f32=True, storage="f32",
+               workload="FP64 decisions on FP32-stored 5,000,000 x 384 Zipf clusters, "
+                        "mean 16 neighbours"),
 }
 
 # Runs in a child process: torch brings its own HIP runtime, which must not
@@ -61,9 +70,10 @@ import numpy as np
 sys.path.insert(0, sys.argv[1])
 import datagen
 x = np.load(sys.argv[2])
-eps = datagen.calibrate_eps(x, int(sys.argv[3]), sample=10000, seed=int(sys.argv[4]),
-                            device="cuda:0", log=lambda m: print("[bench] " + m, file=sys.stderr,
-                                                                   flush=True))
+f32 = bool(int(sys.argv[6]))
+eps = datagen.calibrate_eps(x, int(sys.argv[3]), sample=int(sys.argv[5]), seed=int(sys.argv[4]),
+                            device="cuda:0", f32=f32,
+                            log=lambda m: print("[bench] " + m, file=sys.stderr, flush=True))
 print(json.dumps({"eps": eps}))
 """
 
@@ -74,16 +84,17 @@ def calibrate(x, cfg):
         np.save(path, x)
         out = subprocess.run([sys.executable, "-c", _CALIBRATE,
                               os.path.join(ROOT, "self-join-mistic_amd"), path,
-                              str(cfg["target"]), str(cfg["cal_seed"])],
+                              str(cfg["target"]), str(cfg["cal_seed"]),
+                              str(cfg.get("cal_sample", 10000)), str(int(cfg.get("cal_f32", False)))],
                              stdout=subprocess.PIPE, check=True, text=True).stdout
     return json.loads(out.strip().splitlines()[-1])["eps"]
 
 
-def traffic_bytes(config):
+def traffic_bytes(config, form):
     """HBM bytes per launch of the distance kernel from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload, if any."""
     path = os.path.join(ROOT, "profiles", "k_join_sym_2Mx18_plain_traffic.json")
-    if config != "c2" or not os.path.exists(path):
+    if config != "c2" or form != 0 or not os.path.exists(path):
         return None
     with open(path) as f:
         return json.load(f)["traffic_bytes_per_launch"]
@@ -97,9 +108,13 @@ def make_data(cfg):
     n, d = cfg["n"], cfg["d"]
     if cfg["kind"] == "uniform":
         x = datagen.uniform(n, d, cfg["seed"])
-    else:
+    elif cfg["kind"] == "mixture":
         x = datagen.mixture(n, d, cfg["seed"])
-    order, _ = msj.dim_order(x)
+    elif cfg["kind"] == "correlated":
+        x = datagen.correlated(n, d, cfg["seed"])
+    else:
+        x = datagen.clusters(n, d, cfg["seed"])
+    order, _ = msj.dim_order(x.astype(np.float64, copy=False))
     return datagen.reorder_dims(x, order)
 
 
@@ -143,6 +158,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--form", default="plain", choices=sorted(FORMS))
     ap.add_argument("--eps", type=float, default=None, help="override the calibrated eps")
     ap.add_argument("--calibrate", action="store_true",
                     help="recompute eps from the seeded 10,000-query sample, print it, exit")
@@ -165,7 +181,7 @@ def main():
     elif "eps_hex" in cfg and not args.calibrate:
         eps = struct.unpack(">d", bytes.fromhex(cfg["eps_hex"]))[0]
     else:
-        log("[bench] calibrating eps on the GPU (10,000 seeded queries)")
+        log("[bench] calibrating eps on the GPU (%d seeded queries)" % cfg.get("cal_sample", 10000))
         eps = calibrate(x, cfg)
     eps_hex = struct.pack(">d", eps).hex()
     log("[bench] eps = %r (0x%s)" % (eps, eps_hex))
@@ -174,17 +190,25 @@ def main():
         return
 
     # counting pass (untimed): algorithmic FP64 work of the distance kernel
-    with msj.Context(output=msj.OUT_HOST, count_work=1) as cc:
+    storage = msj.STORAGE_F32 if cfg.get("storage") == "f32" else msj.STORAGE_F64
+    form = FORMS[args.form]
+    with msj.Context(output=msj.OUT_HOST, count_work=1, storage=storage, form=form) as cc:
         cc.load(x)
         ref_run = cc.join(eps)
     dims_eval = ref_run.stats["dims_evaluated"]
     gpu_rows = (ref_run.counts, ref_run.offsets, ref_run.nbrs, ref_run.node_order)
     log("[bench] counting pass: R=%d nodes=%d candidates=%d dims=%d" % (
         ref_run.total_pairs, ref_run.stats["num_nodes"], ref_run.stats["num_calcs"], dims_eval))
+    if form != 0:
+        rs = ref_run.stats
+        log("[bench] filter check: form_used=%d band=%d checked=%d violations=%d "
+            "max|err|=%.3g mean|err|=%.3g" % (rs["form_used"], rs["band_pairs"],
+                                              rs["filter_checked"], rs["filter_violations"],
+                                              rs["max_score_err"], rs["mean_score_err"]))
 
     # msj_ctx_join returns after its stream has drained: the clock below
     # brackets exactly the K joins.
-    with msj.Context(output=msj.OUT_DEVICE) as ctx:
+    with msj.Context(output=msj.OUT_DEVICE, storage=storage, form=form) as ctx:
         ctx.load(x)
         for w in range(args.warmup):
             r = ctx.join(eps, copy_arrays=False)
@@ -202,7 +226,17 @@ def main():
     R = r.total_pairs
     value = C / (ms_step / 1e3)
     k_ms = float(np.mean(kms))
-    flops = 3.0 * dims_eval
+    if st["form_used"] == 0:
+        flops = 3.0 * dims_eval
+        peak, kname = FP64_PEAK_TFLOPS, "k_join_sym (FP64 vector ALU; MI355X FP64 peak)"
+    else:
+        # useful matrix-core work: every unordered candidate pair (self pairs
+        # included) scored over d dimensions, 2 flops per product, 3 products
+        # per dimension for split-BF16
+        mult = 3.0 if st["form_used"] == 2 else 1.0
+        flops = (st["num_calcs"] + cfg["n"]) / 2.0 * cfg["d"] * 2.0 * mult
+        peak = BF16_PEAK_TFLOPS
+        kname = "k_mfma_join + k_band_check (BF16 MFMA; MI355X dense BF16 peak)"
     achieved = flops / (k_ms / 1e3) / 1e12
     log("[bench] %d steps: %.2f ms/step (index %.2f, join %.2f [kernel %.2f], output %.2f ms)"
         % (args.steps, ms_step, *np.mean(stage, axis=0)[:2], k_ms, np.mean(stage, axis=0)[2]))
@@ -210,21 +244,34 @@ def main():
     line = {
         "metric": METRIC, "value": value, "unit": "distance evals/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": {0: "f64", 1: "bf16 MFMA (f32 acc) + f64 band",
+                  2: "split-bf16 MFMA (f32 acc) + f64 band"}[st["form_used"]],
+        "mean_neighbours": R / cfg["n"] - 1.0,
         "data": "synthetic (seeded, generated in-process)",
         "config": {"workload": cfg["workload"], "n": cfg["n"], "d": cfg["d"], "eps": eps,
-                   "eps_hex": eps_hex, "form": "plain", "parallelism": "none (1 GPU)"},
+                   "eps_hex": eps_hex, "form": args.form, "parallelism": "none (1 GPU)"},
         "join_time_s": ms_step / 1e3, "pairs_per_s": R / (ms_step / 1e3), "total_pairs": R,
         "candidate_evals": C, "num_nodes": st["num_nodes"],
         "ms_index": float(np.mean([s[0] for s in stage])),
         "ms_join": float(np.mean([s[1] for s in stage])),
         "ms_output": float(np.mean([s[2] for s in stage])),
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                     "traffic": traffic_bytes(args.config),
-                     "kernel": "k_join_sym (FP64 vector ALU; MI355X FP64 peak)",
-                     "kernel_ms": k_ms, "algorithmic_flops": flops},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak,
+                     "unit": "TFLOP/s", "frac": achieved / peak,
+                     "traffic": traffic_bytes(args.config, st["form_used"]),
+                     "kernel": kname, "kernel_ms": k_ms, "algorithmic_flops": flops},
     }
+    if st["form_used"] != 0:
+        rs = ref_run.stats
+        line["filter"] = {
+            "band_pairs": rs["band_pairs"],
+            "band_fraction": rs["band_pairs"] / max(1.0, (C + cfg["n"]) / 2.0),
+            "max_score_err": rs["max_score_err"], "mean_score_err": rs["mean_score_err"],
+            "sampled_pairs_checked": rs["filter_checked"],
+            "violations": rs["filter_violations"],
+            "mfma_flops_issued": st["mfma_flops"],
+            "mfma_util": st["mfma_flops"] / (k_ms / 1e3) / 1e12 / BF16_PEAK_TFLOPS,
+        }
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle, %d threads, ~%.0f s)" % (args.cpu_threads,
                                                                   args.cpu_seconds))

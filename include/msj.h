@@ -106,6 +106,13 @@ typedef struct msj_stats {
     uint64_t band_pairs;         /* MFMA forms: pairs re-decided in FP64            */
     double   max_score_err;      /* MFMA forms: max |filter score - FP64 score|     */
     double   mean_score_err;     /* MFMA forms: mean of the same over scored pairs  */
+    uint64_t filter_checked;     /* MFMA forms with count_work: sampled candidate
+                                    pairs whose filter class was verified in FP64  */
+    uint64_t filter_violations;  /* of those, certain-accept/reject classes that
+                                    FP64 contradicts (0 unless the bound is wrong) */
+    uint32_t form_used;          /* msj_form the join actually ran                  */
+    uint32_t pad2;
+    double   mfma_flops;         /* MFMA forms: matrix-core flops issued (tiles)    */
 } msj_stats;
 
 typedef struct msj_result {

@@ -52,6 +52,7 @@ struct msj_ctx {
     msj::Index ix;
     msj::IndexBuilder builder;
     msj::JoinRunner join;
+    msj::MfmaRunner mfma;
     msj::DevBuf<uint32_t> r_counts;
     msj::DevBuf<uint64_t> r_offsets;
     msj::DevBuf<uint32_t> r_nbrs;
@@ -93,8 +94,6 @@ void validate_params(const msj_params &p)
         throw Error(MSJ_E_INVAL, "unknown output mode");
     if (p.form != MSJ_FORM_PLAIN && p.form != MSJ_FORM_MFMA_BF16 && p.form != MSJ_FORM_MFMA_SPLIT)
         throw Error(MSJ_E_INVAL, "unknown form");
-    if (p.form != MSJ_FORM_PLAIN)
-        throw Error(MSJ_E_UNSUPPORTED, "MFMA filter forms are not built in this version");
 }
 
 template <class F>
@@ -221,8 +220,12 @@ void do_join(msj_ctx *c, double eps, msj_result *out)
     c->join.order(c->ix, c->xs.p, c->f32, n, d, s);
     c->xr.ensure((size_t)n * msj::padded_dim(d));
     msj::gather_node_rows(c->xs.p, c->xr.p, c->f32, c->join.jperm.p, n, d, s);
-    c->join.plan(c->ix, n, d, s);
+    const bool use_mfma = c->prm.form != MSJ_FORM_PLAIN && msj::mfma_filter_applies(eps, n, d);
+    c->join.plan(c->ix, n, d, s, !use_mfma);
     MSJ_HIP(hipEventRecord(c->ev[1], s));
+    if (use_mfma)  // part of the join: filter rows, thresholds, work items
+        c->mfma.prepare(c->ix, c->join, c->xr.p, n, d, eps, c->prm.form == MSJ_FORM_MFMA_SPLIT,
+                        s);
 
     // ---- join (subsystem c)
     uint64_t cap = c->prm.page_pairs;
@@ -235,9 +238,19 @@ void do_join(msj_ctx *c, double eps, msj_result *out)
     }
     float kms = 0.f;
     uint64_t launches = 0, dims = 0;
-    const uint64_t R = c->join.run_plain(c->ix, c->xr.p, n, d, eps, cap, s, c->ev[2],
-                                         c->ev[3], &launches, &kms,
-                                         c->prm.count_work ? &dims : nullptr);
+    uint64_t R;
+    if (use_mfma) {
+        R = c->mfma.run(c->ix, c->join, c->xr.p, n, d, eps, cap, c->prm.count_work != 0, s,
+                        c->ev[2], c->ev[3], &launches, &kms, &st);
+        const uint32_t kp = (d + 31u) & ~31u;
+        st.mfma_flops = (double)c->mfma.tiles * 128.0 * 128.0 * 2.0 * kp *
+                        (c->prm.form == MSJ_FORM_MFMA_SPLIT ? 3.0 : 1.0);
+        st.form_used = c->prm.form;
+    } else {
+        R = c->join.run_plain(c->ix, c->xr.p, n, d, eps, cap, s, c->ev[2], c->ev[3], &launches,
+                              &kms, c->prm.count_work ? &dims : nullptr);
+        st.form_used = MSJ_FORM_PLAIN;
+    }
     st.dims_evaluated = dims;
     c->last_R = R;
     if (R >= 0xFFFFFFFFull * 16)

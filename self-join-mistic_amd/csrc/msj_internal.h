@@ -177,7 +177,8 @@ struct JoinRunner {
     // along a Morton curve of the first three columns (jperm).
     void order(const Index &ix, const void *xs, bool f32, uint32_t n, uint32_t d, hipStream_t s);
     // Upper neighbour lists and work items of the final node net.
-    void plan(const Index &ix, uint32_t n, uint32_t d, hipStream_t s);
+    // plain_items = false: upper lists only (the MFMA form has its own items).
+    void plan(const Index &ix, uint32_t n, uint32_t d, hipStream_t s, bool plain_items = true);
     // Runs the plain FP64 distance kernel on the node-ordered row-major FP64
     // copy xr; returns the number of pairs R.  Grows `pairs` and reruns when
     // R exceeds the capacity.  With dims_evaluated, a counting build of the
@@ -189,6 +190,56 @@ struct JoinRunner {
     void finish(uint32_t n, uint64_t R, uint32_t *counts, uint64_t *offsets, uint32_t *nbrs,
                 hipStream_t s);
 };
+
+// ------------------------------------------------------------ MFMA filter
+// One workgroup of the filter kernel: A rows [qbeg, qend) against B rows
+// [cbeg, cend) in the padded row space (nodes start on 32-row boundaries);
+// self != 0 when both ranges lie in one node (then only pairs p <= c).
+struct MItem {
+    uint32_t qbeg, qend, cbeg, cend, self;
+};
+
+// Precision form: pairs are scored on the matrix cores from BF16 (or
+// split-BF16) copies of the centred rows, FP32 accumulation; a rigorous
+// per-pair error bound turns eps^2 into a band, and only pairs inside the
+// band are decided again with the FP64 sequence of the plain form.
+struct MfmaRunner {
+    DevBuf<uint32_t> pstart;     // [M+1] padded row offset of each node
+    DevBuf<uint32_t> prow;       // [M]   roundup(size, 32)
+    DevBuf<uint32_t> pjoin;      // [P]   padded row -> join position (~0u: padding)
+    DevBuf<uint32_t> porig;      // [P]   padded row -> caller's row id
+    DevBuf<uint4> fh, fl;        // fragments of the BF16 high / low parts
+    DevBuf<float4> thr;          // [P]   per-row threshold terms
+    DevBuf<double> mu, colpart;  // centre (column means) and partial sums
+    DevBuf<uint64_t> p64;        // scan scratch
+    DevBuf<uint32_t> ecnt;       // per upper edge: work items
+    DevBuf<uint64_t> etiles;     // per upper edge: 128 x 128 tiles
+    DevBuf<uint64_t> eoff;       // exclusive scan of ecnt
+    DevBuf<uint32_t> esrc;       // upper edge -> source node
+    DevBuf<MItem> items;
+    DevBuf<uint64_t> band;       // band pairs (join positions, class bits)
+    DevBuf<float> bscore;        // their filter scores
+    DevBuf<unsigned long long> ctr;  // [0] pairs, [1] band, [2] err bits, [3] checked,
+                                     // [4] violations, [5] err sum (double bits)
+    uint32_t P = 0, KS = 0, nitems = 0;
+    uint64_t tiles = 0;          // 128 x 128 tiles of the last prepare()
+    bool split = false;
+    Scanner scan;
+    // Padded rows, centre, fragments and thresholds from the node-ordered
+    // FP64 rows xr (join order of `J`), and the work items over J's upper
+    // neighbour lists.
+    void prepare(const Index &ix, const JoinRunner &J, const double *xr, uint32_t n,
+                 uint32_t d, double eps, bool split_bf16, hipStream_t s);
+    // Filter + band recheck; writes the result keys into J.pairs (grown and
+    // rerun on overflow) and returns their number.
+    uint64_t run(const Index &ix, JoinRunner &J, const double *xr, uint32_t n, uint32_t d,
+                 double eps, uint64_t cap_hint, bool diag, hipStream_t s, hipEvent_t k0,
+                 hipEvent_t k1, uint64_t *launches, float *kernel_ms, msj_stats *st);
+    uint64_t last_band = 0;
+};
+
+// The filter's bounds hold for 2^-50 <= eps^2 <= 2^60, n < 2^30, d <= 4096.
+bool mfma_filter_applies(double eps, uint32_t n, uint32_t d);
 
 // Row width of the node-ordered FP64 copy: d rounded up to a multiple of 4.
 inline uint32_t padded_dim(uint32_t d) { return (d + 3u) & ~3u; }

@@ -554,7 +554,7 @@ void JoinRunner::order(const Index &ix, const void *xs, bool f32, uint32_t n, ui
     sorter.sort(jkeys.p, jperm.p, n, bits_for(ix.M ? ix.M - 1 : 0) + 30, s);
 }
 
-void JoinRunner::plan(const Index &ix, uint32_t n, uint32_t d, hipStream_t s)
+void JoinRunner::plan(const Index &ix, uint32_t n, uint32_t d, hipStream_t s, bool plain_items)
 {
     const uint32_t M = ix.M;
     const uint32_t ds = padded_dim(d);
@@ -572,6 +572,10 @@ void JoinRunner::plan(const Index &ix, uint32_t n, uint32_t d, hipStream_t s)
     MSJ_HIP(hipStreamSynchronize(s));
     uadj.ensure(std::max<uint32_t>(ue, 1));
     LAUNCH(k_upper_write, grid_for(M, 256), 256, ix.aoff.p, ix.adj.p, M, uoff.p, uadj.p);
+    if (!plain_items) {
+        nitems = 0;
+        return;
+    }
 
     LAUNCH(k_item_count, grid_for(M, 256), 256, ix.nstart.p, ulen.p, M, qmax, itcnt.p);
     scan.excl_u32_to_u64(itcnt.p, tot64.p, M, s);

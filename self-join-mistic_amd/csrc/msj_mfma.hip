@@ -1,0 +1,861 @@
+// msj_mfma.hip -- precision form of the distance-and-compare step: a Gram
+// filter on the matrix cores with an FP64 recheck of an error band.
+//
+// The decision it must reproduce is the plain form's (kernel.cu:833-865
+// cachedDistanceCheck, restated in msj_join.hip): with s the FP64 sum of the
+// chunked sequence, a pair is a result iff !(s > eps^2).  The partial sums of
+// that sequence never decrease (every step adds a non-negative rounded term),
+// so the early exit does not change the decision: it is !(s_full > eps^2).
+//
+// Filter.  Rows are centred on the column means mu (FP64), a' = RN(x - mu),
+// and stored as BF16 (h = bf16(a')) or split-BF16 (h and l = bf16(a' - h)).
+// For a candidate pair the matrix cores give G ~ a'.b' (FP32 accumulation of
+// the exact BF16 products; split: h.h + h.l + l.h), and
+//   D~ = |a'|^2 + |b'|^2 - 2 G
+// estimates the squared distance.  With u = 2^-8 (BF16 rounding, taken with
+// a 2^-15 allowance for rounding through FP32) and m products per pair:
+//   |P - a'.b'|  <= e_p |a'||b'|,  e_p = 2u + u^2           (BF16)
+//                                    e_p = 3u^2 + 2u^3 + 2u^4 (split-BF16)
+//   |G - P|      <= gamma * sum|products|, gamma = 2 m 2^-24 / (1 - 2 m 2^-24)
+// (P: the exact sum of the products; gamma allows every FP32 addition of the
+// accumulation, in any order, an error of two units in the last place), so
+//   |2G - 2a'.b'| <= e_g (|a'|^2 + |b'|^2),  e_g = e_p + gamma (1+u)^2 (1+2u).
+// Per row the kernel keeps alpha = RU((1+C)|a'|^2) and alpha' = RD((1-C)|a'|^2)
+// in FP32, C = 1.01 (e_g + 2^-24) + 2^-40, and decides
+//   certain accept:  fma(-2, G, alpha_p + alpha_c)  <= lo
+//   certain reject:  fma(-2, G, alpha'_p + alpha'_c) > hi
+//   otherwise:       band -> FP64 sequence,
+// lo = RD(eps^2 / ((1+2^-23)(1+g)) (1-2^-30) - 2^-58),
+// hi = RU(eps^2 / ((1-2^-24)(1-g)) (1+2^-30) + 2^-58),
+// g = (ds + 4) 2^-52 bounding the FP64 sequence's relative error against the
+// exact squared distance, 2^-58 covering FP32 underflow of tiny products.
+// C absorbs the FP32 roundings of the two threshold sums, the FP64 norm and
+// the centring error (|(a'-b') - (x_a - x_b)| <= 2^-53 (|a'|+|b'|)(1+2^-53)).
+// Rows whose |a'|^2 exceeds 2^100 (or is not finite) get NaN terms, so every
+// pair with them falls in the band; the filter is used for
+// 2^-50 <= eps^2 <= 2^60 only (msj_api.hip runs the plain form otherwise).
+//
+// Layout.  Nodes are padded to 32-row boundaries (padded row space P).  The
+// BF16 planes are stored in the operand order of V_MFMA_F32_32X32X16_BF16:
+// for row block rb (32 rows) and k-step ks (16 dims), 64 lanes x 16 bytes,
+// lane = row%32 + 32*(k%16 / 8), 8 consecutive dims per lane.  A tile slab is
+// thus a linear copy into LDS and a linear, conflict-free read from it, and
+// the same array serves as operand A (query rows) and operand B (candidates).
+//
+// Work.  A workgroup (4 waves) owns 128 A rows of node X and walks up to
+// CBT tiles of 128 B rows of a node Y >= X of X's upper neighbour list
+// (inside X only tiles on or above the diagonal, pairs p <= c); each wave
+// computes a 64 x 64 block as 2 x 2 MFMA 32x32 tiles.  K slabs of 32 dims are
+// double-staged: the next slab's global loads are in flight while the
+// current slab runs on the matrix cores.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "msj_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace msj {
+
+namespace {
+
+constexpr uint32_t TM = 128;     // rows per workgroup tile, A and B
+constexpr uint32_t KSL = 2;      // 16-dim k-steps per LDS slab
+constexpr uint32_t CBT = 16;     // B tiles per work item
+constexpr uint32_t MHITS = 256;  // per-wave LDS buffer of result keys
+constexpr uint32_t KALIGN = 16 * KSL;
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+__device__ inline bf16x8 as_bf16x8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
+
+// FP32 -> BF16, round to nearest even (NaN kept quiet).
+__device__ inline uint32_t bf16_bits(float f)
+{
+    uint32_t b = __float_as_uint(f);
+    if ((b & 0x7FFFFFFFu) > 0x7F800000u)
+        return (b >> 16) | 0x40u;
+    return (b + 0x7FFFu + ((b >> 16) & 1u)) >> 16;
+}
+__device__ inline float bf16_value(uint32_t h) { return __uint_as_float(h << 16); }
+
+// ------------------------------------------------------------ preparation
+__global__ __launch_bounds__(256) void k_prow(const uint32_t *__restrict__ nstart, uint32_t M,
+                                              uint32_t *__restrict__ prow)
+{
+    uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < M)
+        prow[x] = (nstart[x + 1] - nstart[x] + 31u) & ~31u;
+}
+
+__global__ __launch_bounds__(256) void k_u64_to_u32(const uint64_t *__restrict__ a, size_t n,
+                                                    uint32_t *__restrict__ b)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        b[i] = (uint32_t)a[i];
+}
+
+__global__ __launch_bounds__(256) void k_fill_u32(uint32_t *__restrict__ p, size_t n, uint32_t v)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = v;
+}
+
+// Join position i (node order, Morton order inside a node) -> padded row.
+__global__ __launch_bounds__(256) void k_pad_rows(const uint32_t *__restrict__ nstart, uint32_t M,
+                                                  const uint32_t *__restrict__ pstart,
+                                                  const uint32_t *__restrict__ jperm, uint32_t n,
+                                                  uint32_t *__restrict__ pjoin,
+                                                  uint32_t *__restrict__ porig)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    uint32_t lo = 0, hi = M;
+    while (hi - lo > 1) {
+        uint32_t mid = lo + (hi - lo) / 2;
+        if (nstart[mid] <= i)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const uint32_t r = pstart[lo] + (i - nstart[lo]);
+    pjoin[r] = i;
+    porig[r] = jperm[i];
+}
+
+// Column sums of xr over row blocks (deterministic order), then the means.
+__global__ __launch_bounds__(256) void k_col_partial(const double *__restrict__ xr, uint32_t n,
+                                                     uint32_t ds, uint32_t rows_per,
+                                                     double *__restrict__ part)
+{
+    const uint32_t b = blockIdx.x;
+    const uint32_t r0 = b * rows_per, r1 = min(n, r0 + rows_per);
+    for (uint32_t j = threadIdx.x; j < ds; j += blockDim.x) {
+        double acc = 0.0;
+        for (uint32_t r = r0; r < r1; ++r)
+            acc += xr[(size_t)r * ds + j];
+        part[(size_t)b * ds + j] = acc;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_col_final(const double *__restrict__ part, uint32_t nb,
+                                                   uint32_t ds, uint32_t n,
+                                                   double *__restrict__ mu)
+{
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ds)
+        return;
+    double acc = 0.0;
+    for (uint32_t b = 0; b < nb; ++b)
+        acc += part[(size_t)b * ds + j];
+    mu[j] = acc / (double)n;
+}
+
+// One wave per padded row: centred BF16 (and low BF16) fragments, and the
+// row's threshold terms (alpha, alpha', |a'|^2 as FP32, 0).
+template <bool SPLIT>
+__global__ __launch_bounds__(256) void k_frag_prep(const double *__restrict__ xr, uint32_t ds,
+                                                   uint32_t KS, const double *__restrict__ mu,
+                                                   const uint32_t *__restrict__ pjoin,
+                                                   uint32_t Ptot, double C,
+                                                   uint4 *__restrict__ fh, uint4 *__restrict__ fl,
+                                                   float4 *__restrict__ thr)
+{
+    const uint32_t r = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    const uint32_t lane = threadIdx.x & (WAVE - 1);
+    if (r >= Ptot)
+        return;
+    const uint32_t j = pjoin[r];
+    const bool valid = j != ~0u;
+    const uint32_t groups = KS * 2;  // 8-dim groups
+    double na = 0.0;
+    for (uint32_t g = lane; g < groups; g += WAVE) {
+        uint32_t hw[4] = {0, 0, 0, 0}, lw[4] = {0, 0, 0, 0};
+        if (valid) {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const uint32_t k = g * 8 + e;
+                double a = 0.0;
+                if (k < ds)
+                    a = xr[(size_t)j * ds + k] - mu[k];
+                na = fma(a, a, na);
+                const uint32_t hb = bf16_bits((float)a);
+                const double rem = a - (double)bf16_value(hb);  // exact (Sterbenz)
+                const uint32_t lb = bf16_bits((float)rem);
+                hw[e / 2] |= hb << (16 * (e & 1));
+                lw[e / 2] |= lb << (16 * (e & 1));
+            }
+        }
+        const size_t idx = ((size_t)(r / 32) * KS + g / 2) * WAVE + (r % 32) + 32 * (g & 1);
+        fh[idx] = make_uint4(hw[0], hw[1], hw[2], hw[3]);
+        if (SPLIT)
+            fl[idx] = make_uint4(lw[0], lw[1], lw[2], lw[3]);
+    }
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1)
+        na += __shfl_xor(na, o, WAVE);
+    if (lane == 0) {
+        float4 t = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (valid) {
+            if (na <= 0x1p100) {
+                t.x = __double2float_ru((1.0 + C) * na * (1.0 + 0x1p-40));
+                t.y = __double2float_rd((1.0 - C) * na * (1.0 - 0x1p-40));
+                t.z = (float)na;
+            } else {
+                t.x = t.y = t.z = __builtin_nanf("");
+            }
+        }
+        thr[r] = t;
+    }
+}
+
+// ------------------------------------------------------------- work items
+__global__ __launch_bounds__(256) void k_edge_src(const uint32_t *__restrict__ uoff, uint32_t M,
+                                                  uint32_t *__restrict__ esrc)
+{
+    uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= M)
+        return;
+    for (uint32_t e = uoff[x]; e < uoff[x + 1]; ++e)
+        esrc[e] = x;
+}
+
+// sum_{j=1..m} ceil(j / k)
+__host__ __device__ inline uint64_t tri_chunks(uint64_t m, uint64_t k)
+{
+    const uint64_t q = m / k, r = m % k;
+    return k * q * (q + 1) / 2 + r * (q + 1);
+}
+
+__global__ __launch_bounds__(256) void k_mitem_count(const uint32_t *__restrict__ nstart,
+                                                     const uint32_t *__restrict__ esrc,
+                                                     const uint32_t *__restrict__ uadj,
+                                                     uint32_t E, uint32_t *__restrict__ cnt,
+                                                     uint64_t *__restrict__ tiles)
+{
+    uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E)
+        return;
+    const uint32_t X = esrc[e], Y = uadj[e];
+    const uint64_t TA = (nstart[X + 1] - nstart[X] + TM - 1) / TM;
+    uint64_t c, t;
+    if (X == Y) {
+        c = tri_chunks(TA, CBT);
+        t = TA * (TA + 1) / 2;
+    } else {
+        const uint64_t TB = (nstart[Y + 1] - nstart[Y] + TM - 1) / TM;
+        c = TA * ((TB + CBT - 1) / CBT);
+        t = TA * TB;
+    }
+    cnt[e] = (uint32_t)c;
+    tiles[e] = t;
+}
+
+__global__ __launch_bounds__(256) void k_mitem_fill(const uint32_t *__restrict__ nstart,
+                                                    const uint32_t *__restrict__ pstart,
+                                                    const uint32_t *__restrict__ esrc,
+                                                    const uint32_t *__restrict__ uadj,
+                                                    const uint64_t *__restrict__ eoff, uint32_t E,
+                                                    uint32_t nitems, MItem *__restrict__ items)
+{
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nitems)
+        return;
+    uint32_t lo = 0, hi = E;  // edge with eoff[e] <= k < eoff[e+1]
+    while (hi - lo > 1) {
+        uint32_t mid = lo + (hi - lo) / 2;
+        if (eoff[mid] <= k)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const uint32_t e = lo;
+    const uint64_t l = k - eoff[e];
+    const uint32_t X = esrc[e], Y = uadj[e];
+    const uint32_t sx = nstart[X + 1] - nstart[X], sy = nstart[Y + 1] - nstart[Y];
+    const uint64_t TA = (sx + TM - 1) / TM;
+    uint64_t a, t0, t1;
+    if (X == Y) {
+        // a = largest tile row with F(a) = tri(TA) - tri(TA - a) <= l
+        const uint64_t tot = tri_chunks(TA, CBT);
+        uint64_t alo = 0, ahi = TA;
+        while (ahi - alo > 1) {
+            uint64_t mid = alo + (ahi - alo) / 2;
+            if (tot - tri_chunks(TA - mid, CBT) <= l)
+                alo = mid;
+            else
+                ahi = mid;
+        }
+        a = alo;
+        const uint64_t c = l - (tot - tri_chunks(TA - a, CBT));
+        t0 = a + c * CBT;
+        t1 = std::min<uint64_t>(TA, t0 + CBT);
+    } else {
+        const uint64_t TB = (sy + TM - 1) / TM;
+        const uint64_t nc = (TB + CBT - 1) / CBT;
+        a = l / nc;
+        t0 = (l % nc) * CBT;
+        t1 = std::min<uint64_t>(TB, t0 + CBT);
+    }
+    MItem w;
+    w.qbeg = pstart[X] + (uint32_t)(a * TM);
+    w.qend = pstart[X] + sx;
+    w.cbeg = pstart[Y] + (uint32_t)(t0 * TM);
+    w.cend = pstart[Y] + std::min<uint32_t>(sy, (uint32_t)(t1 * TM));
+    w.self = X == Y ? 1u : 0u;
+    items[k] = w;
+}
+
+// --------------------------------------------------------------- the filter
+// ctr: [0] result keys, [1] band entries, [2] max error (double bits),
+// [3] sampled pairs checked, [4] violations, [5] error sum (double),
+// [6] pairs decided in FP64.
+template <bool SPLIT, bool DIAG>
+__global__ __launch_bounds__(256) void k_mfma_join(
+    const uint4 *__restrict__ fh, const uint4 *__restrict__ fl, uint32_t KS,
+    const float4 *__restrict__ thr, const uint32_t *__restrict__ porig,
+    const uint32_t *__restrict__ pjoin, const MItem *__restrict__ items, uint32_t nitems,
+    uint32_t per_xcd, float lo, float hi, int qbits, unsigned long long *__restrict__ out,
+    unsigned long long cap, unsigned long long *__restrict__ ctr,
+    unsigned long long *__restrict__ band, float *__restrict__ bscore, unsigned long long bcap,
+    uint32_t sample_every)
+{
+    constexpr uint32_t NP = SPLIT ? 2 : 1;
+    constexpr uint32_t SLAB = 4 * KSL * WAVE;  // uint4 per plane per operand
+    constexpr uint32_t PER_T = SLAB / 256;     // per thread
+    __shared__ uint4 sA[NP][SLAB];
+    __shared__ uint4 sB[NP][SLAB];
+    __shared__ float4 tA[TM], tB[TM];
+    __shared__ uint32_t oA[TM], oB[TM], jA[TM], jB[TM];
+    __shared__ unsigned long long hsh[4][MHITS];
+
+    const uint32_t tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid / WAVE;
+    const uint32_t wr = wv >> 1, wc = wv & 1;
+    // consecutive items (same A rows, neighbouring B rows) on one XCD: the
+    // hardware deals workgroups out to the 8 XCDs round robin
+    const uint32_t bid = blockIdx.x;
+    const uint32_t it = (bid % 8u) * per_xcd + bid / 8u;
+    if (it >= nitems)
+        return;  // uniform over the workgroup
+    const MItem w = items[it];
+    const bool sampled = DIAG && (it % sample_every == 0);
+    if (tid < TM) {
+        const uint32_t r = w.qbeg + tid;
+        tA[tid] = thr[r];
+        oA[tid] = porig[r];
+        jA[tid] = pjoin[r];
+    }
+    const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (WAVE - lane));
+    unsigned long long *hb = hsh[wv];
+    uint32_t fill = 0;
+    auto flush = [&]() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        unsigned long long base = 0;
+        if (lane == 0)
+            base = atomicAdd(&ctr[0], (unsigned long long)fill);
+        base = __shfl(base, 0, WAVE);
+        for (uint32_t e = lane; e < fill; e += WAVE)
+            if (base + e < cap)
+                out[base + e] = hb[e];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        fill = 0;
+    };
+
+    const uint32_t rbA = w.qbeg / 32;
+    for (uint32_t cb = w.cbeg; cb < w.cend; cb += TM) {
+        const uint32_t rbB = cb / 32;
+        uint4 ra[NP][PER_T], rb[NP][PER_T];
+        auto gload = [&](uint32_t k0) {
+#pragma unroll
+            for (uint32_t q = 0; q < PER_T; ++q) {
+                const uint32_t e = tid + q * 256;
+                const uint32_t rbl = e / (KSL * WAVE), rem = e % (KSL * WAVE);
+                const size_t ga = ((size_t)(rbA + rbl) * KS + k0) * WAVE + rem;
+                const size_t gb = ((size_t)(rbB + rbl) * KS + k0) * WAVE + rem;
+                ra[0][q] = fh[ga];
+                rb[0][q] = fh[gb];
+                if (SPLIT) {
+                    ra[NP - 1][q] = fl[ga];
+                    rb[NP - 1][q] = fl[gb];
+                }
+            }
+        };
+        gload(0);
+        __syncthreads();  // the previous tile's epilogue is done with tB
+        if (tid < TM) {
+            const uint32_t r = cb + tid;
+            tB[tid] = thr[r];
+            oB[tid] = porig[r];
+            jB[tid] = pjoin[r];
+        }
+        floatx16 acc[2][2];
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+                for (int v = 0; v < 16; ++v)
+                    acc[bi][bj][v] = 0.f;
+        for (uint32_t k0 = 0; k0 < KS; k0 += KSL) {
+            __syncthreads();  // all waves are done reading the previous slab
+#pragma unroll
+            for (uint32_t q = 0; q < PER_T; ++q) {
+#pragma unroll
+                for (uint32_t pl = 0; pl < NP; ++pl) {
+                    sA[pl][tid + q * 256] = ra[pl][q];
+                    sB[pl][tid + q * 256] = rb[pl][q];
+                }
+            }
+            __syncthreads();
+            if (k0 + KSL < KS)
+                gload(k0 + KSL);  // in flight during this slab's MFMAs
+#pragma unroll
+            for (uint32_t kk = 0; kk < KSL; ++kk) {
+                bf16x8 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+                for (int bi = 0; bi < 2; ++bi) {
+                    const uint32_t o = ((2 * wr + bi) * KSL + kk) * WAVE + lane;
+                    ah[bi] = as_bf16x8(sA[0][o]);
+                    if (SPLIT)
+                        al[bi] = as_bf16x8(sA[NP - 1][o]);
+                }
+#pragma unroll
+                for (int bj = 0; bj < 2; ++bj) {
+                    const uint32_t o = ((2 * wc + bj) * KSL + kk) * WAVE + lane;
+                    bh[bj] = as_bf16x8(sB[0][o]);
+                    if (SPLIT)
+                        bl[bj] = as_bf16x8(sB[NP - 1][o]);
+                }
+#pragma unroll
+                for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                    for (int bj = 0; bj < 2; ++bj) {
+                        acc[bi][bj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                            ah[bi], bh[bj], acc[bi][bj], 0, 0, 0);
+                        if (SPLIT) {
+                            acc[bi][bj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                                ah[bi], bl[bj], acc[bi][bj], 0, 0, 0);
+                            acc[bi][bj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                                al[bi], bh[bj], acc[bi][bj], 0, 0, 0);
+                        }
+                    }
+            }
+        }
+
+        // ---- epilogue: C[i][j] of tile (bi, bj) sits in lane j + 32*h,
+        // element v, with i = 8*(v/4) + 4*h + v%4 (h = lane / 32)
+        const uint32_t h = lane / 32;
+        uint32_t cidx[2];
+        bool cv[2];
+        float4 tb[2];
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) {
+            const uint32_t jl = 64 * wc + 32 * bj + (lane & 31);
+            cidx[bj] = jl;
+            cv[bj] = cb + jl < w.cend;
+            tb[bj] = tB[jl];
+        }
+        bool need = false;
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const uint32_t il = 64 * wr + 32 * bi + 8 * (v / 4) + 4 * h + (v % 4);
+                const uint32_t p = w.qbeg + il;
+                const float ay = tA[il].y;
+#pragma unroll
+                for (int bj = 0; bj < 2; ++bj) {
+                    const uint32_t c = cb + cidx[bj];
+                    const bool valid = p < w.qend && cv[bj] && (!w.self || p <= c);
+                    const bool rej = fmaf(-2.f, acc[bi][bj][v], ay + tb[bj].y) > hi;
+                    need = need || (valid && (sampled || !rej));
+                }
+            }
+        if (!__builtin_amdgcn_ballot_w64(need))
+            continue;
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const uint32_t il = 64 * wr + 32 * bi + 8 * (v / 4) + 4 * h + (v % 4);
+                const uint32_t p = w.qbeg + il;
+                const float4 ta = tA[il];
+#pragma unroll
+                for (int bj = 0; bj < 2; ++bj) {
+                    const uint32_t jl = cidx[bj];
+                    const uint32_t c = cb + jl;
+                    const bool valid = p < w.qend && cv[bj] && (!w.self || p <= c);
+                    const float G = acc[bi][bj][v];
+                    const bool acc_ok = valid && fmaf(-2.f, G, ta.x + tb[bj].x) <= lo;
+                    const bool rej = fmaf(-2.f, G, ta.y + tb[bj].y) > hi;
+                    const bool inband = valid && !acc_ok && !rej;
+                    const uint64_t m1 = __builtin_amdgcn_ballot_w64(acc_ok);
+                    if (m1) {
+                        const bool two = acc_ok && p != c;
+                        const uint64_t m2 = __builtin_amdgcn_ballot_w64(two);
+                        const uint32_t tot = (uint32_t)(__popcll(m1) + __popcll(m2));
+                        if (fill + tot > MHITS)
+                            flush();
+                        if (acc_ok) {
+                            const unsigned long long po = oA[il], co = oB[jl];
+                            const uint32_t idx =
+                                fill + (uint32_t)(__popcll(m1 & lt) + __popcll(m2 & lt));
+                            hb[idx] = (po << qbits) | co;
+                            if (two)
+                                hb[idx + 1] = (co << qbits) | po;
+                        }
+                        fill += tot;
+                    }
+                    // band entries (and, when sampling, every valid pair)
+                    const bool push = sampled ? valid : inband;
+                    const uint64_t mb = __builtin_amdgcn_ballot_w64(push);
+                    if (mb) {
+                        unsigned long long base = 0;
+                        if (lane == 0)
+                            base = atomicAdd(&ctr[1], (unsigned long long)__popcll(mb));
+                        base = __shfl(base, 0, WAVE);
+                        if (push) {
+                            const unsigned long long idx = base + (unsigned long long)__popcll(mb & lt);
+                            if (idx < bcap) {
+                                const uint32_t cls =
+                                    sampled ? (acc_ok ? 1u : (inband ? 3u : 2u)) : 0u;
+                                band[idx] = ((unsigned long long)jA[il] << 32) |
+                                            ((unsigned long long)cls << 30) | jB[jl];
+                                bscore[idx] = fmaf(-2.f, G, ta.z + tb[bj].z);
+                            }
+                        }
+                    }
+                }
+            }
+    }
+    if (fill)
+        flush();
+}
+
+// FP64 decision of the band: the plain form's sequence over the whole padded
+// row (same decision as with early exit; the full sum is the score that the
+// filter's estimate is compared with).
+template <bool DIAG>
+__global__ __launch_bounds__(256) void k_band_check(
+    const double *__restrict__ xr, uint32_t ds, double eps2, const uint32_t *__restrict__ jperm,
+    const unsigned long long *__restrict__ band, const float *__restrict__ bscore,
+    unsigned long long bcap, int qbits, unsigned long long *__restrict__ out,
+    unsigned long long cap, unsigned long long *__restrict__ ctr)
+{
+    const unsigned long long nb = min(ctr[1], bcap);
+    const uint32_t lane = threadIdx.x & (WAVE - 1);
+    const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (WAVE - lane));
+    double emax = 0.0, esum = 0.0;
+    unsigned long long checked = 0, viol = 0, decided = 0;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    // whole waves iterate together so the ballots below see every lane
+    for (size_t base = (size_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(WAVE - 1)); base < nb;
+         base += stride) {
+        const size_t i = base + lane;
+        const bool have = i < nb;
+        bool emit = false, two = false;
+        unsigned long long ko = 0, kr = 0;
+        if (have) {
+            const unsigned long long e = band[i];
+            const uint32_t ja = (uint32_t)(e >> 32), jb = (uint32_t)e & 0x3FFFFFFFu;
+            const uint32_t cls = ((uint32_t)e >> 30) & 3u;
+            const double *qa = xr + (size_t)ja * ds;
+            const double *qb = xr + (size_t)jb * ds;
+            double s = 0.0;
+            for (uint32_t l = 0; l < ds; l += 4) {
+                double t0 = qa[l] - qb[l];
+                s = fma(t0, t0, s);
+                double t1 = qa[l + 1] - qb[l + 1];
+                double q1 = t1 * t1;
+                s = s + q1;
+                double t2 = qa[l + 2] - qb[l + 2];
+                double q2 = t2 * t2;
+                s = s + q2;
+                double t3 = qa[l + 3] - qb[l + 3];
+                double q3 = t3 * t3;
+                s = s + q3;
+            }
+            const bool ok = !(s > eps2);
+            const double err = fabs((double)bscore[i] - s);
+            if (err == err) {
+                emax = err > emax ? err : emax;
+                esum += err;
+            }
+            if (cls == 0 || cls == 3) {
+                ++decided;
+                emit = ok;
+                two = ok && ja != jb;
+                ko = (unsigned long long)jperm[ja];
+                kr = (unsigned long long)jperm[jb];
+            }
+            if (DIAG && cls != 0) {
+                ++checked;
+                if ((cls == 1 && !ok) || (cls == 2 && ok))
+                    ++viol;
+            }
+        }
+        const uint64_t m1 = __builtin_amdgcn_ballot_w64(emit);
+        if (m1) {
+            const uint64_t m2 = __builtin_amdgcn_ballot_w64(two);
+            unsigned long long b0 = 0;
+            if (lane == 0)
+                b0 = atomicAdd(&ctr[0], (unsigned long long)(__popcll(m1) + __popcll(m2)));
+            b0 = __shfl(b0, 0, WAVE);
+            if (emit) {
+                const unsigned long long o = b0 + __popcll(m1 & lt) + __popcll(m2 & lt);
+                if (o < cap)
+                    out[o] = (ko << qbits) | kr;
+                if (two && o + 1 < cap)
+                    out[o + 1] = (kr << qbits) | ko;
+            }
+        }
+    }
+    // wave reductions, one atomic each
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) {
+        double t = __shfl_xor(emax, o, WAVE);
+        emax = t > emax ? t : emax;
+        esum += __shfl_xor(esum, o, WAVE);
+        checked += __shfl_xor(checked, o, WAVE);
+        viol += __shfl_xor(viol, o, WAVE);
+        decided += __shfl_xor(decided, o, WAVE);
+    }
+    if (lane == 0) {
+        if (emax > 0.0)
+            atomicMax(&ctr[2], (unsigned long long)__double_as_longlong(emax));
+        if (esum > 0.0)
+            atomicAdd((double *)&ctr[5], esum);
+        if (checked)
+            atomicAdd(&ctr[3], checked);
+        if (viol)
+            atomicAdd(&ctr[4], viol);
+        if (decided)
+            atomicAdd(&ctr[6], decided);
+    }
+}
+
+__global__ void k_zero_ctr(unsigned long long *c, uint32_t n)
+{
+    if (threadIdx.x < n)
+        c[threadIdx.x] = 0ull;
+}
+
+#define LAUNCH(kern, grid, block, ...)                                                   \
+    do {                                                                                 \
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, s, __VA_ARGS__);            \
+        MSJ_HIP(hipGetLastError());                                                      \
+    } while (0)
+
+// Bound constant C of the threshold terms (see the file comment).
+double bound_c(uint32_t d, bool split)
+{
+    const double u = 0x1p-8 * (1.0 + 0x1p-15);
+    const double ep = split ? (3 * u * u + 2 * u * u * u + 2 * u * u * u * u) : (2 * u + u * u);
+    const double m = split ? 3.0 * d : (double)d;
+    const double t = 2.0 * m * 0x1p-24;
+    const double gam = t / (1.0 - t);
+    const double eg = ep + gam * (1 + u) * (1 + u) * (1 + 2 * u);
+    return 1.01 * (eg + 0x1p-24) + 0x1p-40;
+}
+
+float round_down_f(double v)
+{
+    float f = (float)v;
+    if ((double)f > v)
+        f = std::nextafter(f, -INFINITY);
+    return f;
+}
+float round_up_f(double v)
+{
+    float f = (float)v;
+    if ((double)f < v)
+        f = std::nextafter(f, INFINITY);
+    return f;
+}
+
+}  // namespace
+
+bool mfma_filter_applies(double eps, uint32_t n, uint32_t d)
+{
+    const double e2 = eps * eps;
+    return e2 >= 0x1p-50 && e2 <= 0x1p60 && n < (1u << 30) && d <= 4096;
+}
+
+void MfmaRunner::prepare(const Index &ix, const JoinRunner &J, const double *xr, uint32_t n,
+                         uint32_t d, double eps, bool split_bf16, hipStream_t s)
+{
+    (void)eps;
+    split = split_bf16;
+    const uint32_t M = ix.M;
+    const uint32_t ds = padded_dim(d);
+    const uint32_t Kp = (d + KALIGN - 1) / KALIGN * KALIGN;
+    KS = Kp / 16;
+
+    // padded row space
+    prow.ensure(M);
+    p64.ensure((size_t)M + 1);
+    LAUNCH(k_prow, grid_for(M, 256), 256, ix.nstart.p, M, prow.p);
+    scan.excl_u32_to_u64(prow.p, p64.p, M, s);
+    uint64_t Pt = 0;
+    MSJ_HIP(hipMemcpyAsync(&Pt, p64.p + M, sizeof(Pt), hipMemcpyDeviceToHost, s));
+    MSJ_HIP(hipStreamSynchronize(s));
+    if (Pt + TM >= (1ull << 31))
+        throw Error(MSJ_E_OVERFLOW, "padded rows exceed 2^31");
+    P = (uint32_t)Pt;
+    const uint32_t Ptot = P + TM;
+    pstart.ensure((size_t)M + 1);
+    LAUNCH(k_u64_to_u32, grid_for((size_t)M + 1, 256), 256, p64.p, (size_t)M + 1, pstart.p);
+    pjoin.ensure(Ptot);
+    porig.ensure(Ptot);
+    LAUNCH(k_fill_u32, grid_for(Ptot, 256), 256, pjoin.p, (size_t)Ptot, ~0u);
+    LAUNCH(k_fill_u32, grid_for(Ptot, 256), 256, porig.p, (size_t)Ptot, 0u);
+    LAUNCH(k_pad_rows, grid_for(n, 256), 256, ix.nstart.p, M, pstart.p, J.jperm.p, n, pjoin.p,
+           porig.p);
+
+    // centre
+    const uint32_t nb = std::min<uint32_t>(1024, std::max<uint32_t>(1, (n + 255) / 256));
+    const uint32_t rows_per = (n + nb - 1) / nb;
+    colpart.ensure((size_t)nb * ds);
+    mu.ensure(ds);
+    LAUNCH(k_col_partial, nb, 256, xr, n, ds, rows_per, colpart.p);
+    LAUNCH(k_col_final, grid_for(ds, 256), 256, colpart.p, nb, ds, n, mu.p);
+
+    // fragments and threshold terms
+    const double C = bound_c(d, split);
+    const size_t nfrag = (size_t)(Ptot / 32) * KS * WAVE;
+    fh.ensure(nfrag);
+    if (split)
+        fl.ensure(nfrag);
+    thr.ensure(Ptot);
+    if (split)
+        LAUNCH(k_frag_prep<true>, grid_for(Ptot, 4), 256, xr, ds, KS, mu.p, pjoin.p, Ptot, C,
+               fh.p, fl.p, thr.p);
+    else
+        LAUNCH(k_frag_prep<false>, grid_for(Ptot, 4), 256, xr, ds, KS, mu.p, pjoin.p, Ptot, C,
+               fh.p, fh.p, thr.p);
+
+    // work items over the upper neighbour lists
+    uint32_t E = 0;
+    MSJ_HIP(hipMemcpyAsync(&E, J.uoff.p + M, sizeof(E), hipMemcpyDeviceToHost, s));
+    MSJ_HIP(hipStreamSynchronize(s));
+    esrc.ensure(std::max<uint32_t>(E, 1));
+    ecnt.ensure(std::max<uint32_t>(E, 1));
+    etiles.ensure((size_t)E + 1);
+    eoff.ensure((size_t)E + 1);
+    LAUNCH(k_edge_src, grid_for(M, 256), 256, J.uoff.p, M, esrc.p);
+    LAUNCH(k_mitem_count, grid_for(E, 256), 256, ix.nstart.p, esrc.p, J.uadj.p, E, ecnt.p,
+           etiles.p);
+    scan.excl_u32_to_u64(ecnt.p, eoff.p, E, s);
+    p64.ensure((size_t)E + 1);  // pstart is converted already; contents not needed
+    scan.excl_u64_to_u64(etiles.p, p64.p, E, s);
+    uint64_t total = 0;
+    MSJ_HIP(hipMemcpyAsync(&total, eoff.p + E, sizeof(total), hipMemcpyDeviceToHost, s));
+    MSJ_HIP(hipMemcpyAsync(&tiles, p64.p + E, sizeof(tiles), hipMemcpyDeviceToHost, s));
+    MSJ_HIP(hipStreamSynchronize(s));
+    if (total >= 0xFFFFFFF0ull)
+        throw Error(MSJ_E_OVERFLOW, "filter work items exceed 2^32");
+    nitems = (uint32_t)total;
+    items.ensure(std::max<uint32_t>(nitems, 1));
+    if (nitems)
+        LAUNCH(k_mitem_fill, grid_for(nitems, 256), 256, ix.nstart.p, pstart.p, esrc.p, J.uadj.p,
+               eoff.p, E, nitems, items.p);
+}
+
+uint64_t MfmaRunner::run(const Index &ix, JoinRunner &J, const double *xr, uint32_t n, uint32_t d,
+                         double eps, uint64_t cap_hint, bool diag, hipStream_t s, hipEvent_t k0,
+                         hipEvent_t k1, uint64_t *launches, float *kernel_ms, msj_stats *st)
+{
+    (void)ix;
+    const double eps2 = eps * eps;  // launcher.cu:3200
+    const uint32_t ds = padded_dim(d);
+    const double g = (ds + 4.0) * 0x1p-52;
+    const float lo = round_down_f(eps2 / ((1.0 + 0x1p-23) * (1.0 + g)) * (1.0 - 0x1p-30) - 0x1p-58);
+    const float hi = round_up_f(eps2 / ((1.0 - 0x1p-24) * (1.0 - g)) * (1.0 + 0x1p-30) + 0x1p-58);
+    const int qbits = std::max(1, bits_for(n ? n - 1 : 0));
+    const uint32_t sample_every = 61;
+    ctr.ensure(8);
+    if (J.pairs.cap < cap_hint)
+        J.pairs.ensure(cap_hint);
+    uint64_t bwant = std::max<uint64_t>(std::max<uint64_t>(last_band + last_band / 4, 4ull * n),
+                                        1ull << 20);
+    if (band.cap < bwant) {
+        band.ensure(bwant);
+        bscore.ensure(bwant);
+    }
+    *kernel_ms = 0.f;
+    const uint32_t per_xcd = (nitems + 7) / 8;
+    const uint32_t grid = per_xcd * 8;
+    for (int attempt = 0; attempt < 3; ++attempt) {
+        LAUNCH(k_zero_ctr, 1, 64, ctr.p, 8u);
+        MSJ_HIP(hipEventRecord(k0, s));
+        if (nitems) {
+#define MSJ_MJ(SP, DG)                                                                         \
+    LAUNCH((k_mfma_join<SP, DG>), grid, 256, fh.p, SP ? fl.p : fh.p, KS, thr.p, porig.p, pjoin.p, \
+           items.p, nitems, per_xcd, lo, hi, qbits, (unsigned long long *)J.pairs.p,           \
+           (unsigned long long)J.pairs.cap, ctr.p, (unsigned long long *)band.p, bscore.p,     \
+           (unsigned long long)band.cap, sample_every)
+            if (split) {
+                if (diag)
+                    MSJ_MJ(true, true);
+                else
+                    MSJ_MJ(true, false);
+            } else {
+                if (diag)
+                    MSJ_MJ(false, true);
+                else
+                    MSJ_MJ(false, false);
+            }
+#undef MSJ_MJ
+            if (diag)
+                LAUNCH(k_band_check<true>, 2048, 256, xr, ds, eps2, J.jperm.p,
+                       (const unsigned long long *)band.p, bscore.p,
+                       (unsigned long long)band.cap, qbits, (unsigned long long *)J.pairs.p,
+                       (unsigned long long)J.pairs.cap, ctr.p);
+            else
+                LAUNCH(k_band_check<false>, 2048, 256, xr, ds, eps2, J.jperm.p,
+                       (const unsigned long long *)band.p, bscore.p,
+                       (unsigned long long)band.cap, qbits, (unsigned long long *)J.pairs.p,
+                       (unsigned long long)J.pairs.cap, ctr.p);
+            *launches += 2;
+        }
+        MSJ_HIP(hipEventRecord(k1, s));
+        unsigned long long h[8];
+        MSJ_HIP(hipMemcpyAsync(h, ctr.p, sizeof(h), hipMemcpyDeviceToHost, s));
+        MSJ_HIP(hipStreamSynchronize(s));
+        float ms = 0.f;
+        MSJ_HIP(hipEventElapsedTime(&ms, k0, k1));
+        *kernel_ms += ms;
+        const uint64_t R = h[0], nband = h[1];
+        if (R <= J.pairs.cap && nband <= band.cap) {
+            last_band = nband;
+            double emax, esum;
+            std::memcpy(&emax, &h[2], sizeof(double));
+            std::memcpy(&esum, &h[5], sizeof(double));
+            st->band_pairs = h[6];
+            st->max_score_err = emax;
+            st->mean_score_err = nband ? esum / (double)nband : 0.0;
+            st->filter_checked = h[3];
+            st->filter_violations = h[4];
+            return R;
+        }
+        if (nband > band.cap) {
+            band.ensure(nband);
+            bscore.ensure(nband);
+        }
+        if (R > J.pairs.cap)
+            J.pairs.ensure(R);
+    }
+    throw Error(MSJ_E_OVERFLOW, "result count changed between identical join passes");
+}
+
+}  // namespace msj

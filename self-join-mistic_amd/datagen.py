@@ -72,7 +72,7 @@ def reorder_dims(x, order):
     return np.ascontiguousarray(x[:, np.asarray(order, dtype=np.int64)])
 
 
-def calibrate_eps(x, target, sample=10000, seed=2000, device=None, log=None):
+def calibrate_eps(x, target, sample=10000, seed=2000, device=None, log=None, f32=False):
     """eps such that the mean neighbour count (self excluded) over a seeded
     sample of `sample` query rows is `target`: eps^2 is the (target*sample)-th
     smallest of the pooled query-to-all squared distances (self excluded).
@@ -81,7 +81,8 @@ def calibrate_eps(x, target, sample=10000, seed=2000, device=None, log=None):
     |q|^2 + |x|^2 - 2 q.x (clamped at 0), and the selection is an exact radix
     select over their IEEE bit patterns: three streaming passes, each a
     histogram of the next 22/21/21 bits among the values sharing the prefix
-    found so far; nothing of size sample*n is stored.  Without a device,
+    found so far; nothing of size sample*n is stored.  With `f32`, an FP32
+    GEMM and a 32-bit select instead (see _calibrate_f32).  Without a device,
     numpy computes the pool directly (small inputs only)."""
     n = x.shape[0]
     rng = np.random.default_rng(seed)
@@ -99,6 +100,8 @@ def calibrate_eps(x, target, sample=10000, seed=2000, device=None, log=None):
         return float(np.sqrt(np.partition(pool, want - 1)[want - 1]))
 
     import torch
+    if f32:
+        return _calibrate_f32(x, qidx, want, device, log)
     X = torch.as_tensor(np.ascontiguousarray(x, dtype=np.float64), device=device)
     x2 = (X * X).sum(dim=1)
     Q = torch.as_tensor(qidx, device=device)
@@ -131,4 +134,49 @@ def calibrate_eps(x, target, sample=10000, seed=2000, device=None, log=None):
         if log:
             log("calibration pass %d/3 done" % (used // 21))
     e2 = float(np.array([prefix], dtype=np.int64).view(np.float64)[0])
+    return float(np.sqrt(e2))
+
+
+def _calibrate_f32(x, qidx, want, device, log):
+    """FP32 variant for large n*d: column-mean-centred rows, squared distances
+    |q|^2 + |x|^2 - 2 q.x (clamped at 0) from an FP32 GEMM, and an exact radix
+    select of the want-th smallest over their 32-bit patterns (two streaming
+    passes of 16 bits)."""
+    import torch
+    n = x.shape[0]
+    mu = np.zeros(x.shape[1], dtype=np.float64)
+    for a in range(0, n, 65536):
+        mu += np.asarray(x[a:a + 65536], dtype=np.float64).sum(axis=0)
+    mu /= max(n, 1)
+    X = torch.empty((n, x.shape[1]), dtype=torch.float32, device=device)
+    mu_t = torch.as_tensor(mu, dtype=torch.float64, device=device)
+    for a in range(0, n, 1 << 20):
+        blk = torch.as_tensor(np.ascontiguousarray(x[a:a + (1 << 20)]), device=device)
+        X[a:a + blk.shape[0]] = (blk.to(torch.float64) - mu_t).to(torch.float32)
+    x2 = (X * X).sum(dim=1)
+    Q = torch.as_tensor(qidx, device=device)
+    s = len(qidx)
+    chunk = max(1, min(s, int(2e9 // (4 * n))))
+    inf_bits = 0x7F800000
+    prefix, remaining = 0, want
+    for p in range(2):
+        hist = torch.zeros(1 << 16, dtype=torch.int64, device=device)
+        for a in range(0, s, chunk):
+            rows = Q[a:a + chunk]
+            d2 = torch.addmm(x2[None, :], X[rows], X.t(), beta=1.0, alpha=-2.0)
+            d2 += x2[rows][:, None]
+            d2.clamp_(min=0.0)
+            d2[torch.arange(len(rows), device=device), rows] = float("inf")
+            b = d2.view(torch.int32)
+            sel = b[b < inf_bits] if p == 0 else b[(b >> 16) == prefix]
+            hist += torch.bincount((sel >> (16 - 16 * p)) & 0xFFFF, minlength=1 << 16)
+            del d2, b, sel
+            if log:
+                log("calibration pass %d/2: %d/%d queries" % (p + 1, min(s, a + chunk), s))
+        cum = torch.cumsum(hist, 0)
+        digit = int(torch.searchsorted(cum, torch.tensor([remaining], device=device)).item())
+        if digit > 0:
+            remaining -= int(cum[digit - 1].item())
+        prefix = (prefix << 16) | digit
+    e2 = float(np.array([prefix], dtype=np.uint32).view(np.float32)[0])
     return float(np.sqrt(e2))

@@ -73,6 +73,11 @@ class Stats(C.Structure):
         ("band_pairs", C.c_uint64),
         ("max_score_err", C.c_double),
         ("mean_score_err", C.c_double),
+        ("filter_checked", C.c_uint64),
+        ("filter_violations", C.c_uint64),
+        ("form_used", C.c_uint32),
+        ("pad2", C.c_uint32),
+        ("mfma_flops", C.c_double),
     ]
 
 
@@ -182,6 +187,8 @@ class JoinResult:
             "join_launches": st.join_launches, "dims_evaluated": st.dims_evaluated,
             "band_pairs": st.band_pairs,
             "max_score_err": st.max_score_err, "mean_score_err": st.mean_score_err,
+            "filter_checked": st.filter_checked, "filter_violations": st.filter_violations,
+            "form_used": st.form_used, "mfma_flops": st.mfma_flops,
         }
         self.on_device = bool(r.on_device)
         if copy_arrays and not r.on_device:

@@ -1,0 +1,149 @@
+"""GPU parity of the precision forms (BF16 / split-BF16 MFMA filter + FP64
+band recheck) against the CPU oracle, through the C ABI.
+
+The bar is the plain form's: identical counts, offsets, neighbour lists,
+node order and node-net statistics.  With count_work the library also
+verifies, in FP64, the filter's class of every pair of a sample of work
+items; a certain-accept or certain-reject class that FP64 contradicts would
+be a wrong error bound (filter_violations must be 0).
+"""
+import numpy as np
+import pytest
+
+import datagen
+
+pytestmark = pytest.mark.gpu
+
+FORMS = ["bf16", "split"]
+
+
+def _form(msj_mod, name):
+    return msj_mod.FORM_MFMA_BF16 if name == "bf16" else msj_mod.FORM_MFMA_SPLIT
+
+
+def _announce(name, x, eps, form):
+    print("\n[case] %s form=%s n=%d d=%d eps=%r" % (name, form, x.shape[0], x.shape[1], eps),
+          flush=True)
+
+
+def _assert_same(got, ref):
+    assert got.total_pairs == ref.total_pairs
+    assert np.array_equal(got.counts, ref.counts)
+    assert np.array_equal(got.offsets, ref.offsets)
+    assert np.array_equal(got.nbrs, ref.nbrs)
+    assert np.array_equal(got.node_order, ref.node_order)
+    st = got.stats
+    assert st["num_nodes"] == ref.num_nodes
+    assert st["num_calcs"] == ref.num_calcs
+    assert st["chosen"] == ref.chosen
+    assert st["sse"] == ref.sse
+
+
+def _prep(x, msj_mod):
+    order, _ = msj_mod.dim_order(np.asarray(x, dtype=np.float64))
+    return datagen.reorder_dims(x, order)
+
+
+CASES = [
+    # name, generator, eps
+    ("correlated 6000x90", lambda: datagen.correlated(6000, 90, 22), 1.14),
+    ("correlated 20000x90 dense", lambda: datagen.correlated(20000, 90, 23), 0.77),
+    ("mixture 6000x57", lambda: datagen.mixture(6000, 57, 21), 0.553),
+    ("mixture 20000x18", lambda: datagen.mixture(20000, 18, 13), 0.204),
+    ("uniform 4096x2", lambda: datagen.uniform(4096, 2, 11), 0.02),
+    ("mixture 3000x33", lambda: datagen.mixture(3000, 33, 24), 0.45),
+]
+
+
+@pytest.mark.parametrize("form", FORMS)
+@pytest.mark.parametrize("name,gen,eps", CASES, ids=[c[0] for c in CASES])
+def test_filter_parity(name, gen, eps, form, msj_mod, oracle_mod):
+    x = _prep(gen(), msj_mod)
+    _announce(name, x, eps, form)
+    got = msj_mod.self_join(x, eps, form=_form(msj_mod, form))
+    ref = oracle_mod.self_join(x, eps)
+    _assert_same(got, ref)
+    st = got.stats
+    assert st["form_used"] == _form(msj_mod, form)
+    assert st["mfma_flops"] > 0
+    print("[case] band %d pairs, max score err %.3g" % (st["band_pairs"], st["max_score_err"]),
+          flush=True)
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_filter_fp32_384(form, msj_mod, oracle_mod):
+    x = _prep(datagen.clusters(3000, 384, seed=31, ncl=40), msj_mod)
+    eps = 1.26
+    _announce("fp32 clusters 3000x384", x, eps, form)
+    got = msj_mod.self_join(x, eps, form=_form(msj_mod, form), storage=msj_mod.STORAGE_F32)
+    ref = oracle_mod.self_join(x.astype(np.float64), eps)
+    _assert_same(got, ref)
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_filter_bound_verified(form, msj_mod, oracle_mod):
+    """count_work: sampled pairs' filter classes re-decided in FP64."""
+    x = _prep(datagen.correlated(20000, 90, 25), msj_mod)
+    eps = 0.77
+    _announce("bound check", x, eps, form)
+    got = msj_mod.self_join(x, eps, form=_form(msj_mod, form), count_work=1)
+    ref = oracle_mod.self_join(x, eps)
+    _assert_same(got, ref)
+    st = got.stats
+    assert st["filter_checked"] > 100000
+    assert st["filter_violations"] == 0
+    print("[case] checked %d sampled pairs, mean |score err| %.3g" % (
+        st["filter_checked"], st["mean_score_err"]), flush=True)
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_filter_extreme_rows(form, msj_mod, oracle_mod):
+    """Rows far outside the data (|a'|^2 > 2^100) fall in the band; a
+    duplicate pair of them is a result."""
+    x = datagen.mixture(3000, 40, 26)
+    x[17] = 1e60
+    x[2900] = 1e60
+    x[5, :7] = -3e55
+    x = _prep(x, msj_mod)
+    eps = 0.5
+    _announce("extreme rows", x, eps, form)
+    got = msj_mod.self_join(x, eps, form=_form(msj_mod, form))
+    ref = oracle_mod.self_join(x, eps)
+    _assert_same(got, ref)
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_filter_offset_data(form, msj_mod, oracle_mod):
+    """A large common offset: centring keeps the filter useful and exact."""
+    x = datagen.correlated(5000, 64, 27) + 1000.0
+    x = _prep(x, msj_mod)
+    eps = 1.0
+    _announce("offset data", x, eps, form)
+    got = msj_mod.self_join(x, eps, form=_form(msj_mod, form))
+    ref = oracle_mod.self_join(x, eps)
+    _assert_same(got, ref)
+
+
+def test_filter_outside_range_runs_plain(msj_mod, oracle_mod):
+    x = _prep(datagen.uniform(3000, 40, 28), msj_mod)
+    eps = 1e-20  # eps^2 below the filter's range
+    _announce("eps below filter range", x, eps, "split")
+    got = msj_mod.self_join(x, eps, form=msj_mod.FORM_MFMA_SPLIT)
+    ref = oracle_mod.self_join(x, eps)
+    _assert_same(got, ref)
+    assert got.stats["form_used"] == msj_mod.FORM_PLAIN
+
+
+def test_filter_context_reuse(msj_mod, oracle_mod):
+    x = _prep(datagen.correlated(8000, 90, 29), msj_mod)
+    _announce("filter context reuse", x, 0.8, "split")
+    with msj_mod.Context(form=msj_mod.FORM_MFMA_SPLIT) as ctx:
+        ctx.load(x)
+        for eps in (0.8, 0.5, 0.8):
+            got = ctx.join(eps)
+            ref = oracle_mod.self_join(x, eps)
+            _assert_same(got, ref)
+        small = x[:1500].copy()
+        ctx.load(small)
+        got = ctx.join(0.8)
+        _assert_same(got, oracle_mod.self_join(small, 0.8))
