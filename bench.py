@@ -57,7 +57,7 @@ CONFIGS = {
                eps_hex="3fe8aa41cdaa80fa",
                workload="FP64 join, 515,345 x 90 correlated Gaussian, mean 32 neighbours"),
     "c4": dict(n=5_000_000, d=384, kind="clusters", seed=1004, target=16, cal_seed=2004,
-               cal_sample=2000, cal_f32=True, storage="f32",
+               cal_sample=2000, cal_f32=True, storage="f32", eps_hex="3ff28ce26ea18cb2",
                workload="FP64 decisions on FP32-stored 5,000,000 x 384 Zipf clusters, "
                         "mean 16 neighbours"),
 }

@@ -117,14 +117,14 @@ int guarded(F &&f)
 __global__ void k_soa(const unsigned char *__restrict__ rm, unsigned char *__restrict__ soa,
                       uint32_t n, uint32_t d, uint32_t esz)
 {
-    size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (size_t)n * d)
-        return;
-    uint32_t j = (uint32_t)(idx / n), i = (uint32_t)(idx - (size_t)j * n);
-    if (esz == 8)
-        ((double *)soa)[idx] = ((const double *)rm)[(size_t)i * d + j];
-    else
-        ((float *)soa)[idx] = ((const float *)rm)[(size_t)i * d + j];
+    const size_t total = (size_t)n * d, stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += stride) {
+        uint32_t j = (uint32_t)(idx / n), i = (uint32_t)(idx - (size_t)j * n);
+        if (esz == 8)
+            ((double *)soa)[idx] = ((const double *)rm)[(size_t)i * d + j];
+        else
+            ((float *)soa)[idx] = ((const float *)rm)[(size_t)i * d + j];
+    }
 }
 
 // launcher.cu:3203-3233: the reference's batch plan, kept for its stderr record.
@@ -174,7 +174,7 @@ void do_load(msj_ctx *c, const void *data, uint32_t n, uint32_t d)
     c->staging.ensure(bytes);
     c->xs.ensure(bytes);
     MSJ_HIP(hipMemcpyAsync(c->staging.p, data, bytes, hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_soa, dim3(msj::grid_for((size_t)n * d, 256)), dim3(256), 0, c->stream,
+    hipLaunchKernelGGL(k_soa, dim3(msj::grid_stride((size_t)n * d, 256)), dim3(256), 0, c->stream,
                        c->staging.p, c->xs.p, n, d, esz);
     MSJ_HIP(hipGetLastError());
     MSJ_HIP(hipStreamSynchronize(c->stream));
@@ -242,10 +242,6 @@ void do_join(msj_ctx *c, double eps, msj_result *out)
     if (use_mfma) {
         R = c->mfma.run(c->ix, c->join, c->xr.p, n, d, eps, cap, c->prm.count_work != 0, s,
                         c->ev[2], c->ev[3], &launches, &kms, &st);
-        const uint32_t kp = (d + 31u) & ~31u;
-        st.mfma_flops = (double)c->mfma.tiles * 128.0 * 128.0 * 2.0 * kp *
-                        (c->prm.form == MSJ_FORM_MFMA_SPLIT ? 3.0 : 1.0);
-        st.form_used = c->prm.form;
     } else {
         R = c->join.run_plain(c->ix, c->xr.p, n, d, eps, cap, s, c->ev[2], c->ev[3], &launches,
                               &kms, c->prm.count_work ? &dims : nullptr);

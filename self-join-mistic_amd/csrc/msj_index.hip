@@ -525,6 +525,7 @@ T d2h_scalar(const T *dptr, hipStream_t s)
 
 #define LAUNCH(kern, grid, block, ...)                                                   \
     do {                                                                                 \
+        ::msj::check_grid((grid), (block));                                              \
         hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, s, __VA_ARGS__);            \
         MSJ_HIP(hipGetLastError());                                                      \
     } while (0)

@@ -93,6 +93,21 @@ inline unsigned grid_for(size_t n, unsigned block)
     return (unsigned)(g ? g : 1);
 }
 
+// A dispatch covers fewer than 2^32 work-items (the AQL packet's grid size
+// is 32 bits): larger launches would be truncated silently, so they are
+// refused here and the kernels that can be that large loop over their work.
+constexpr unsigned MAX_GRID = 1u << 20;  // blocks of a grid-stride launch
+inline void check_grid(size_t grid, unsigned block)
+{
+    if (grid == 0 || grid >= (1ull << 31) || (uint64_t)grid * block >= (1ull << 32))
+        throw Error(MSJ_E_OVERFLOW, "kernel launch exceeds 2^32 work-items");
+}
+inline unsigned grid_stride(size_t n, unsigned block)
+{
+    size_t g = (n + block - 1) / block;
+    return (unsigned)(g == 0 ? 1 : (g < MAX_GRID ? g : MAX_GRID));
+}
+
 inline int bits_for(uint64_t maxval)  // bits needed to represent 0..maxval
 {
     int b = 0;

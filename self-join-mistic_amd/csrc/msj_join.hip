@@ -100,30 +100,10 @@ __global__ __launch_bounds__(256) void k_join_sym(
     __shared__ unsigned long long hsh[WPB][HITS];
     const uint32_t lane = threadIdx.x & (WAVE - 1);
     const uint32_t wv = threadIdx.x / WAVE;
-    // wave-uniform by construction; readfirstlane lets the compiler keep the
-    // item and the loop counters in SGPRs
-    const uint32_t it = __builtin_amdgcn_readfirstlane(blockIdx.x * WPB + wv);
-    const bool have = it < nitems;
-    JoinItem w = {0, 0, 0, 0, 0};
-    if (have)
-        w = items[it];
-    const uint32_t X = w.x;
-    const uint32_t x0 = have ? nstart[X] : 0u;
-    const uint32_t stride = TAIL ? ds : (uint32_t)DS;
-    const uint32_t qn = w.qend - w.qbeg;
-    // stage the item's query rows (first DS dimensions) in LDS
-    double *qs = qsh[wv];
-    for (uint32_t e = lane; e < qn * DS; e += WAVE) {
-        const uint32_t r = e / DS, k = e - r * DS;
-        qs[e] = xr[(size_t)(x0 + w.qbeg + r) * stride + k];
-    }
-    __syncthreads();
-    if (!have)
-        return;
     const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (WAVE - lane));
-    const uint64_t qall = qn >= WAVE ? ~0ull : ((1ull << qn) - 1ull);
     unsigned long long *hb = hsh[wv];
     uint32_t fill = 0;  // wave-uniform
+    uint64_t dims = 0;
     // copies the buffered keys out: one atomic for the whole buffer
     auto flush = [&]() {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -141,104 +121,130 @@ __global__ __launch_bounds__(256) void k_join_sym(
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         fill = 0;
     };
-
-    // per-lane cursor into X's upper neighbour list
-    uint32_t a = uoff[X];
-    uint32_t Y = uadj[a];
-    uint32_t ybase = 0;
-    uint32_t ystart = nstart[Y];
-    uint32_t ysz = nstart[Y + 1] - ystart;
-    uint64_t dims = 0;
-
-    for (uint32_t cs = w.cbeg; cs < w.cend; cs += WAVE) {
-        const uint32_t ci = cs + lane;
-        const bool cvalid = ci < w.cend;
-        uint32_t c = 0;
-        if (cvalid) {
-            while (ci >= ybase + ysz) {
-                ybase += ysz;
-                ++a;
-                Y = uadj[a];
-                ystart = nstart[Y];
-                ysz = nstart[Y + 1] - ystart;
-            }
-            c = ystart + (ci - ybase);
+    // grid-stride over groups of WPB items (a launch stays under 2^32 lanes)
+    for (uint64_t it0 = (uint64_t)blockIdx.x * WPB; it0 < nitems;
+         it0 += (uint64_t)gridDim.x * WPB) {
+        // wave-uniform by construction; readfirstlane lets the compiler keep the
+        // item and the loop counters in SGPRs
+        const bool have = it0 + wv < nitems;
+        const uint32_t it = __builtin_amdgcn_readfirstlane(have ? (uint32_t)(it0 + wv) : 0u);
+        JoinItem w = {0, 0, 0, 0, 0};
+        if (have)
+            w = items[it];
+        const uint32_t X = w.x;
+        const uint32_t x0 = have ? nstart[X] : 0u;
+        const uint32_t stride = TAIL ? ds : (uint32_t)DS;
+        const uint32_t qn = w.qend - w.qbeg;
+        // stage the item's query rows (first DS dimensions) in LDS
+        double *qs = qsh[wv];
+        for (uint32_t e = lane; e < qn * DS; e += WAVE) {
+            const uint32_t r = e / DS, k = e - r * DS;
+            qs[e] = xr[(size_t)(x0 + w.qbeg + r) * stride + k];
         }
-        const bool self_node = cvalid && Y == X;
-        const double *crow = xr + (size_t)c * stride;  // c = 0 for idle lanes
-        double cc[DS];
-#pragma unroll
-        for (int k = 0; k < DS; ++k)
-            cc[k] = crow[k];
-        const unsigned long long corig = perm[c];
+        __syncthreads();
+        if (have) {
+            const uint64_t qall = qn >= WAVE ? ~0ull : ((1ull << qn) - 1ull);
 
-        uint64_t qmask = qall;
-        while (qmask) {
-            const uint32_t qi = (uint32_t)__builtin_ctzll(qmask);
-            qmask &= qmask - 1ull;
-            const uint32_t p = x0 + w.qbeg + qi;
-            bool alive = cvalid && (!self_node || p <= c);
-            if (!wave_vote(alive))
-                continue;
-            const double *qrow = qs + qi * DS;  // wave-uniform LDS address
-            double s = 0.0;
-#pragma unroll
-            for (int l = 0; l < DS; l += 4) {
-                if (l > 0 && !wave_vote(alive))
-                    break;
-                if (COUNT && alive && (uint32_t)l < d)
-                    dims += min(4u, d - (uint32_t)l);
-                const double2 qa = *(const double2 *)(qrow + l);
-                const double2 qb = *(const double2 *)(qrow + l + 2);
-                double t0 = qa.x - cc[l];
-                s = fma(t0, t0, s);
-                double t1 = qa.y - cc[l + 1];
-                double q1 = t1 * t1;
-                s = s + q1;
-                double t2 = qb.x - cc[l + 2];
-                double q2 = t2 * t2;
-                s = s + q2;
-                double t3 = qb.y - cc[l + 3];
-                double q3 = t3 * t3;
-                s = s + q3;
-                alive = alive && !(s > eps2);
-            }
-            if (TAIL) {
-                const double *grow = xr + (size_t)p * stride;
-                for (uint32_t l = DS; l < ds && wave_vote(alive); l += 4) {
-                    if (COUNT && alive && l < d)
-                        dims += min(4u, d - l);
-                    double t0 = grow[l] - crow[l];
-                    s = fma(t0, t0, s);
-                    double t1 = grow[l + 1] - crow[l + 1];
-                    double q1 = t1 * t1;
-                    s = s + q1;
-                    double t2 = grow[l + 2] - crow[l + 2];
-                    double q2 = t2 * t2;
-                    s = s + q2;
-                    double t3 = grow[l + 3] - crow[l + 3];
-                    double q3 = t3 * t3;
-                    s = s + q3;
-                    alive = alive && !(s > eps2);
+            // per-lane cursor into X's upper neighbour list
+            uint32_t a = uoff[X];
+            uint32_t Y = uadj[a];
+            uint32_t ybase = 0;
+            uint32_t ystart = nstart[Y];
+            uint32_t ysz = nstart[Y + 1] - ystart;
+
+            for (uint32_t cs = w.cbeg; cs < w.cend; cs += WAVE) {
+                const uint32_t ci = cs + lane;
+                const bool cvalid = ci < w.cend;
+                uint32_t c = 0;
+                if (cvalid) {
+                    while (ci >= ybase + ysz) {
+                        ybase += ysz;
+                        ++a;
+                        Y = uadj[a];
+                        ystart = nstart[Y];
+                        ysz = nstart[Y + 1] - ystart;
+                    }
+                    c = ystart + (ci - ybase);
+                }
+                const bool self_node = cvalid && Y == X;
+                const double *crow = xr + (size_t)c * stride;  // c = 0 for idle lanes
+                double cc[DS];
+        #pragma unroll
+                for (int k = 0; k < DS; ++k)
+                    cc[k] = crow[k];
+                const unsigned long long corig = perm[c];
+
+                uint64_t qmask = qall;
+                while (qmask) {
+                    const uint32_t qi = (uint32_t)__builtin_ctzll(qmask);
+                    qmask &= qmask - 1ull;
+                    const uint32_t p = x0 + w.qbeg + qi;
+                    bool alive = cvalid && (!self_node || p <= c);
+                    if (!wave_vote(alive))
+                        continue;
+                    const double *qrow = qs + qi * DS;  // wave-uniform LDS address
+                    double s = 0.0;
+        #pragma unroll
+                    for (int l = 0; l < DS; l += 4) {
+                        if (l > 0 && !wave_vote(alive))
+                            break;
+                        if (COUNT && alive && (uint32_t)l < d)
+                            dims += min(4u, d - (uint32_t)l);
+                        const double2 qa = *(const double2 *)(qrow + l);
+                        const double2 qb = *(const double2 *)(qrow + l + 2);
+                        double t0 = qa.x - cc[l];
+                        s = fma(t0, t0, s);
+                        double t1 = qa.y - cc[l + 1];
+                        double q1 = t1 * t1;
+                        s = s + q1;
+                        double t2 = qb.x - cc[l + 2];
+                        double q2 = t2 * t2;
+                        s = s + q2;
+                        double t3 = qb.y - cc[l + 3];
+                        double q3 = t3 * t3;
+                        s = s + q3;
+                        alive = alive && !(s > eps2);
+                    }
+                    if (TAIL) {
+                        const double *grow = xr + (size_t)p * stride;
+                        for (uint32_t l = DS; l < ds && wave_vote(alive); l += 4) {
+                            if (COUNT && alive && l < d)
+                                dims += min(4u, d - l);
+                            double t0 = grow[l] - crow[l];
+                            s = fma(t0, t0, s);
+                            double t1 = grow[l + 1] - crow[l + 1];
+                            double q1 = t1 * t1;
+                            s = s + q1;
+                            double t2 = grow[l + 2] - crow[l + 2];
+                            double q2 = t2 * t2;
+                            s = s + q2;
+                            double t3 = grow[l + 3] - crow[l + 3];
+                            double q3 = t3 * t3;
+                            s = s + q3;
+                            alive = alive && !(s > eps2);
+                        }
+                    }
+                    const uint64_t m1 = wave_vote(alive);
+                    if (m1) {
+                        const bool two = alive && c != p;
+                        const uint64_t m2 = wave_vote(two);
+                        const uint32_t tot = (uint32_t)(__popcll(m1) + __popcll(m2));
+                        if (fill + tot > HITS)
+                            flush();
+                        if (alive) {
+                            const unsigned long long porig = perm[p];
+                            const uint32_t idx =
+                                fill + (uint32_t)(__popcll(m1 & lt) + __popcll(m2 & lt));
+                            hb[idx] = (porig << qbits) | corig;
+                            if (two)
+                                hb[idx + 1] = (corig << qbits) | porig;
+                        }
+                        fill += tot;
+                    }
                 }
             }
-            const uint64_t m1 = wave_vote(alive);
-            if (m1) {
-                const bool two = alive && c != p;
-                const uint64_t m2 = wave_vote(two);
-                const uint32_t tot = (uint32_t)(__popcll(m1) + __popcll(m2));
-                if (fill + tot > HITS)
-                    flush();
-                if (alive) {
-                    const unsigned long long porig = perm[p];
-                    const uint32_t idx = fill + (uint32_t)(__popcll(m1 & lt) + __popcll(m2 & lt));
-                    hb[idx] = (porig << qbits) | corig;
-                    if (two)
-                        hb[idx + 1] = (corig << qbits) | porig;
-                }
-                fill += tot;
-            }
         }
+        __syncthreads();  // the next group's staging reuses qsh
     }
     if (fill)
         flush();
@@ -356,39 +362,39 @@ __global__ __launch_bounds__(256) void k_gather_rows(const T *__restrict__ xs,
                                                      const uint32_t *__restrict__ perm, uint32_t n,
                                                      uint32_t d, uint32_t ds)
 {
-    size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (size_t)n * ds)
-        return;
-    uint32_t i = (uint32_t)(idx / ds), j = (uint32_t)(idx - (size_t)i * ds);
-    xr[idx] = j < d ? (double)xs[(size_t)j * n + perm[i]] : 0.0;
+    const size_t total = (size_t)n * ds, stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += stride) {
+        uint32_t i = (uint32_t)(idx / ds), j = (uint32_t)(idx - (size_t)i * ds);
+        xr[idx] = j < d ? (double)xs[(size_t)j * n + perm[i]] : 0.0;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_rows_count(const uint64_t *__restrict__ pairs, size_t R,
                                                     int qbits, uint32_t *__restrict__ counts)
 {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= R)
-        return;
-    uint32_t row = (uint32_t)(pairs[i] >> qbits);
-    // sorted keys: count each row once at the end of its run
-    if (i + 1 == R || (uint32_t)(pairs[i + 1] >> qbits) != row) {
-        size_t lo = 0, hi = i + 1;  // first index of this row
-        while (lo < hi) {
-            size_t mid = lo + (hi - lo) / 2;
-            if ((uint32_t)(pairs[mid] >> qbits) < row)
-                lo = mid + 1;
-            else
-                hi = mid;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < R; i += stride) {
+        uint32_t row = (uint32_t)(pairs[i] >> qbits);
+        // sorted keys: count each row once at the end of its run
+        if (i + 1 == R || (uint32_t)(pairs[i + 1] >> qbits) != row) {
+            size_t lo = 0, hi = i + 1;  // first index of this row
+            while (lo < hi) {
+                size_t mid = lo + (hi - lo) / 2;
+                if ((uint32_t)(pairs[mid] >> qbits) < row)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            counts[row] = (uint32_t)(i + 1 - lo);
         }
-        counts[row] = (uint32_t)(i + 1 - lo);
     }
 }
 
 __global__ __launch_bounds__(256) void k_rows_nbrs(const uint64_t *__restrict__ pairs, size_t R,
                                                    uint64_t qmask, uint32_t *__restrict__ nbrs)
 {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < R)
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < R; i += stride)
         nbrs[i] = (uint32_t)(pairs[i] & qmask);
 }
 
@@ -477,6 +483,7 @@ __global__ void k_zero(T *p, size_t n)
 
 #define LAUNCH(kern, grid, block, ...)                                                   \
     do {                                                                                 \
+        ::msj::check_grid((grid), (block));                                              \
         hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, s, __VA_ARGS__);            \
         MSJ_HIP(hipGetLastError());                                                      \
     } while (0)
@@ -488,11 +495,11 @@ void launch_sym(const JoinRunner &J, const Index &ix, const double *xr, uint32_t
 {
     const unsigned wpb = WPB;
     if (work)
-        LAUNCH((k_join_sym<DS, TAIL, true>), grid_for(J.nitems, wpb), WAVE * wpb, xr, d, ds, eps2,
-               ix.nstart.p, J.uoff.p, J.uadj.p, J.jperm.p, J.items.p, J.nitems, qbits, out,
+        LAUNCH((k_join_sym<DS, TAIL, true>), grid_stride(J.nitems, wpb), WAVE * wpb, xr, d, ds,
+               eps2, ix.nstart.p, J.uoff.p, J.uadj.p, J.jperm.p, J.items.p, J.nitems, qbits, out,
                J.counter.p, cap, work);
     else
-        LAUNCH((k_join_sym<DS, TAIL, false>), grid_for(J.nitems, wpb), WAVE * wpb, xr, d, ds,
+        LAUNCH((k_join_sym<DS, TAIL, false>), grid_stride(J.nitems, wpb), WAVE * wpb, xr, d, ds,
                eps2, ix.nstart.p, J.uoff.p, J.uadj.p, J.jperm.p, J.items.p, J.nitems, qbits, out,
                J.counter.p, cap, work);
 }
@@ -529,10 +536,11 @@ void gather_node_rows(const void *xs, double *xr, bool f32, const uint32_t *perm
     const uint32_t ds = padded_dim(d);
     size_t m = (size_t)n * ds;
     if (f32)
-        LAUNCH(k_gather_rows<float>, grid_for(m, 256), 256, (const float *)xs, xr, perm, n, d, ds);
-    else
-        LAUNCH(k_gather_rows<double>, grid_for(m, 256), 256, (const double *)xs, xr, perm, n, d,
+        LAUNCH(k_gather_rows<float>, grid_stride(m, 256), 256, (const float *)xs, xr, perm, n, d,
                ds);
+    else
+        LAUNCH(k_gather_rows<double>, grid_stride(m, 256), 256, (const double *)xs, xr, perm, n,
+               d, ds);
 }
 
 void JoinRunner::order(const Index &ix, const void *xs, bool f32, uint32_t n, uint32_t d,
@@ -642,8 +650,8 @@ void JoinRunner::finish(uint32_t n, uint64_t R, uint32_t *counts, uint64_t *offs
     sorter.sort(pairs.p, nullptr, R, qbits + rbits, s);
     LAUNCH(k_zero<uint32_t>, grid_for(n, 256), 256, counts, (size_t)n);
     if (R) {
-        LAUNCH(k_rows_count, grid_for(R, 256), 256, pairs.p, (size_t)R, qbits, counts);
-        LAUNCH(k_rows_nbrs, grid_for(R, 256), 256, pairs.p, (size_t)R, (1ull << qbits) - 1,
+        LAUNCH(k_rows_count, grid_stride(R, 256), 256, pairs.p, (size_t)R, qbits, counts);
+        LAUNCH(k_rows_nbrs, grid_stride(R, 256), 256, pairs.p, (size_t)R, (1ull << qbits) - 1,
                nbrs);
     }
     scan.excl_u32_to_u64(counts, offsets, n, s);

@@ -314,50 +314,61 @@ __global__ __launch_bounds__(256) void k_mitem_fill(const uint32_t *__restrict__
 
 // --------------------------------------------------------------- the filter
 // ctr: [0] result keys, [1] band entries, [2] max error (double bits),
-// [3] sampled pairs checked, [4] violations, [5] error sum (double),
-// [6] pairs decided in FP64.
+// [3] verified pairs, [4] violations, [5] error sum (double), [6] pairs
+// decided in FP64, [7] class-0 band entries.  band_budget != 0: workgroups
+// stop taking items once [7] exceeds it (the BF16 bound was too loose; the
+// host reruns with split-BF16).
+//
+// Band entries: (join position p << 32 | class << 30 | join position c) and
+// the filter score.  Class 0: band pair, decided in FP64.  With DIAG
+// (count_work): class 1, a certain accept (every one is verified); class 2,
+// a certain reject of a sampled work item (verified).
+//
+// Epilogue.  The A rows' terms are staged per item and the B rows' per tile;
+// rows outside the item (padding, the next node) get +inf terms, so the
+// fast pass -- one add, one fma, one compare per pair -- rejects them
+// without a validity test.  Only a wave whose 64 x 64 block holds a pair
+// that is not a certain reject (or NaN-scored) takes the full pass, which
+// tests validity explicitly, collects certain accepts into a lane mask and
+// stages band entries in LDS.
 template <bool SPLIT, bool DIAG>
-__global__ __launch_bounds__(256) void k_mfma_join(
+__global__ __launch_bounds__(256, 2) void k_mfma_join(
     const uint4 *__restrict__ fh, const uint4 *__restrict__ fl, uint32_t KS,
     const float4 *__restrict__ thr, const uint32_t *__restrict__ porig,
     const uint32_t *__restrict__ pjoin, const MItem *__restrict__ items, uint32_t nitems,
     uint32_t per_xcd, float lo, float hi, int qbits, unsigned long long *__restrict__ out,
     unsigned long long cap, unsigned long long *__restrict__ ctr,
     unsigned long long *__restrict__ band, float *__restrict__ bscore, unsigned long long bcap,
-    uint32_t sample_every)
+    uint32_t sample_every, unsigned long long band_budget)
 {
     constexpr uint32_t NP = SPLIT ? 2 : 1;
     constexpr uint32_t SLAB = 4 * KSL * WAVE;  // uint4 per plane per operand
     constexpr uint32_t PER_T = SLAB / 256;     // per thread
+    constexpr uint32_t BHITS = 128;            // per-wave band staging
     __shared__ uint4 sA[NP][SLAB];
     __shared__ uint4 sB[NP][SLAB];
     __shared__ float4 tA[TM], tB[TM];
     __shared__ uint32_t oA[TM], oB[TM], jA[TM], jB[TM];
     __shared__ unsigned long long hsh[4][MHITS];
+    __shared__ unsigned long long bsh[4][BHITS];
+    __shared__ float ssh[4][BHITS];
+    __shared__ uint32_t s_stop;
 
     const uint32_t tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid / WAVE;
-    const uint32_t wr = wv >> 1, wc = wv & 1;
-    // consecutive items (same A rows, neighbouring B rows) on one XCD: the
-    // hardware deals workgroups out to the 8 XCDs round robin
-    const uint32_t bid = blockIdx.x;
-    const uint32_t it = (bid % 8u) * per_xcd + bid / 8u;
-    if (it >= nitems)
-        return;  // uniform over the workgroup
-    const MItem w = items[it];
-    const bool sampled = DIAG && (it % sample_every == 0);
-    if (tid < TM) {
-        const uint32_t r = w.qbeg + tid;
-        tA[tid] = thr[r];
-        oA[tid] = porig[r];
-        jA[tid] = pjoin[r];
-    }
+    const uint32_t wr = wv >> 1, wc = wv & 1, h = lane / 32;
     const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (WAVE - lane));
+    const float INF = __builtin_inff();
     unsigned long long *hb = hsh[wv];
-    uint32_t fill = 0;
-    auto flush = [&]() {
+    unsigned long long *bb = bsh[wv];
+    float *sb = ssh[wv];
+    uint32_t fill = 0, bfill = 0, b0fill = 0;  // wave-uniform
+    auto wave_sync = [&]() {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+    auto flush = [&]() {  // result keys: one atomic per flush
+        wave_sync();
         unsigned long long base = 0;
         if (lane == 0)
             base = atomicAdd(&ctr[0], (unsigned long long)fill);
@@ -365,181 +376,252 @@ __global__ __launch_bounds__(256) void k_mfma_join(
         for (uint32_t e = lane; e < fill; e += WAVE)
             if (base + e < cap)
                 out[base + e] = hb[e];
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        wave_sync();
         fill = 0;
     };
-
-    const uint32_t rbA = w.qbeg / 32;
-    for (uint32_t cb = w.cbeg; cb < w.cend; cb += TM) {
-        const uint32_t rbB = cb / 32;
-        uint4 ra[NP][PER_T], rb[NP][PER_T];
-        auto gload = [&](uint32_t k0) {
-#pragma unroll
-            for (uint32_t q = 0; q < PER_T; ++q) {
-                const uint32_t e = tid + q * 256;
-                const uint32_t rbl = e / (KSL * WAVE), rem = e % (KSL * WAVE);
-                const size_t ga = ((size_t)(rbA + rbl) * KS + k0) * WAVE + rem;
-                const size_t gb = ((size_t)(rbB + rbl) * KS + k0) * WAVE + rem;
-                ra[0][q] = fh[ga];
-                rb[0][q] = fh[gb];
-                if (SPLIT) {
-                    ra[NP - 1][q] = fl[ga];
-                    rb[NP - 1][q] = fl[gb];
-                }
-            }
-        };
-        gload(0);
-        __syncthreads();  // the previous tile's epilogue is done with tB
-        if (tid < TM) {
-            const uint32_t r = cb + tid;
-            tB[tid] = thr[r];
-            oB[tid] = porig[r];
-            jB[tid] = pjoin[r];
+    auto bflush = [&]() {  // band entries
+        wave_sync();
+        unsigned long long base = 0;
+        if (lane == 0) {
+            base = atomicAdd(&ctr[1], (unsigned long long)bfill);
+            if (b0fill)
+                atomicAdd(&ctr[7], (unsigned long long)b0fill);
         }
-        floatx16 acc[2][2];
-#pragma unroll
-        for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-            for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-                for (int v = 0; v < 16; ++v)
-                    acc[bi][bj][v] = 0.f;
-        for (uint32_t k0 = 0; k0 < KS; k0 += KSL) {
-            __syncthreads();  // all waves are done reading the previous slab
-#pragma unroll
-            for (uint32_t q = 0; q < PER_T; ++q) {
-#pragma unroll
-                for (uint32_t pl = 0; pl < NP; ++pl) {
-                    sA[pl][tid + q * 256] = ra[pl][q];
-                    sB[pl][tid + q * 256] = rb[pl][q];
-                }
+        base = __shfl(base, 0, WAVE);
+        for (uint32_t e = lane; e < bfill; e += WAVE)
+            if (base + e < bcap) {
+                band[base + e] = bb[e];
+                bscore[base + e] = sb[e];
             }
-            __syncthreads();
-            if (k0 + KSL < KS)
-                gload(k0 + KSL);  // in flight during this slab's MFMAs
+        wave_sync();
+        bfill = 0;
+        b0fill = 0;
+    };
+
+    // Work items in a grid-stride loop over virtual block ids; id v runs
+    // item (v % 8) * per_xcd + v / 8, so the items of one XCD (the hardware
+    // deals workgroups out round robin) are consecutive: same A rows,
+    // neighbouring B rows, shared in that XCD's L2.
+    const uint64_t vtotal = 8ull * per_xcd;
+    for (uint64_t v = blockIdx.x; v < vtotal; v += gridDim.x) {
+        const uint32_t it = (uint32_t)((v % 8u) * per_xcd + v / 8u);
+        __syncthreads();  // the previous item is done with tA and the slabs
+        if (tid == 0)
+            s_stop = band_budget != 0 &&
+                     __hip_atomic_load(&ctr[7], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >
+                         band_budget;
+        if (it < nitems && tid < TM) {
+            const MItem w = items[it];
+            const uint32_t r = w.qbeg + tid;
+            float4 t = thr[r];
+            if (r >= w.qend)
+                t = make_float4(INF, INF, INF, INF);
+            tA[tid] = t;
+            oA[tid] = porig[r];
+            jA[tid] = pjoin[r];
+        }
+        __syncthreads();
+        if (it >= nitems || s_stop)
+            continue;  // uniform over the workgroup
+        const MItem w = items[it];
+        const bool sampled = DIAG && (it % sample_every == 0);
+        const uint32_t rbA = w.qbeg / 32;
+
+        for (uint32_t cb = w.cbeg; cb < w.cend; cb += TM) {
+            const uint32_t rbB = cb / 32;
+            // two uint4 per thread per plane and operand (SLAB = 2 * 256):
+            // named registers, so the prefetch stays in VGPRs
+            static_assert(PER_T == 2, "prefetch registers are written out for two per thread");
+            uint4 pa0, pa1, pb0, pb1, qa0, qa1, qb0, qb1;  // p: high plane, q: low plane
+            const uint32_t rbl0 = tid / (KSL * WAVE), rem0 = tid % (KSL * WAVE);
+            const uint32_t rbl1 = (tid + 256) / (KSL * WAVE), rem1 = (tid + 256) % (KSL * WAVE);
+#define MSJ_GLOAD(K0)                                                                  \
+    {                                                                                  \
+        const size_t ga0 = ((size_t)(rbA + rbl0) * KS + (K0)) * WAVE + rem0;           \
+        const size_t ga1 = ((size_t)(rbA + rbl1) * KS + (K0)) * WAVE + rem1;           \
+        const size_t gb0 = ((size_t)(rbB + rbl0) * KS + (K0)) * WAVE + rem0;           \
+        const size_t gb1 = ((size_t)(rbB + rbl1) * KS + (K0)) * WAVE + rem1;           \
+        pa0 = fh[ga0];                                                                 \
+        pa1 = fh[ga1];                                                                 \
+        pb0 = fh[gb0];                                                                 \
+        pb1 = fh[gb1];                                                                 \
+        if (SPLIT) {                                                                   \
+            qa0 = fl[ga0];                                                             \
+            qa1 = fl[ga1];                                                             \
+            qb0 = fl[gb0];                                                             \
+            qb1 = fl[gb1];                                                             \
+        }                                                                              \
+    }
+            MSJ_GLOAD(0u)
+            __syncthreads();  // the previous tile's epilogue is done with tB
+            if (tid < TM) {
+                const uint32_t r = cb + tid;
+                float4 t = thr[r];
+                if (r >= w.cend)
+                    t = make_float4(INF, INF, INF, INF);
+                tB[tid] = t;
+                oB[tid] = porig[r];
+                jB[tid] = pjoin[r];
+            }
+            floatx16 acc[2][2];
 #pragma unroll
-            for (uint32_t kk = 0; kk < KSL; ++kk) {
-                bf16x8 ah[2], al[2], bh[2], bl[2];
+            for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-                for (int bi = 0; bi < 2; ++bi) {
-                    const uint32_t o = ((2 * wr + bi) * KSL + kk) * WAVE + lane;
-                    ah[bi] = as_bf16x8(sA[0][o]);
-                    if (SPLIT)
-                        al[bi] = as_bf16x8(sA[NP - 1][o]);
+                for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+                    for (int e = 0; e < 16; ++e)
+                        acc[bi][bj][e] = 0.f;
+            for (uint32_t k0 = 0; k0 < KS; k0 += KSL) {
+                __syncthreads();  // all waves are done reading the previous slab
+                sA[0][tid] = pa0;
+                sA[0][tid + 256] = pa1;
+                sB[0][tid] = pb0;
+                sB[0][tid + 256] = pb1;
+                if (SPLIT) {
+                    sA[NP - 1][tid] = qa0;
+                    sA[NP - 1][tid + 256] = qa1;
+                    sB[NP - 1][tid] = qb0;
+                    sB[NP - 1][tid + 256] = qb1;
+                }
+                __syncthreads();
+                if (k0 + KSL < KS) {
+                    MSJ_GLOAD(k0 + KSL)  // in flight during this slab's MFMAs
                 }
 #pragma unroll
-                for (int bj = 0; bj < 2; ++bj) {
-                    const uint32_t o = ((2 * wc + bj) * KSL + kk) * WAVE + lane;
-                    bh[bj] = as_bf16x8(sB[0][o]);
-                    if (SPLIT)
-                        bl[bj] = as_bf16x8(sB[NP - 1][o]);
-                }
+                for (uint32_t kk = 0; kk < KSL; ++kk) {
+                    bf16x8 ah[2], al[2], bh[2], bl[2];
 #pragma unroll
-                for (int bi = 0; bi < 2; ++bi)
+                    for (int bi = 0; bi < 2; ++bi) {
+                        const uint32_t o = ((2 * wr + bi) * KSL + kk) * WAVE + lane;
+                        ah[bi] = as_bf16x8(sA[0][o]);
+                        if (SPLIT)
+                            al[bi] = as_bf16x8(sA[NP - 1][o]);
+                    }
 #pragma unroll
                     for (int bj = 0; bj < 2; ++bj) {
-                        acc[bi][bj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-                            ah[bi], bh[bj], acc[bi][bj], 0, 0, 0);
-                        if (SPLIT) {
-                            acc[bi][bj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-                                ah[bi], bl[bj], acc[bi][bj], 0, 0, 0);
-                            acc[bi][bj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-                                al[bi], bh[bj], acc[bi][bj], 0, 0, 0);
-                        }
+                        const uint32_t o = ((2 * wc + bj) * KSL + kk) * WAVE + lane;
+                        bh[bj] = as_bf16x8(sB[0][o]);
+                        if (SPLIT)
+                            bl[bj] = as_bf16x8(sB[NP - 1][o]);
                     }
-            }
-        }
-
-        // ---- epilogue: C[i][j] of tile (bi, bj) sits in lane j + 32*h,
-        // element v, with i = 8*(v/4) + 4*h + v%4 (h = lane / 32)
-        const uint32_t h = lane / 32;
-        uint32_t cidx[2];
-        bool cv[2];
-        float4 tb[2];
 #pragma unroll
-        for (int bj = 0; bj < 2; ++bj) {
-            const uint32_t jl = 64 * wc + 32 * bj + (lane & 31);
-            cidx[bj] = jl;
-            cv[bj] = cb + jl < w.cend;
-            tb[bj] = tB[jl];
-        }
-        bool need = false;
+                    for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-        for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-            for (int v = 0; v < 16; ++v) {
-                const uint32_t il = 64 * wr + 32 * bi + 8 * (v / 4) + 4 * h + (v % 4);
-                const uint32_t p = w.qbeg + il;
-                const float ay = tA[il].y;
-#pragma unroll
-                for (int bj = 0; bj < 2; ++bj) {
-                    const uint32_t c = cb + cidx[bj];
-                    const bool valid = p < w.qend && cv[bj] && (!w.self || p <= c);
-                    const bool rej = fmaf(-2.f, acc[bi][bj][v], ay + tb[bj].y) > hi;
-                    need = need || (valid && (sampled || !rej));
-                }
-            }
-        if (!__builtin_amdgcn_ballot_w64(need))
-            continue;
-#pragma unroll
-        for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-            for (int v = 0; v < 16; ++v) {
-                const uint32_t il = 64 * wr + 32 * bi + 8 * (v / 4) + 4 * h + (v % 4);
-                const uint32_t p = w.qbeg + il;
-                const float4 ta = tA[il];
-#pragma unroll
-                for (int bj = 0; bj < 2; ++bj) {
-                    const uint32_t jl = cidx[bj];
-                    const uint32_t c = cb + jl;
-                    const bool valid = p < w.qend && cv[bj] && (!w.self || p <= c);
-                    const float G = acc[bi][bj][v];
-                    const bool acc_ok = valid && fmaf(-2.f, G, ta.x + tb[bj].x) <= lo;
-                    const bool rej = fmaf(-2.f, G, ta.y + tb[bj].y) > hi;
-                    const bool inband = valid && !acc_ok && !rej;
-                    const uint64_t m1 = __builtin_amdgcn_ballot_w64(acc_ok);
-                    if (m1) {
-                        const bool two = acc_ok && p != c;
-                        const uint64_t m2 = __builtin_amdgcn_ballot_w64(two);
-                        const uint32_t tot = (uint32_t)(__popcll(m1) + __popcll(m2));
-                        if (fill + tot > MHITS)
-                            flush();
-                        if (acc_ok) {
-                            const unsigned long long po = oA[il], co = oB[jl];
-                            const uint32_t idx =
-                                fill + (uint32_t)(__popcll(m1 & lt) + __popcll(m2 & lt));
-                            hb[idx] = (po << qbits) | co;
-                            if (two)
-                                hb[idx + 1] = (co << qbits) | po;
-                        }
-                        fill += tot;
-                    }
-                    // band entries (and, when sampling, every valid pair)
-                    const bool push = sampled ? valid : inband;
-                    const uint64_t mb = __builtin_amdgcn_ballot_w64(push);
-                    if (mb) {
-                        unsigned long long base = 0;
-                        if (lane == 0)
-                            base = atomicAdd(&ctr[1], (unsigned long long)__popcll(mb));
-                        base = __shfl(base, 0, WAVE);
-                        if (push) {
-                            const unsigned long long idx = base + (unsigned long long)__popcll(mb & lt);
-                            if (idx < bcap) {
-                                const uint32_t cls =
-                                    sampled ? (acc_ok ? 1u : (inband ? 3u : 2u)) : 0u;
-                                band[idx] = ((unsigned long long)jA[il] << 32) |
-                                            ((unsigned long long)cls << 30) | jB[jl];
-                                bscore[idx] = fmaf(-2.f, G, ta.z + tb[bj].z);
+                        for (int bj = 0; bj < 2; ++bj) {
+                            acc[bi][bj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                                ah[bi], bh[bj], acc[bi][bj], 0, 0, 0);
+                            if (SPLIT) {
+                                acc[bi][bj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                                    ah[bi], bl[bj], acc[bi][bj], 0, 0, 0);
+                                acc[bi][bj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                                    al[bi], bh[bj], acc[bi][bj], 0, 0, 0);
                             }
                         }
-                    }
                 }
             }
+
+#undef MSJ_GLOAD
+            // ---- epilogue.  C[i][j] of MFMA tile (bi, bj) is in lane
+            // j + 32h, element e, with i = 8 (e / 4) + 4 h + e % 4.
+            const bool diag_tile = w.self && cb == w.qbeg;  // inside: pairs p <= c only
+            float4 tb[2];
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj)
+                tb[bj] = tB[64 * wc + 32 * bj + (lane & 31)];
+            bool need = sampled;
+#pragma unroll
+            for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                for (int e = 0; e < 16; ++e) {
+                    const uint32_t il = 64 * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
+                    const float ay = tA[il].y;
+#pragma unroll
+                    for (int bj = 0; bj < 2; ++bj) {
+                        const bool keep = !(fmaf(-2.f, acc[bi][bj][e], ay + tb[bj].y) > hi);
+                        if (diag_tile)
+                            need = need || (keep && il <= 64 * wc + 32 * bj + (lane & 31));
+                        else
+                            need = need || keep;
+                    }
+                }
+            if (!__builtin_amdgcn_ballot_w64(need))
+                continue;
+
+            uint64_t amask = 0;  // certain accepts, bit 32 bi + 2 e + bj
+#pragma unroll
+            for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                for (int e = 0; e < 16; ++e) {
+                    const uint32_t il = 64 * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
+                    const float4 ta = tA[il];
+                    const uint32_t p = w.qbeg + il;
+#pragma unroll
+                    for (int bj = 0; bj < 2; ++bj) {
+                        const uint32_t jl = 64 * wc + 32 * bj + (lane & 31);
+                        const uint32_t c = cb + jl;
+                        const bool valid = p < w.qend && c < w.cend && (!diag_tile || p <= c);
+                        const float G = acc[bi][bj][e];
+                        const bool a_ok = valid && fmaf(-2.f, G, ta.x + tb[bj].x) <= lo;
+                        const bool r_ok = fmaf(-2.f, G, ta.y + tb[bj].y) > hi;
+                        const bool inband = valid && !a_ok && !r_ok;
+                        amask |= (uint64_t)a_ok << (32 * bi + 2 * e + bj);
+                        uint32_t cls = 0;
+                        bool push = inband;
+                        if (DIAG) {
+                            cls = a_ok ? 1u : (inband ? 0u : 2u);
+                            push = inband || a_ok || (sampled && valid);
+                        }
+                        const uint64_t mb = __builtin_amdgcn_ballot_w64(push);
+                        if (mb) {
+                            const uint32_t k = (uint32_t)__popcll(mb);
+                            if (bfill + k > BHITS)
+                                bflush();
+                            if (push) {
+                                const uint32_t idx = bfill + (uint32_t)__popcll(mb & lt);
+                                bb[idx] = ((unsigned long long)jA[il] << 32) |
+                                          ((unsigned long long)cls << 30) | jB[jl];
+                                sb[idx] = fmaf(-2.f, G, ta.z + tb[bj].z);
+                            }
+                            bfill += k;
+                            b0fill += DIAG ? (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(inband))
+                                           : k;
+                        }
+                    }
+                }
+            // certain accepts: both orders (the self pair once)
+            while (true) {
+                const bool has = amask != 0;
+                const uint64_t m1 = __builtin_amdgcn_ballot_w64(has);
+                if (!m1)
+                    break;
+                uint32_t il = 0, jl = 0;
+                if (has) {
+                    const uint32_t k = (uint32_t)__builtin_ctzll(amask);
+                    amask &= amask - 1;
+                    const uint32_t bi = k >> 5, e = (k >> 1) & 15u, bj = k & 1u;
+                    il = 64 * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
+                    jl = 64 * wc + 32 * bj + (lane & 31);
+                }
+                const bool two = has && (w.qbeg + il) != (cb + jl);
+                const uint64_t m2 = __builtin_amdgcn_ballot_w64(two);
+                const uint32_t tot = (uint32_t)(__popcll(m1) + __popcll(m2));
+                if (fill + tot > MHITS)
+                    flush();
+                if (has) {
+                    const unsigned long long po = oA[il], co = oB[jl];
+                    const uint32_t idx = fill + (uint32_t)(__popcll(m1 & lt) + __popcll(m2 & lt));
+                    hb[idx] = (po << qbits) | co;
+                    if (two)
+                        hb[idx + 1] = (co << qbits) | po;
+                }
+                fill += tot;
+            }
+        }
     }
     if (fill)
         flush();
+    if (bfill)
+        bflush();
 }
 
 // FP64 decision of the band: the plain form's sequence over the whole padded
@@ -652,6 +734,7 @@ __global__ void k_zero_ctr(unsigned long long *c, uint32_t n)
 
 #define LAUNCH(kern, grid, block, ...)                                                   \
     do {                                                                                 \
+        ::msj::check_grid((grid), (block));                                              \
         hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, s, __VA_ARGS__);            \
         MSJ_HIP(hipGetLastError());                                                      \
     } while (0)
@@ -775,35 +858,44 @@ uint64_t MfmaRunner::run(const Index &ix, JoinRunner &J, const double *xr, uint3
                          double eps, uint64_t cap_hint, bool diag, hipStream_t s, hipEvent_t k0,
                          hipEvent_t k1, uint64_t *launches, float *kernel_ms, msj_stats *st)
 {
-    (void)ix;
     const double eps2 = eps * eps;  // launcher.cu:3200
     const uint32_t ds = padded_dim(d);
     const double g = (ds + 4.0) * 0x1p-52;
     const float lo = round_down_f(eps2 / ((1.0 + 0x1p-23) * (1.0 + g)) * (1.0 - 0x1p-30) - 0x1p-58);
     const float hi = round_up_f(eps2 / ((1.0 - 0x1p-24) * (1.0 - g)) * (1.0 + 0x1p-30) + 0x1p-58);
     const int qbits = std::max(1, bits_for(n ? n - 1 : 0));
-    const uint32_t sample_every = 61;
+    // BF16 band budget: beyond it the BF16 bound is too loose for this data
+    // and the join is rerun with split-BF16 (the kernel stops early)
+    const uint64_t bf16_budget = std::max<uint64_t>(16ull * n, 1ull << 22);
     ctr.ensure(8);
     if (J.pairs.cap < cap_hint)
         J.pairs.ensure(cap_hint);
-    uint64_t bwant = std::max<uint64_t>(std::max<uint64_t>(last_band + last_band / 4, 4ull * n),
-                                        1ull << 20);
-    if (band.cap < bwant) {
-        band.ensure(bwant);
-        bscore.ensure(bwant);
-    }
     *kernel_ms = 0.f;
-    const uint32_t per_xcd = (nitems + 7) / 8;
-    const uint32_t grid = per_xcd * 8;
-    for (int attempt = 0; attempt < 3; ++attempt) {
+    for (int attempt = 0; attempt < 4; ++attempt) {
+        // count_work: verify every pair of one item in sample_every (at least
+        // 61, about 5e7 sampled pairs at most)
+        const uint64_t pairs_bound = tiles * (uint64_t)TM * TM;
+        const uint32_t sample_every = (uint32_t)std::min<uint64_t>(
+            std::max<uint64_t>(61, pairs_bound / 50000000ull + 1), 0x7FFFFFFFull);
+        const uint64_t budget = split ? 0 : bf16_budget;
+        uint64_t bwant = std::max<uint64_t>(
+            std::max<uint64_t>(last_band + last_band / 4, 4ull * n), 1ull << 20);
+        if (!split)
+            bwant = std::max<uint64_t>(bwant, bf16_budget + bf16_budget / 8);
+        if (band.cap < bwant) {
+            band.ensure(bwant);
+            bscore.ensure(bwant);
+        }
+        const uint32_t per_xcd = (nitems + 7) / 8;
+        const uint32_t grid = std::max<uint32_t>(8, std::min<uint32_t>(per_xcd * 8, MAX_GRID));
         LAUNCH(k_zero_ctr, 1, 64, ctr.p, 8u);
         MSJ_HIP(hipEventRecord(k0, s));
         if (nitems) {
-#define MSJ_MJ(SP, DG)                                                                         \
+#define MSJ_MJ(SP, DG)                                                                          \
     LAUNCH((k_mfma_join<SP, DG>), grid, 256, fh.p, SP ? fl.p : fh.p, KS, thr.p, porig.p, pjoin.p, \
-           items.p, nitems, per_xcd, lo, hi, qbits, (unsigned long long *)J.pairs.p,           \
-           (unsigned long long)J.pairs.cap, ctr.p, (unsigned long long *)band.p, bscore.p,     \
-           (unsigned long long)band.cap, sample_every)
+           items.p, nitems, per_xcd, lo, hi, qbits, (unsigned long long *)J.pairs.p,            \
+           (unsigned long long)J.pairs.cap, ctr.p, (unsigned long long *)band.p, bscore.p,      \
+           (unsigned long long)band.cap, sample_every, (unsigned long long)budget)
             if (split) {
                 if (diag)
                     MSJ_MJ(true, true);
@@ -836,6 +928,12 @@ uint64_t MfmaRunner::run(const Index &ix, JoinRunner &J, const double *xr, uint3
         MSJ_HIP(hipEventElapsedTime(&ms, k0, k1));
         *kernel_ms += ms;
         const uint64_t R = h[0], nband = h[1];
+        if (!split && h[7] > bf16_budget) {
+            // escalate: the same items with split-BF16 operands
+            prepare(ix, J, xr, n, d, eps, true, s);
+            last_band = 0;
+            continue;
+        }
         if (R <= J.pairs.cap && nband <= band.cap) {
             last_band = nband;
             double emax, esum;
@@ -846,6 +944,8 @@ uint64_t MfmaRunner::run(const Index &ix, JoinRunner &J, const double *xr, uint3
             st->mean_score_err = nband ? esum / (double)nband : 0.0;
             st->filter_checked = h[3];
             st->filter_violations = h[4];
+            st->form_used = split ? MSJ_FORM_MFMA_SPLIT : MSJ_FORM_MFMA_BF16;
+            st->mfma_flops = (double)tiles * TM * TM * 2.0 * (16.0 * KS) * (split ? 3.0 : 1.0);
             return R;
         }
         if (nband > band.cap) {
