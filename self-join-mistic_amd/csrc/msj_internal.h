@@ -207,11 +207,20 @@ struct JoinRunner {
 };
 
 // ------------------------------------------------------------ MFMA filter
-// One workgroup of the filter kernel: A rows [qbeg, qend) against B rows
-// [cbeg, cend) in the padded row space (nodes start on 32-row boundaries);
-// self != 0 when both ranges lie in one node (then only pairs p <= c).
+// A super-block of the filter: s x s tiles of 128 x 128 pairs, A rows from
+// qbeg (node X ends at qend) against B rows from cbeg (node Y ends at cend),
+// in the padded row space (nodes start on 32-row boundaries).  self: s in
+// bits 0-7, bit 8 X == Y (pairs p <= c only), bit 9 a diagonal super-block
+// (tiles on or above its diagonal only).
 struct MItem {
     uint32_t qbeg, qend, cbeg, cend, self;
+};
+
+// Tile slots -> super-blocks: items of class c (s = 8 - c) start at
+// item_base[c] and own slots [slot_base[c], slot_base[c+1]), s*s each.
+struct SlotMap {
+    unsigned long long slot_base[9];
+    unsigned long long item_base[8];
 };
 
 // Precision form: pairs are scored on the matrix cores from BF16 (or
@@ -227,17 +236,19 @@ struct MfmaRunner {
     DevBuf<float4> thr;          // [P]   per-row threshold terms
     DevBuf<double> mu, colpart;  // centre (column means) and partial sums
     DevBuf<uint64_t> p64;        // scan scratch
-    DevBuf<uint32_t> ecnt;       // per upper edge: work items
+    DevBuf<uint32_t> cnt8;       // [8][E] super-blocks per (class, upper edge)
     DevBuf<uint64_t> etiles;     // per upper edge: 128 x 128 tiles
-    DevBuf<uint64_t> eoff;       // exclusive scan of ecnt
+    DevBuf<uint64_t> off8;       // exclusive scan of cnt8
     DevBuf<uint32_t> esrc;       // upper edge -> source node
     DevBuf<MItem> items;
     DevBuf<uint64_t> band;       // band pairs (join positions, class bits)
     DevBuf<float> bscore;        // their filter scores
     DevBuf<unsigned long long> ctr;  // [0] pairs, [1] band, [2] err bits, [3] checked,
                                      // [4] violations, [5] err sum (double bits)
-    uint32_t P = 0, KS = 0, nitems = 0;
+    uint32_t P = 0, KS = 0;
+    uint64_t nitems = 0;         // super-blocks
     uint64_t tiles = 0;          // 128 x 128 tiles of the last prepare()
+    SlotMap smap = {};
     bool split = false;
     Scanner scan;
     // Padded rows, centre, fragments and thresholds from the node-ordered

@@ -258,6 +258,183 @@ __global__ __launch_bounds__(256) void k_join_sym(
     }
 }
 
+// High-dimensional rows (ds > 32).  Slab-major order: a wave holds the
+// 64 candidates' next HSL dimensions in VGPRs (each loaded once) and
+// advances the partial sums of all of the item's queries (up to 32, one
+// FP64 sum per lane and query in VGPRs) through that slab, chunk by chunk in
+// the reference's order, with the same early-exit test; a query row slab is
+// staged in LDS and read as a broadcast.  Decisions are those of the
+// per-pair sequence: the chunks of every pair run in order, each followed by
+// the s > eps^2 test.
+constexpr int HSL = 16;  // dimensions per slab
+constexpr int HQ = 32;   // queries per item
+
+template <bool COUNT>
+__global__ __launch_bounds__(256) void k_join_hd(
+    const double *__restrict__ xr, uint32_t d, uint32_t ds, double eps2,
+    const uint32_t *__restrict__ nstart, const uint32_t *__restrict__ uoff,
+    const uint32_t *__restrict__ uadj, const uint32_t *__restrict__ perm,
+    const JoinItem *__restrict__ items, uint32_t nitems, int qbits,
+    unsigned long long *__restrict__ out, unsigned long long *__restrict__ counter,
+    unsigned long long cap, unsigned long long *__restrict__ work)
+{
+    __shared__ double qsh[WPB][HQ * HSL];
+    __shared__ unsigned long long hsh[WPB][HITS];
+    const uint32_t lane = threadIdx.x & (WAVE - 1);
+    const uint32_t wv = threadIdx.x / WAVE;
+    const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (WAVE - lane));
+    unsigned long long *hb = hsh[wv];
+    double *qs = qsh[wv];
+    uint32_t fill = 0;  // wave-uniform
+    uint64_t dims = 0;
+    auto wave_sync = [&]() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+    auto flush = [&]() {
+        wave_sync();
+        unsigned long long base = 0;
+        if (lane == 0)
+            base = atomicAdd(counter, (unsigned long long)fill);
+        base = __shfl(base, 0, WAVE);
+        for (uint32_t e = lane; e < fill; e += WAVE)
+            if (base + e < cap)
+                out[base + e] = hb[e];
+        wave_sync();
+        fill = 0;
+    };
+    // one wave per item, grid-stride (no block-wide synchronisation here)
+    for (uint64_t it0 = (uint64_t)blockIdx.x * WPB + wv; it0 < nitems;
+         it0 += (uint64_t)gridDim.x * WPB) {
+        const uint32_t it = __builtin_amdgcn_readfirstlane((uint32_t)it0);
+        const JoinItem w = items[it];
+        const uint32_t X = w.x;
+        const uint32_t x0 = nstart[X];
+        const uint32_t qn = w.qend - w.qbeg;  // <= HQ
+        const uint32_t q0 = x0 + w.qbeg;      // first query (node order)
+        uint32_t a = uoff[X];
+        uint32_t Y = uadj[a];
+        uint32_t ybase = 0;
+        uint32_t ystart = nstart[Y];
+        uint32_t ysz = nstart[Y + 1] - ystart;
+        for (uint32_t cs = w.cbeg; cs < w.cend; cs += WAVE) {
+            const uint32_t ci = cs + lane;
+            const bool cvalid = ci < w.cend;
+            uint32_t c = 0;
+            if (cvalid) {
+                while (ci >= ybase + ysz) {
+                    ybase += ysz;
+                    ++a;
+                    Y = uadj[a];
+                    ystart = nstart[Y];
+                    ysz = nstart[Y + 1] - ystart;
+                }
+                c = ystart + (ci - ybase);
+            }
+            const bool self_node = cvalid && Y == X;
+            uint32_t alive = 0;  // bit q: pair (query q, this lane's candidate) undecided
+            for (uint32_t q = 0; q < qn; ++q)
+                if (cvalid && (!self_node || q0 + q <= c))
+                    alive |= 1u << q;
+            double sq[HQ];
+#pragma unroll
+            for (int q = 0; q < HQ; ++q)
+                sq[q] = 0.0;
+            const double *crow = xr + (size_t)c * ds;
+            for (uint32_t l0 = 0; l0 < ds; l0 += HSL) {
+                if (!wave_vote(alive != 0))
+                    break;
+                // query slab -> LDS (qn rows of HSL dims), candidate slab -> VGPRs
+                wave_sync();
+                for (uint32_t e = lane; e < HQ * HSL; e += WAVE) {
+                    const uint32_t q = e / HSL, j = e % HSL;
+                    qs[e] = (q < qn && l0 + j < ds) ? xr[(size_t)(q0 + q) * ds + l0 + j] : 0.0;
+                }
+                double cc[HSL];
+#pragma unroll
+                for (int j = 0; j < HSL; j += 2) {
+                    if (l0 + j < ds) {
+                        const double2 v = *(const double2 *)(crow + l0 + j);
+                        cc[j] = v.x;
+                        cc[j + 1] = v.y;
+                    } else {
+                        cc[j] = 0.0;
+                        cc[j + 1] = 0.0;
+                    }
+                }
+                wave_sync();
+                // fully unrolled over q (sq[] stays in VGPRs): no break or
+                // continue; a decided query or chunk is skipped by its vote
+#pragma unroll
+                for (int q = 0; q < HQ; ++q) {
+                    bool al = (alive >> q) & 1u;  // 0 for q >= qn
+                    if (!wave_vote(al))
+                        continue;
+                    const double *qrow = qs + q * HSL;  // wave-uniform LDS address
+                    double s = sq[q];
+#pragma unroll
+                    for (int k = 0; k < HSL; k += 4) {
+                        if (l0 + k >= ds || !wave_vote(al))
+                            continue;
+                        if (COUNT && al && l0 + k < d)
+                            dims += min(4u, d - (l0 + k));
+                        const double2 qa = *(const double2 *)(qrow + k);
+                        const double2 qb = *(const double2 *)(qrow + k + 2);
+                        double t0 = qa.x - cc[k];
+                        s = fma(t0, t0, s);
+                        double t1 = qa.y - cc[k + 1];
+                        double p1 = t1 * t1;
+                        s = s + p1;
+                        double t2 = qb.x - cc[k + 2];
+                        double p2 = t2 * t2;
+                        s = s + p2;
+                        double t3 = qb.y - cc[k + 3];
+                        double p3 = t3 * t3;
+                        s = s + p3;
+                        al = al && !(s > eps2);
+                    }
+                    sq[q] = s;
+                    if (!al)
+                        alive &= ~(1u << q);
+                }
+            }
+            // emission: the pairs still undecided after every slab are results
+            const unsigned long long corig = perm[c];
+            for (uint32_t q = 0; q < qn; ++q) {
+                const bool acc = (alive >> q) & 1u;
+                const uint64_t m1 = wave_vote(acc);
+                if (!m1)
+                    continue;
+                const uint32_t p = q0 + q;
+                const bool two = acc && c != p;
+                const uint64_t m2 = wave_vote(two);
+                const uint32_t tot = (uint32_t)(__popcll(m1) + __popcll(m2));
+                if (fill + tot > HITS)
+                    flush();
+                if (acc) {
+                    const unsigned long long porig = perm[p];
+                    const uint32_t idx = fill + (uint32_t)(__popcll(m1 & lt) + __popcll(m2 & lt));
+                    hb[idx] = (porig << qbits) | corig;
+                    if (two)
+                        hb[idx + 1] = (corig << qbits) | porig;
+                }
+                fill += tot;
+            }
+        }
+    }
+    if (fill)
+        flush();
+    if (COUNT) {
+        unsigned long long v = dims;
+#pragma unroll
+        for (int o = WAVE / 2; o > 0; o >>= 1)
+            v += __shfl_down(v, o, WAVE);
+        if (lane == 0)
+            atomicAdd(work, v);
+    }
+}
+
 // Upper neighbour lists: entries Y >= X of X's adjacency, and their sizes.
 __global__ __launch_bounds__(256) void k_upper_count(const uint32_t *__restrict__ nstart,
                                                      const uint32_t *__restrict__ aoff,
@@ -524,7 +701,14 @@ void dispatch_sym(const JoinRunner &J, const Index &ix, const double *xr, uint32
         MSJ_DS(32)
 #undef MSJ_DS
     default:
-        launch_sym<32, true>(J, ix, xr, d, ds, eps2, qbits, out, cap, work, s);
+        if (work)
+            LAUNCH(k_join_hd<true>, grid_stride(J.nitems, WPB), WAVE * WPB, xr, d, ds, eps2,
+                   ix.nstart.p, J.uoff.p, J.uadj.p, J.jperm.p, J.items.p, J.nitems, qbits, out,
+                   J.counter.p, cap, work);
+        else
+            LAUNCH(k_join_hd<false>, grid_stride(J.nitems, WPB), WAVE * WPB, xr, d, ds, eps2,
+                   ix.nstart.p, J.uoff.p, J.uadj.p, J.jperm.p, J.items.p, J.nitems, qbits, out,
+                   J.counter.p, cap, work);
     }
 }
 
@@ -566,7 +750,7 @@ void JoinRunner::plan(const Index &ix, uint32_t n, uint32_t d, hipStream_t s, bo
 {
     const uint32_t M = ix.M;
     const uint32_t ds = padded_dim(d);
-    const uint32_t qmax = qmax_for(ds <= 32 ? ds : 32u);
+    const uint32_t qmax = ds <= 32 ? qmax_for(ds) : (uint32_t)HQ;
     itcnt.ensure(M);
     itoff.ensure((size_t)M + 1);
     ulen.ensure(M);

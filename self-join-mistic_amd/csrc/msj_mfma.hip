@@ -63,7 +63,7 @@ namespace {
 
 constexpr uint32_t TM = 128;     // rows per workgroup tile, A and B
 constexpr uint32_t KSL = 2;      // 16-dim k-steps per LDS slab
-constexpr uint32_t CBT = 16;     // B tiles per work item
+constexpr uint32_t SBS = 8;      // tiles per side of a super-block
 constexpr uint32_t MHITS = 256;  // per-wave LDS buffer of result keys
 constexpr uint32_t KALIGN = 16 * KSL;
 
@@ -226,89 +226,82 @@ __global__ __launch_bounds__(256) void k_edge_src(const uint32_t *__restrict__ u
         esrc[e] = x;
 }
 
-// sum_{j=1..m} ceil(j / k)
-__host__ __device__ inline uint64_t tri_chunks(uint64_t m, uint64_t k)
-{
-    const uint64_t q = m / k, r = m % k;
-    return k * q * (q + 1) / 2 + r * (q + 1);
-}
-
-__global__ __launch_bounds__(256) void k_mitem_count(const uint32_t *__restrict__ nstart,
-                                                     const uint32_t *__restrict__ esrc,
-                                                     const uint32_t *__restrict__ uadj,
-                                                     uint32_t E, uint32_t *__restrict__ cnt,
-                                                     uint64_t *__restrict__ tiles)
+// Work is cut into 128 x 128 tiles of (A rows of X) x (B rows of Y), and the
+// tiles into super-blocks of s x s tiles, s = min(8, tiles of X, tiles of
+// Y); inside X only super-blocks and tiles on or above the diagonal.  The
+// super-blocks are the work items; items are grouped by s (class 8 - s,
+// largest s first) so that a tile slot t maps to its item arithmetically.
+// Consecutive slots, which run concurrently on one XCD, lie in one
+// super-block: 8 A tiles and 8 B tiles shared through that XCD's L2.
+__global__ __launch_bounds__(256) void k_sblk_count(const uint32_t *__restrict__ nstart,
+                                                    const uint32_t *__restrict__ esrc,
+                                                    const uint32_t *__restrict__ uadj,
+                                                    uint32_t E, uint32_t *__restrict__ cnt8,
+                                                    uint64_t *__restrict__ tiles)
 {
     uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= E)
         return;
     const uint32_t X = esrc[e], Y = uadj[e];
     const uint64_t TA = (nstart[X + 1] - nstart[X] + TM - 1) / TM;
-    uint64_t c, t;
-    if (X == Y) {
-        c = tri_chunks(TA, CBT);
-        t = TA * (TA + 1) / 2;
-    } else {
-        const uint64_t TB = (nstart[Y + 1] - nstart[Y] + TM - 1) / TM;
-        c = TA * ((TB + CBT - 1) / CBT);
-        t = TA * TB;
-    }
-    cnt[e] = (uint32_t)c;
-    tiles[e] = t;
+    const uint64_t TB = (nstart[Y + 1] - nstart[Y] + TM - 1) / TM;
+    const uint64_t sz = std::min<uint64_t>(SBS, std::min(TA, TB));
+    const uint64_t SA = (TA + sz - 1) / sz, SB = (TB + sz - 1) / sz;
+    const uint64_t nsb = X == Y ? SA * (SA + 1) / 2 : SA * SB;
+    const uint32_t cls = SBS - (uint32_t)sz;
+    for (uint32_t c = 0; c < SBS; ++c)
+        cnt8[(size_t)c * E + e] = c == cls ? (uint32_t)nsb : 0u;
+    tiles[e] = X == Y ? TA * (TA + 1) / 2 : TA * TB;
 }
 
-__global__ __launch_bounds__(256) void k_mitem_fill(const uint32_t *__restrict__ nstart,
-                                                    const uint32_t *__restrict__ pstart,
-                                                    const uint32_t *__restrict__ esrc,
-                                                    const uint32_t *__restrict__ uadj,
-                                                    const uint64_t *__restrict__ eoff, uint32_t E,
-                                                    uint32_t nitems, MItem *__restrict__ items)
+__global__ __launch_bounds__(256) void k_sblk_fill(const uint32_t *__restrict__ nstart,
+                                                   const uint32_t *__restrict__ pstart,
+                                                   const uint32_t *__restrict__ esrc,
+                                                   const uint32_t *__restrict__ uadj,
+                                                   const uint64_t *__restrict__ off8, uint32_t E,
+                                                   uint64_t nitems, MItem *__restrict__ items)
 {
-    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nitems)
         return;
-    uint32_t lo = 0, hi = E;  // edge with eoff[e] <= k < eoff[e+1]
+    uint64_t lo = 0, hi = (uint64_t)SBS * E;  // entry with off8[i] <= k < off8[i+1]
     while (hi - lo > 1) {
-        uint32_t mid = lo + (hi - lo) / 2;
-        if (eoff[mid] <= k)
+        uint64_t mid = lo + (hi - lo) / 2;
+        if (off8[mid] <= k)
             lo = mid;
         else
             hi = mid;
     }
-    const uint32_t e = lo;
-    const uint64_t l = k - eoff[e];
+    const uint32_t cls = (uint32_t)(lo / E), e = (uint32_t)(lo % E);
+    const uint64_t l = k - off8[lo];
     const uint32_t X = esrc[e], Y = uadj[e];
     const uint32_t sx = nstart[X + 1] - nstart[X], sy = nstart[Y + 1] - nstart[Y];
-    const uint64_t TA = (sx + TM - 1) / TM;
-    uint64_t a, t0, t1;
+    const uint64_t sz = SBS - cls;
+    const uint64_t TA = (sx + TM - 1) / TM, TB = (sy + TM - 1) / TM;
+    const uint64_t SA = (TA + sz - 1) / sz, SB = (TB + sz - 1) / sz;
+    uint64_t I, J;
     if (X == Y) {
-        // a = largest tile row with F(a) = tri(TA) - tri(TA - a) <= l
-        const uint64_t tot = tri_chunks(TA, CBT);
-        uint64_t alo = 0, ahi = TA;
-        while (ahi - alo > 1) {
-            uint64_t mid = alo + (ahi - alo) / 2;
-            if (tot - tri_chunks(TA - mid, CBT) <= l)
-                alo = mid;
+        // row I holds super-blocks J = I .. SA-1; F(I) = I SA - I (I - 1) / 2
+        uint64_t ilo = 0, ihi = SA;
+        while (ihi - ilo > 1) {
+            uint64_t mid = ilo + (ihi - ilo) / 2;
+            if (mid * SA - mid * (mid - 1) / 2 <= l)
+                ilo = mid;
             else
-                ahi = mid;
+                ihi = mid;
         }
-        a = alo;
-        const uint64_t c = l - (tot - tri_chunks(TA - a, CBT));
-        t0 = a + c * CBT;
-        t1 = std::min<uint64_t>(TA, t0 + CBT);
+        I = ilo;
+        J = I + (l - (I * SA - I * (I - 1) / 2));
     } else {
-        const uint64_t TB = (sy + TM - 1) / TM;
-        const uint64_t nc = (TB + CBT - 1) / CBT;
-        a = l / nc;
-        t0 = (l % nc) * CBT;
-        t1 = std::min<uint64_t>(TB, t0 + CBT);
+        I = l / SB;
+        J = l % SB;
     }
     MItem w;
-    w.qbeg = pstart[X] + (uint32_t)(a * TM);
+    w.qbeg = pstart[X] + (uint32_t)(I * sz * TM);
     w.qend = pstart[X] + sx;
-    w.cbeg = pstart[Y] + (uint32_t)(t0 * TM);
-    w.cend = pstart[Y] + std::min<uint32_t>(sy, (uint32_t)(t1 * TM));
-    w.self = X == Y ? 1u : 0u;
+    w.cbeg = pstart[Y] + (uint32_t)(J * sz * TM);
+    w.cend = pstart[Y] + sy;
+    w.self = (uint32_t)sz | (X == Y ? 0x100u : 0u) | (X == Y && I == J ? 0x200u : 0u);
     items[k] = w;
 }
 
@@ -335,8 +328,8 @@ template <bool SPLIT, bool DIAG>
 __global__ __launch_bounds__(256, 2) void k_mfma_join(
     const uint4 *__restrict__ fh, const uint4 *__restrict__ fl, uint32_t KS,
     const float4 *__restrict__ thr, const uint32_t *__restrict__ porig,
-    const uint32_t *__restrict__ pjoin, const MItem *__restrict__ items, uint32_t nitems,
-    uint32_t per_xcd, float lo, float hi, int qbits, unsigned long long *__restrict__ out,
+    const uint32_t *__restrict__ pjoin, const MItem *__restrict__ items, SlotMap sm,
+    uint64_t per_xcd, float lo, float hi, int qbits, unsigned long long *__restrict__ out,
     unsigned long long cap, unsigned long long *__restrict__ ctr,
     unsigned long long *__restrict__ band, float *__restrict__ bscore, unsigned long long bcap,
     uint32_t sample_every, unsigned long long band_budget)
@@ -353,6 +346,8 @@ __global__ __launch_bounds__(256, 2) void k_mfma_join(
     __shared__ unsigned long long bsh[4][BHITS];
     __shared__ float ssh[4][BHITS];
     __shared__ uint32_t s_stop;
+    if (threadIdx.x == 0)
+        s_stop = 0;
 
     const uint32_t tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid / WAVE;
     const uint32_t wr = wv >> 1, wc = wv & 1, h = lane / 32;
@@ -398,37 +393,53 @@ __global__ __launch_bounds__(256, 2) void k_mfma_join(
         b0fill = 0;
     };
 
-    // Work items in a grid-stride loop over virtual block ids; id v runs
-    // item (v % 8) * per_xcd + v / 8, so the items of one XCD (the hardware
-    // deals workgroups out round robin) are consecutive: same A rows,
-    // neighbouring B rows, shared in that XCD's L2.
+    // Tile slots in a grid-stride loop over virtual block ids; id v runs
+    // slot (v % 8) * per_xcd + v / 8, so the slots of one XCD (the hardware
+    // deals workgroups out round robin) are consecutive: one super-block.
     const uint64_t vtotal = 8ull * per_xcd;
     for (uint64_t v = blockIdx.x; v < vtotal; v += gridDim.x) {
-        const uint32_t it = (uint32_t)((v % 8u) * per_xcd + v / 8u);
-        __syncthreads();  // the previous item is done with tA and the slabs
-        if (tid == 0)
-            s_stop = band_budget != 0 &&
-                     __hip_atomic_load(&ctr[7], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >
-                         band_budget;
-        if (it < nitems && tid < TM) {
-            const MItem w = items[it];
-            const uint32_t r = w.qbeg + tid;
-            float4 t = thr[r];
-            if (r >= w.qend)
-                t = make_float4(INF, INF, INF, INF);
-            tA[tid] = t;
-            oA[tid] = porig[r];
-            jA[tid] = pjoin[r];
+        const uint64_t t = (v % 8u) * per_xcd + v / 8u;
+        bool have = t < sm.slot_base[SBS];
+        uint32_t cls = 0;
+        while (have && cls + 1 < SBS && t >= sm.slot_base[cls + 1])
+            ++cls;
+        const uint32_t sz = SBS - cls;
+        const uint64_t local = have ? t - sm.slot_base[cls] : 0;
+        const uint64_t it = sm.item_base[cls] + local / (sz * sz);
+        const uint32_t r = (uint32_t)(local % (sz * sz)), ta = r / sz, tb = r % sz;
+        MItem w = {0, 0, 0, 0, 0};
+        if (have)
+            w = items[it];
+        const uint32_t qbeg = w.qbeg + ta * TM, cb = w.cbeg + tb * TM;
+        have = have && qbeg < w.qend && cb < w.cend && !((w.self & 0x200u) && tb < ta);
+        __syncthreads();  // the previous tile is done with tA, tB and the slabs
+        // the band counter is read every 32nd slot of a workgroup: one global
+        // address polled by every workgroup on every tile is a hot spot
+        if (tid == 0 && band_budget != 0 && ((v / gridDim.x) & 31u) == 0)
+            s_stop = __hip_atomic_load(&ctr[7], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >
+                     band_budget;
+        if (have && tid < TM) {
+            const uint32_t ra_ = qbeg + tid, rb_ = cb + tid;
+            float4 t4 = thr[ra_];
+            if (ra_ >= w.qend)
+                t4 = make_float4(INF, INF, INF, INF);
+            tA[tid] = t4;
+            oA[tid] = porig[ra_];
+            jA[tid] = pjoin[ra_];
+            t4 = thr[rb_];
+            if (rb_ >= w.cend)
+                t4 = make_float4(INF, INF, INF, INF);
+            tB[tid] = t4;
+            oB[tid] = porig[rb_];
+            jB[tid] = pjoin[rb_];
         }
         __syncthreads();
-        if (it >= nitems || s_stop)
+        if (!have || s_stop)
             continue;  // uniform over the workgroup
-        const MItem w = items[it];
         const bool sampled = DIAG && (it % sample_every == 0);
-        const uint32_t rbA = w.qbeg / 32;
-
-        for (uint32_t cb = w.cbeg; cb < w.cend; cb += TM) {
-            const uint32_t rbB = cb / 32;
+        const bool self = (w.self & 0x100u) != 0;
+        const uint32_t rbA = qbeg / 32, rbB = cb / 32;
+        {
             // two uint4 per thread per plane and operand (SLAB = 2 * 256):
             // named registers, so the prefetch stays in VGPRs
             static_assert(PER_T == 2, "prefetch registers are written out for two per thread");
@@ -453,16 +464,6 @@ __global__ __launch_bounds__(256, 2) void k_mfma_join(
         }                                                                              \
     }
             MSJ_GLOAD(0u)
-            __syncthreads();  // the previous tile's epilogue is done with tB
-            if (tid < TM) {
-                const uint32_t r = cb + tid;
-                float4 t = thr[r];
-                if (r >= w.cend)
-                    t = make_float4(INF, INF, INF, INF);
-                tB[tid] = t;
-                oB[tid] = porig[r];
-                jB[tid] = pjoin[r];
-            }
             floatx16 acc[2][2];
 #pragma unroll
             for (int bi = 0; bi < 2; ++bi)
@@ -523,7 +524,7 @@ __global__ __launch_bounds__(256, 2) void k_mfma_join(
 #undef MSJ_GLOAD
             // ---- epilogue.  C[i][j] of MFMA tile (bi, bj) is in lane
             // j + 32h, element e, with i = 8 (e / 4) + 4 h + e % 4.
-            const bool diag_tile = w.self && cb == w.qbeg;  // inside: pairs p <= c only
+            const bool diag_tile = self && cb == qbeg;  // inside: pairs p <= c only
             float4 tb[2];
 #pragma unroll
             for (int bj = 0; bj < 2; ++bj)
@@ -554,7 +555,7 @@ __global__ __launch_bounds__(256, 2) void k_mfma_join(
                 for (int e = 0; e < 16; ++e) {
                     const uint32_t il = 64 * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
                     const float4 ta = tA[il];
-                    const uint32_t p = w.qbeg + il;
+                    const uint32_t p = qbeg + il;
 #pragma unroll
                     for (int bj = 0; bj < 2; ++bj) {
                         const uint32_t jl = 64 * wc + 32 * bj + (lane & 31);
@@ -602,7 +603,7 @@ __global__ __launch_bounds__(256, 2) void k_mfma_join(
                     il = 64 * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
                     jl = 64 * wc + 32 * bj + (lane & 31);
                 }
-                const bool two = has && (w.qbeg + il) != (cb + jl);
+                const bool two = has && (qbeg + il) != (cb + jl);
                 const uint64_t m2 = __builtin_amdgcn_ballot_w64(two);
                 const uint32_t tot = (uint32_t)(__popcll(m1) + __popcll(m2));
                 if (fill + tot > MHITS)
@@ -827,31 +828,47 @@ void MfmaRunner::prepare(const Index &ix, const JoinRunner &J, const double *xr,
         LAUNCH(k_frag_prep<false>, grid_for(Ptot, 4), 256, xr, ds, KS, mu.p, pjoin.p, Ptot, C,
                fh.p, fh.p, thr.p);
 
-    // work items over the upper neighbour lists
+    // work items: super-blocks over the upper neighbour lists
     uint32_t E = 0;
     MSJ_HIP(hipMemcpyAsync(&E, J.uoff.p + M, sizeof(E), hipMemcpyDeviceToHost, s));
     MSJ_HIP(hipStreamSynchronize(s));
-    esrc.ensure(std::max<uint32_t>(E, 1));
-    ecnt.ensure(std::max<uint32_t>(E, 1));
+    nitems = 0;
+    tiles = 0;
+    smap = SlotMap{};
+    if (E == 0)
+        return;
+    const size_t E8 = (size_t)SBS * E;
+    esrc.ensure(E);
+    cnt8.ensure(E8);
     etiles.ensure((size_t)E + 1);
-    eoff.ensure((size_t)E + 1);
+    off8.ensure(E8 + 1);
     LAUNCH(k_edge_src, grid_for(M, 256), 256, J.uoff.p, M, esrc.p);
-    LAUNCH(k_mitem_count, grid_for(E, 256), 256, ix.nstart.p, esrc.p, J.uadj.p, E, ecnt.p,
+    LAUNCH(k_sblk_count, grid_for(E, 256), 256, ix.nstart.p, esrc.p, J.uadj.p, E, cnt8.p,
            etiles.p);
-    scan.excl_u32_to_u64(ecnt.p, eoff.p, E, s);
+    scan.excl_u32_to_u64(cnt8.p, off8.p, E8, s);
     p64.ensure((size_t)E + 1);  // pstart is converted already; contents not needed
     scan.excl_u64_to_u64(etiles.p, p64.p, E, s);
-    uint64_t total = 0;
-    MSJ_HIP(hipMemcpyAsync(&total, eoff.p + E, sizeof(total), hipMemcpyDeviceToHost, s));
+    std::vector<uint64_t> cb(SBS + 1);
+    for (uint32_t c = 0; c <= SBS; ++c)
+        MSJ_HIP(hipMemcpyAsync(&cb[c], off8.p + (size_t)c * E, sizeof(uint64_t),
+                               hipMemcpyDeviceToHost, s));
     MSJ_HIP(hipMemcpyAsync(&tiles, p64.p + E, sizeof(tiles), hipMemcpyDeviceToHost, s));
     MSJ_HIP(hipStreamSynchronize(s));
-    if (total >= 0xFFFFFFF0ull)
-        throw Error(MSJ_E_OVERFLOW, "filter work items exceed 2^32");
-    nitems = (uint32_t)total;
-    items.ensure(std::max<uint32_t>(nitems, 1));
+    nitems = cb[SBS];
+    uint64_t slot = 0;
+    for (uint32_t c = 0; c < SBS; ++c) {
+        const uint64_t sz = SBS - c;
+        smap.item_base[c] = cb[c];
+        smap.slot_base[c] = slot;
+        slot += (cb[c + 1] - cb[c]) * sz * sz;
+    }
+    smap.slot_base[SBS] = slot;
+    if (slot >= (1ull << 40))
+        throw Error(MSJ_E_OVERFLOW, "filter tile slots exceed 2^40");
+    items.ensure(std::max<uint64_t>(nitems, 1));
     if (nitems)
-        LAUNCH(k_mitem_fill, grid_for(nitems, 256), 256, ix.nstart.p, pstart.p, esrc.p, J.uadj.p,
-               eoff.p, E, nitems, items.p);
+        LAUNCH(k_sblk_fill, grid_for(nitems, 256), 256, ix.nstart.p, pstart.p, esrc.p, J.uadj.p,
+               off8.p, E, nitems, items.p);
 }
 
 uint64_t MfmaRunner::run(const Index &ix, JoinRunner &J, const double *xr, uint32_t n, uint32_t d,
@@ -879,21 +896,23 @@ uint64_t MfmaRunner::run(const Index &ix, JoinRunner &J, const double *xr, uint3
             std::max<uint64_t>(61, pairs_bound / 50000000ull + 1), 0x7FFFFFFFull);
         const uint64_t budget = split ? 0 : bf16_budget;
         uint64_t bwant = std::max<uint64_t>(
-            std::max<uint64_t>(last_band + last_band / 4, 4ull * n), 1ull << 20);
+            std::max<uint64_t>(last_band + last_band / 4, 16ull * n), 1ull << 20);
         if (!split)
             bwant = std::max<uint64_t>(bwant, bf16_budget + bf16_budget / 8);
         if (band.cap < bwant) {
             band.ensure(bwant);
             bscore.ensure(bwant);
         }
-        const uint32_t per_xcd = (nitems + 7) / 8;
-        const uint32_t grid = std::max<uint32_t>(8, std::min<uint32_t>(per_xcd * 8, MAX_GRID));
+        const uint64_t slots = smap.slot_base[SBS];
+        const uint64_t per_xcd = (slots + 7) / 8;
+        const uint32_t grid =
+            (uint32_t)std::max<uint64_t>(8, std::min<uint64_t>(per_xcd * 8, MAX_GRID));
         LAUNCH(k_zero_ctr, 1, 64, ctr.p, 8u);
         MSJ_HIP(hipEventRecord(k0, s));
         if (nitems) {
 #define MSJ_MJ(SP, DG)                                                                          \
     LAUNCH((k_mfma_join<SP, DG>), grid, 256, fh.p, SP ? fl.p : fh.p, KS, thr.p, porig.p, pjoin.p, \
-           items.p, nitems, per_xcd, lo, hi, qbits, (unsigned long long *)J.pairs.p,            \
+           items.p, smap, per_xcd, lo, hi, qbits, (unsigned long long *)J.pairs.p,              \
            (unsigned long long)J.pairs.cap, ctr.p, (unsigned long long *)band.p, bscore.p,      \
            (unsigned long long)band.cap, sample_every, (unsigned long long)budget)
             if (split) {
