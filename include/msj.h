@@ -76,14 +76,15 @@ typedef struct msj_params {
     uint32_t verbose;            /* 1: per-layer progress on stdout                 */
     uint32_t count_work;         /* 1: count FP64 dimensions evaluated (slower)     */
     uint32_t reserved0;
-    uint64_t page_pairs;         /* result page capacity in pairs (0: automatic)    */
+    uint64_t page_pairs;         /* MSJ_OUT_HOST: neighbour ids per page of the
+                                    pinned double buffer (0: 2^24)                 */
 } msj_params;
 
 typedef struct msj_stats {
     uint32_t num_nodes;          /* final node count                                */
     uint32_t num_layers;         /* layers built (numSplits)                        */
     uint32_t num_candidates;     /* candidate partitions per layer                  */
-    uint32_t num_pages;          /* result pages produced                           */
+    uint32_t num_pages;          /* pages of the neighbour list copied to host      */
     uint64_t num_calcs;          /* candidate distance evaluations (nodes.cu:697)   */
     uint64_t total_pairs;        /* "Total results Set Size" (launcher.cu:3513)     */
     uint32_t chosen[MSJ_MAX_LAYERS];          /* selected candidate per layer       */
@@ -98,7 +99,7 @@ typedef struct msj_stats {
     double   ms_join;            /* distance-and-compare kernels                    */
     double   ms_join_kernel;     /* the same, kernel launches only                  */
     double   ms_output;          /* pair sort, counts, offsets                      */
-    double   ms_d2h;             /* result copy to host (MSJ_OUT_HOST)              */
+    double   ms_d2h;             /* result pages to host (MSJ_OUT_HOST), host clock */
     double   ms_total;           /* host wall clock of msj_ctx_join                 */
     uint64_t join_launches;      /* distance kernel launches (incl. retries)        */
     uint64_t dims_evaluated;     /* count_work: sum over evaluated pairs of the

@@ -40,10 +40,44 @@ void hip_check(hipError_t e, const char *what, const char *file, int line)
 
 }  // namespace msj
 
+// Page-locked host buffer (hipHostMalloc); grow-only.
+struct PinnedBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf &) = delete;
+    PinnedBuf &operator=(const PinnedBuf &) = delete;
+    ~PinnedBuf() { release(); }
+    void ensure(size_t bytes)
+    {
+        if (bytes <= cap)
+            return;
+        release();
+        hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            throw msj::Error(MSJ_E_NOMEM, std::string("hipHostMalloc of ") +
+                                              std::to_string(bytes) + " bytes failed");
+        }
+        cap = bytes;
+    }
+    void release()
+    {
+        if (p)
+            (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
 struct msj_ctx {
     msj_params prm;
     hipStream_t stream = nullptr;
+    hipStream_t copy_stream = nullptr;  // result pages to host
     hipEvent_t ev[8] = {};
+    hipEvent_t page_ev[2] = {};
+    PinnedBuf pin[2];                   // double buffer of result pages
     uint32_t n = 0, d = 0;
     bool f32 = false;
     msj::DevBuf<unsigned char> staging;  // row-major upload
@@ -181,6 +215,37 @@ void do_load(msj_ctx *c, const void *data, uint32_t n, uint32_t d)
     c->staging.release();
 }
 
+// Device -> host copy in pages through two pinned buffers on the copy
+// stream: the DMA of page k+1 runs while the host copies page k out of the
+// other buffer (the reference's batch loop pages results the same way,
+// launcher.cu:3418-3506).  `ready` is recorded on the compute stream after
+// the source is complete.  Returns the number of pages.
+uint64_t page_to_host(msj_ctx *c, void *dst, const void *src, size_t bytes, size_t page,
+                      hipEvent_t ready)
+{
+    if (bytes == 0)
+        return 0;
+    c->pin[0].ensure(page);
+    c->pin[1].ensure(page);
+    MSJ_HIP(hipStreamWaitEvent(c->copy_stream, ready, 0));
+    const size_t np = (bytes + page - 1) / page;
+    auto issue = [&](size_t k) {
+        const size_t off = k * page, len = std::min(page, bytes - off);
+        MSJ_HIP(hipMemcpyAsync(c->pin[k & 1].p, (const char *)src + off, len,
+                               hipMemcpyDeviceToHost, c->copy_stream));
+        MSJ_HIP(hipEventRecord(c->page_ev[k & 1], c->copy_stream));
+    };
+    issue(0);
+    for (size_t k = 0; k < np; ++k) {
+        if (k + 1 < np)
+            issue(k + 1);  // its buffer was drained by the host at step k-1
+        MSJ_HIP(hipEventSynchronize(c->page_ev[k & 1]));
+        const size_t off = k * page, len = std::min(page, bytes - off);
+        std::memcpy((char *)dst + off, c->pin[k & 1].p, len);
+    }
+    return np;
+}
+
 float elapsed(hipEvent_t a, hipEvent_t b)
 {
     float ms = 0.f;
@@ -228,8 +293,8 @@ void do_join(msj_ctx *c, double eps, msj_result *out)
                         s);
 
     // ---- join (subsystem c)
-    uint64_t cap = c->prm.page_pairs;
-    if (cap == 0) {
+    uint64_t cap = 0;
+    {
         size_t freeb = 0, totalb = 0;
         MSJ_HIP(hipMemGetInfo(&freeb, &totalb));
         uint64_t want = std::max<uint64_t>(std::max<uint64_t>(c->last_R, 128ull * n), 1ull << 22);
@@ -269,7 +334,7 @@ void do_join(msj_ctx *c, double eps, msj_result *out)
     st.num_nodes = c->ix.M;
     st.num_layers = ls.layers;
     st.num_candidates = ls.ncand;
-    st.num_pages = 1;
+    st.num_pages = 0;
     st.num_calcs = c->ix.num_calcs;
     st.total_pairs = R;
     for (uint32_t l = 0; l < ls.layers; ++l) {
@@ -305,18 +370,17 @@ void do_join(msj_ctx *c, double eps, msj_result *out)
             msj_result_free(out);
             throw std::bad_alloc();
         }
-        MSJ_HIP(hipMemcpyAsync(out->counts, c->r_counts.p, sizeof(uint32_t) * n,
-                               hipMemcpyDeviceToHost, s));
-        MSJ_HIP(hipMemcpyAsync(out->offsets, c->r_offsets.p, sizeof(uint64_t) * ((size_t)n + 1),
-                               hipMemcpyDeviceToHost, s));
-        if (R)
-            MSJ_HIP(hipMemcpyAsync(out->nbrs, c->r_nbrs.p, sizeof(uint32_t) * R,
-                                   hipMemcpyDeviceToHost, s));
-        MSJ_HIP(hipMemcpyAsync(out->node_order, c->ix.perm.p, sizeof(uint32_t) * n,
-                               hipMemcpyDeviceToHost, s));
-        MSJ_HIP(hipEventRecord(c->ev[7], s));
-        MSJ_HIP(hipStreamSynchronize(s));
-        st.ms_d2h = elapsed(c->ev[6], c->ev[7]);
+        const auto t_copy = std::chrono::steady_clock::now();
+        const size_t page = (size_t)(c->prm.page_pairs ? c->prm.page_pairs : (1ull << 24)) * 4;
+        page_to_host(c, out->counts, c->r_counts.p, sizeof(uint32_t) * n, page, c->ev[6]);
+        page_to_host(c, out->offsets, c->r_offsets.p, sizeof(uint64_t) * ((size_t)n + 1), page,
+                     c->ev[6]);
+        st.num_pages = (uint32_t)page_to_host(c, out->nbrs, c->r_nbrs.p, sizeof(uint32_t) * R,
+                                              page, c->ev[6]);
+        page_to_host(c, out->node_order, c->ix.perm.p, sizeof(uint32_t) * n, page, c->ev[6]);
+        st.ms_d2h = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() -
+                                                              t_copy)
+                        .count();
     }
     st.ms_total =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start)
@@ -390,8 +454,11 @@ int msj_ctx_create(const msj_params *p, msj_ctx **out)
         c->prm = prm;
         try {
             MSJ_HIP(hipStreamCreate(&c->stream));
+            MSJ_HIP(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
             for (auto &e : c->ev)
                 MSJ_HIP(hipEventCreate(&e));
+            for (auto &e : c->page_ev)
+                MSJ_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         } catch (...) {
             msj_ctx_destroy(c);
             throw;
@@ -410,10 +477,19 @@ void msj_ctx_destroy(msj_ctx *c)
         (void)hipSetDevice(c->prm.device);
         if (c->stream)
             (void)hipStreamSynchronize(c->stream);
+        if (c->copy_stream)
+            (void)hipStreamSynchronize(c->copy_stream);
         for (auto &e : c->ev)
             if (e)
                 (void)hipEventDestroy(e);
-        delete c;  // releases device buffers on this device
+        for (auto &e : c->page_ev)
+            if (e)
+                (void)hipEventDestroy(e);
+        if (c->copy_stream)
+            (void)hipStreamDestroy(c->copy_stream);
+        if (c->stream)
+            (void)hipStreamDestroy(c->stream);
+        delete c;  // releases device and pinned buffers on this device
         if (prev >= 0)
             (void)hipSetDevice(prev);
         return;

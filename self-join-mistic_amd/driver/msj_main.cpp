@@ -170,7 +170,7 @@ int main(int argc, char *argv[])
     std::printf("Node Construct time: %f\n", constructT);
     std::fprintf(stderr, "%f ", constructT);
     std::fprintf(stderr, "%u %f ", st.ref_num_batches, st.ref_var_calc);
-    std::printf("Batchs: %u\n", st.num_pages);
+    std::printf("Batchs: %u\n", st.ref_num_batches);
     const double kernelT = (st.ms_join + st.ms_output + st.ms_d2h) / 1e3;
     std::printf("Total results Set Size: %llu \n", (unsigned long long)res.total_pairs);
     std::printf("Kernel Time: %f\n", kernelT);
