@@ -152,6 +152,53 @@ int  msj_ctx_load(msj_ctx *ctx, const void *data, uint32_t n, uint32_t d);
  * destruction of, this context; msj_result_free does not free them. */
 int  msj_ctx_join(msj_ctx *ctx, double eps, msj_result *out);
 
+/* ---- Batching form: many independent joins of uneven sizes and dimensions.
+ * No counterpart in the reference (its main runs one join per process); the
+ * per-job semantics are msj_self_join's.  All jobs' node nets are built
+ * concurrently on worker streams, then one persistent kernel, replayed from a
+ * captured hipGraph, drains a device work queue holding every job's work
+ * items; tile shapes (work per item, workgroups per CU) are chosen per row
+ * width by a built-in autotuner on the first run and kept by the batch. */
+typedef struct msj_job {
+    const void *data;            /* host, n x d row-major, dims sigma-ordered,
+                                    precision of params.storage                   */
+    uint32_t    n;
+    uint32_t    d;
+    double      eps;
+} msj_job;
+
+typedef struct msj_batch_stats {
+    uint32_t njobs;
+    uint32_t workers;            /* host threads / streams building node nets      */
+    uint32_t grid_blocks;        /* persistent launch: workgroups                  */
+    uint32_t classes;            /* distinct padded row widths                     */
+    uint32_t class_ds[8];        /* per class: padded row width                    */
+    uint32_t class_epi_log2[8];  /* per class: log2 of lane evaluations per item   */
+    uint64_t work_items;         /* items in the device queue                      */
+    uint64_t total_pairs;        /* sum over jobs                                   */
+    uint64_t num_calcs;          /* sum over jobs                                   */
+    double   ms_index;           /* host clock: all node nets, join orders, plans   */
+    double   ms_join;            /* device time of the persistent launch            */
+    double   ms_output;          /* host clock: all pair sorts and CSR arrays       */
+    double   ms_tune;            /* host clock: autotuning (first run of a class)   */
+    double   ms_total;           /* host clock of msj_batch_run                     */
+    uint32_t graph_replays;      /* launches through the captured graph             */
+    uint32_t pad0;
+} msj_batch_stats;
+
+typedef struct msj_batch msj_batch;
+
+int  msj_batch_create(const msj_params *p, msj_batch **b);
+void msj_batch_destroy(msj_batch *b);
+/* Uploads the jobs' data (copied; the caller's buffers may be freed after). */
+int  msj_batch_load(msj_batch *b, const msj_job *jobs, uint32_t njobs);
+/* Joins every loaded job; outs has njobs entries (free each with
+ * msj_result_free; with MSJ_OUT_DEVICE the arrays stay valid until the next
+ * run or msj_batch_destroy).  st may be NULL. */
+int  msj_batch_run(msj_batch *b, msj_result *outs, msj_batch_stats *st);
+/* One-shot: create, load, run, destroy. */
+int  msj_join_batch(const msj_job *jobs, uint32_t njobs, const msj_params *p, msj_result *outs);
+
 /* utils.cu:22-53 stddev(): dimension order by decreasing population standard
  * deviation (ties: higher index first), as main.cu:111 uses it. */
 int  msj_dim_order(const double *data, uint32_t n, uint32_t d, uint32_t *order, double *sigma);

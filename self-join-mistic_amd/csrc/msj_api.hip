@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <string>
 #include <thread>
@@ -69,6 +70,12 @@ struct PinnedBuf {
         p = nullptr;
         cap = 0;
     }
+};
+
+struct msj_batch {
+    msj_params prm;
+    msj::BatchEngine *e = nullptr;
+    uint32_t njobs = 0;
 };
 
 struct msj_ctx {
@@ -161,6 +168,28 @@ __global__ void k_soa(const unsigned char *__restrict__ rm, unsigned char *__res
     }
 }
 
+}  // namespace
+
+void msj::load_soa(const void *data, uint32_t n, uint32_t d, bool f32,
+                   msj::DevBuf<unsigned char> &xs, msj::DevBuf<unsigned char> &staging,
+                   hipStream_t s)
+{
+    const uint32_t esz = f32 ? 4 : 8;
+    const size_t bytes = (size_t)n * d * esz;
+    if (bytes == 0)
+        return;
+    staging.ensure(bytes);
+    xs.ensure(bytes);
+    MSJ_HIP(hipMemcpyAsync(staging.p, data, bytes, hipMemcpyHostToDevice, s));
+    msj::check_grid(msj::grid_stride((size_t)n * d, 256), 256);
+    hipLaunchKernelGGL(k_soa, dim3(msj::grid_stride((size_t)n * d, 256)), dim3(256), 0, s,
+                       staging.p, xs.p, n, d, esz);
+    MSJ_HIP(hipGetLastError());
+    MSJ_HIP(hipStreamSynchronize(s));
+}
+
+namespace {
+
 // launcher.cu:3203-3233: the reference's batch plan, kept for its stderr record.
 void reference_batch_plan(const std::vector<uint32_t> &nstart, const std::vector<uint64_t> &ncalcs,
                           uint32_t n, double eps, uint64_t sum_calcs, uint32_t *batches,
@@ -199,19 +228,11 @@ void do_load(msj_ctx *c, const void *data, uint32_t n, uint32_t d)
         throw Error(MSJ_E_OVERFLOW, "n*d too large");
     DeviceGuard g(c->prm.device);
     c->f32 = c->prm.storage == MSJ_STORAGE_F32;
-    const uint32_t esz = c->f32 ? 4 : 8;
-    const size_t bytes = (size_t)n * d * esz;
     c->n = n;
     c->d = d;
     if (n == 0)
         return;
-    c->staging.ensure(bytes);
-    c->xs.ensure(bytes);
-    MSJ_HIP(hipMemcpyAsync(c->staging.p, data, bytes, hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_soa, dim3(msj::grid_stride((size_t)n * d, 256)), dim3(256), 0, c->stream,
-                       c->staging.p, c->xs.p, n, d, esz);
-    MSJ_HIP(hipGetLastError());
-    MSJ_HIP(hipStreamSynchronize(c->stream));
+    msj::load_soa(data, n, d, c->f32, c->xs, c->staging, c->stream);
     c->staging.release();
 }
 
@@ -494,6 +515,92 @@ void msj_ctx_destroy(msj_ctx *c)
             (void)hipSetDevice(prev);
         return;
     }
+}
+
+int msj_batch_create(const msj_params *p, msj_batch **out)
+{
+    return guarded([&] {
+        if (!out)
+            throw Error(MSJ_E_INVAL, "out is NULL");
+        *out = nullptr;
+        msj_params prm;
+        if (p)
+            prm = *p;
+        else
+            msj_default_params(&prm);
+        validate_params(prm);
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+            (void)hipGetLastError();
+            throw Error(MSJ_E_NODEV, "no HIP device visible");
+        }
+        if (prm.device < 0 || prm.device >= ndev)
+            throw Error(MSJ_E_NODEV, "device ordinal out of range");
+        DeviceGuard g(prm.device);
+        std::unique_ptr<msj_batch> b(new msj_batch());
+        b->prm = prm;
+        b->e = msj::batch_create(prm);
+        *out = b.release();
+    });
+}
+
+void msj_batch_destroy(msj_batch *b)
+{
+    if (!b)
+        return;
+    int prev = -1;
+    (void)hipGetDevice(&prev);
+    (void)hipSetDevice(b->prm.device);
+    msj::batch_destroy(b->e);
+    delete b;
+    if (prev >= 0)
+        (void)hipSetDevice(prev);
+}
+
+int msj_batch_load(msj_batch *b, const msj_job *jobs, uint32_t njobs)
+{
+    return guarded([&] {
+        if (!b || (!jobs && njobs))
+            throw Error(MSJ_E_INVAL, "batch or jobs is NULL");
+        DeviceGuard g(b->prm.device);
+        b->njobs = 0;
+        msj::batch_load(b->e, jobs, njobs);
+        b->njobs = njobs;
+    });
+}
+
+int msj_batch_run(msj_batch *b, msj_result *outs, msj_batch_stats *st)
+{
+    return guarded([&] {
+        if (!b || (!outs && b->njobs))
+            throw Error(MSJ_E_INVAL, "batch or outs is NULL");
+        DeviceGuard g(b->prm.device);
+        try {
+            msj::batch_run(b->e, outs, st);
+        } catch (...) {
+            for (uint32_t j = 0; j < b->njobs; ++j)
+                msj_result_free(&outs[j]);
+            throw;
+        }
+    });
+}
+
+int msj_join_batch(const msj_job *jobs, uint32_t njobs, const msj_params *p, msj_result *outs)
+{
+    msj_params prm;
+    if (p)
+        prm = *p;
+    else
+        msj_default_params(&prm);
+    prm.output = MSJ_OUT_HOST;
+    msj_batch *b = nullptr;
+    int rc = msj_batch_create(&prm, &b);
+    if (rc == MSJ_OK)
+        rc = msj_batch_load(b, jobs, njobs);
+    if (rc == MSJ_OK)
+        rc = msj_batch_run(b, outs, nullptr);
+    msj_batch_destroy(b);
+    return rc;
 }
 
 int msj_ctx_load(msj_ctx *c, const void *data, uint32_t n, uint32_t d)

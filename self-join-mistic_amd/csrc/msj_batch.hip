@@ -1,0 +1,531 @@
+// msj_batch.hip -- batching form: many independent joins in one persistent
+// launch.
+//
+// A step (batch_run):
+//   1. node nets: every job's index build, join order, FP64 rows and work
+//      items, spread over worker threads, each with its own stream (the
+//      builders' per-layer host round trips of different jobs overlap);
+//   2. join: one persistent kernel (grid = workgroups per CU x CUs) whose
+//      waves pop work items from a device queue (one atomic ticket per
+//      CHUNK items) holding the items of all jobs in job order; an item is
+//      run with the plain form's per-item code for its job's row width
+//      (msj_join_items.h), and its result keys go to that job's pair buffer.
+//      The reset of the counters and the persistent kernel are captured once
+//      as a hipGraph and replayed (the job table they read is refreshed by a
+//      copy before each replay);
+//   3. output: every job's pair sort and CSR arrays, on the worker streams.
+// Per-job results are those of msj_self_join (the same per-item code, the
+// same node nets).
+//
+// Autotuner.  The tile shape of the plain form is the work per item (query
+// rows x candidate columns, set by the lane evaluations per item) and the
+// number of resident workgroups per CU.  On the first run of a batch, for
+// each padded row width present, the jobs of that width are planned with
+// 2^16, 2^18 and 2^20 evaluations per item and run alone through the
+// persistent kernel; the fastest is kept.  Then the whole batch is timed
+// with 1, 2 and 4 workgroups per CU.  The choices are kept for later runs.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <exception>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "msj_join_items.h"
+
+#pragma clang fp contract(off)
+
+namespace msj {
+
+namespace {
+
+constexpr uint32_t CHUNK = 1;  // items per queue ticket
+
+// Persistent kernel: waves pop items of any job from the queue.
+__global__ __launch_bounds__(256) void k_join_batch(const JobView *__restrict__ views,
+                                                    const uint64_t *__restrict__ job_off,
+                                                    uint32_t njobs,
+                                                    unsigned long long *__restrict__ queue)
+{
+    __shared__ double qsh[WPB][LDS_WAVE_BYTES / 8];
+    __shared__ unsigned long long hsh[WPB][HITS];
+    const uint32_t lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+    WaveHits H(hsh[wv], lane);
+    uint64_t dims = 0;
+    const uint64_t total = job_off[njobs];
+    // every wave leaves the loop once the queue is drained
+    while (true) {
+        unsigned long long t = 0;
+        if (lane == 0)
+            t = atomicAdd(queue, (unsigned long long)CHUNK);
+        t = __shfl(t, 0, WAVE);
+        if (t >= total)
+            break;
+        const uint64_t tend = min((uint64_t)t + CHUNK, total);
+        for (uint64_t i = t; i < tend; ++i) {
+            uint32_t lo = 0, hi = njobs;  // job with job_off[j] <= i < job_off[j+1]
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) / 2;
+                if (job_off[mid] <= i)
+                    lo = mid;
+                else
+                    hi = mid;
+            }
+            const JobView J = views[lo];
+            const JoinItem w = J.items[i - job_off[lo]];
+            item_any<false>(J, w, qsh[wv], H, dims);
+            if (H.fill)
+                H.flush(J);  // the keys belong to this job
+        }
+    }
+}
+
+__global__ void k_batch_reset(const JobView *__restrict__ views, uint32_t njobs,
+                              unsigned long long *__restrict__ queue)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < njobs)
+        *views[j].counter = 0ull;
+    if (j == 0)
+        *queue = 0ull;
+}
+
+__global__ void k_batch_counts(const JobView *__restrict__ views, uint32_t njobs,
+                               unsigned long long *__restrict__ dst)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < njobs)
+        dst[j] = *views[j].counter;
+}
+
+#define LAUNCH(kern, grid, block, ...)                                                   \
+    do {                                                                                 \
+        ::msj::check_grid((grid), (block));                                              \
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, s, __VA_ARGS__);            \
+        MSJ_HIP(hipGetLastError());                                                      \
+    } while (0)
+
+struct BatchJob {
+    uint32_t n = 0, d = 0;
+    double eps = 0.0;
+    DevBuf<unsigned char> xs;
+    DevBuf<double> xr;
+    Index ix;
+    JoinRunner join;
+    LayerStats ls;
+    DevBuf<uint32_t> counts;
+    DevBuf<uint64_t> offsets;
+    DevBuf<uint32_t> nbrs;
+    uint64_t R = 0, last_R = 0;
+    float ms_index = 0.f;
+};
+
+struct Worker {
+    hipStream_t s = nullptr;
+    IndexBuilder builder;
+    DevBuf<unsigned char> staging;
+};
+
+double now_ms()
+{
+    return std::chrono::duration<double, std::milli>(
+               std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+}  // namespace
+
+struct BatchEngine {
+    msj_params prm;
+    std::vector<std::unique_ptr<BatchJob>> jobs;
+    std::vector<std::unique_ptr<Worker>> workers;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[2] = {};
+    DevBuf<JobView> views;
+    DevBuf<uint64_t> job_off;
+    DevBuf<unsigned long long> queue, rcounts;
+    std::map<uint32_t, uint64_t> epi;  // padded row width -> evaluations per item
+    uint32_t wg_per_cu = 0;            // 0: not tuned yet
+    int cus = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    uint32_t graph_njobs = 0, graph_grid = 0;
+    const void *graph_views = nullptr;
+
+    // Runs f(job index, worker) for all jobs, job j on worker j % W.
+    template <class F>
+    void parallel(F &&f)
+    {
+        const uint32_t W = (uint32_t)workers.size();
+        std::vector<std::thread> th;
+        std::vector<std::exception_ptr> err(W);
+        for (uint32_t w = 0; w < W; ++w)
+            th.emplace_back([&, w] {
+                try {
+                    MSJ_HIP(hipSetDevice(prm.device));
+                    for (uint32_t j = w; j < jobs.size(); j += W)
+                        f(j, *workers[w]);
+                    MSJ_HIP(hipStreamSynchronize(workers[w]->s));
+                } catch (...) {
+                    err[w] = std::current_exception();
+                }
+            });
+        for (auto &t : th)
+            t.join();
+        for (auto &e : err)
+            if (e)
+                std::rethrow_exception(e);
+    }
+
+    void upload_table(const std::vector<uint32_t> &sel, hipStream_t s)
+    {
+        std::vector<JobView> hv;
+        std::vector<uint64_t> off(1, 0);
+        for (uint32_t j : sel) {
+            BatchJob &B = *jobs[j];
+            hv.push_back(B.join.view(B.ix, B.xr.p, B.n, B.d, B.eps));
+            off.push_back(off.back() + B.join.nitems);
+        }
+        views.ensure(std::max<size_t>(hv.size(), 1));
+        job_off.ensure(off.size());
+        if (!hv.empty())
+            MSJ_HIP(hipMemcpyAsync(views.p, hv.data(), hv.size() * sizeof(JobView),
+                                   hipMemcpyHostToDevice, s));
+        MSJ_HIP(hipMemcpyAsync(job_off.p, off.data(), off.size() * sizeof(uint64_t),
+                               hipMemcpyHostToDevice, s));
+        MSJ_HIP(hipStreamSynchronize(s));  // the host vectors go out of scope
+    }
+
+    uint32_t grid_for_wg(uint32_t wg) const { return (uint32_t)std::max(1, cus) * wg; }
+
+    // Direct launch over the uploaded table of `nj` jobs; returns device ms.
+    float launch_direct(uint32_t nj, uint32_t grid)
+    {
+        hipStream_t s = stream;
+        MSJ_HIP(hipEventRecord(ev[0], s));
+        LAUNCH(k_batch_reset, grid_for(std::max<uint32_t>(nj, 1), 64), 64, views.p, nj, queue.p);
+        LAUNCH(k_join_batch, grid, WAVE * WPB, views.p, job_off.p, nj, queue.p);
+        MSJ_HIP(hipEventRecord(ev[1], s));
+        MSJ_HIP(hipEventSynchronize(ev[1]));
+        float ms = 0.f;
+        MSJ_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+        return ms;
+    }
+
+    // The same through the captured graph (captured again when the job
+    // count, the grid or the table's address changes).
+    float launch_graph(uint32_t nj, uint32_t grid)
+    {
+        hipStream_t s = stream;
+        if (!gexec || graph_njobs != nj || graph_grid != grid || graph_views != views.p) {
+            if (gexec)
+                (void)hipGraphExecDestroy(gexec);
+            if (graph)
+                (void)hipGraphDestroy(graph);
+            gexec = nullptr;
+            graph = nullptr;
+            MSJ_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            hipLaunchKernelGGL(k_batch_reset, dim3(grid_for(std::max<uint32_t>(nj, 1), 64)),
+                               dim3(64), 0, s, views.p, nj, queue.p);
+            hipLaunchKernelGGL(k_join_batch, dim3(grid), dim3(WAVE * WPB), 0, s, views.p,
+                               job_off.p, nj, queue.p);
+            MSJ_HIP(hipStreamEndCapture(s, &graph));
+            MSJ_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+            graph_njobs = nj;
+            graph_grid = grid;
+            graph_views = views.p;
+        }
+        MSJ_HIP(hipEventRecord(ev[0], s));
+        MSJ_HIP(hipGraphLaunch(gexec, s));
+        MSJ_HIP(hipEventRecord(ev[1], s));
+        MSJ_HIP(hipEventSynchronize(ev[1]));
+        float ms = 0.f;
+        MSJ_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+        return ms;
+    }
+
+    void plan_job(uint32_t j, Worker &W, uint64_t e)
+    {
+        BatchJob &B = *jobs[j];
+        B.join.plan(B.ix, B.n, B.d, W.s, true, e);
+    }
+
+    // Autotuning (first run): evaluations per item per row width, then
+    // workgroups per CU for the whole batch.
+    void tune()
+    {
+        std::map<uint32_t, std::vector<uint32_t>> cls;
+        for (uint32_t j = 0; j < jobs.size(); ++j)
+            cls[padded_dim(jobs[j]->d)].push_back(j);
+        const uint64_t cand[3] = {1ull << 16, 1ull << 18, 1ull << 20};
+        for (auto &kv : cls) {
+            if (epi.count(kv.first))
+                continue;
+            float best = 0.f;
+            uint64_t best_e = cand[1];
+            for (uint64_t e : cand) {
+                for (uint32_t j : kv.second)
+                    plan_job(j, *workers[0], e);
+                MSJ_HIP(hipStreamSynchronize(workers[0]->s));
+                upload_table(kv.second, stream);
+                uint32_t nj = (uint32_t)kv.second.size();
+                launch_direct(nj, grid_for_wg(2));  // warm
+                const float ms = launch_direct(nj, grid_for_wg(2));
+                if (best == 0.f || ms < best) {
+                    best = ms;
+                    best_e = e;
+                }
+            }
+            epi[kv.first] = best_e;
+        }
+        if (wg_per_cu == 0) {
+            parallel([&](uint32_t j, Worker &W) { plan_job(j, W, epi[padded_dim(jobs[j]->d)]); });
+            std::vector<uint32_t> all(jobs.size());
+            for (uint32_t j = 0; j < all.size(); ++j)
+                all[j] = j;
+            upload_table(all, stream);
+            float best = 0.f;
+            for (uint32_t wg : {1u, 2u, 4u}) {
+                const float ms = launch_direct((uint32_t)all.size(), grid_for_wg(wg));
+                if (best == 0.f || ms < best) {
+                    best = ms;
+                    wg_per_cu = wg;
+                }
+            }
+        }
+    }
+
+    ~BatchEngine()
+    {
+        (void)hipSetDevice(prm.device);
+        if (stream)
+            (void)hipStreamSynchronize(stream);
+        if (gexec)
+            (void)hipGraphExecDestroy(gexec);
+        if (graph)
+            (void)hipGraphDestroy(graph);
+        for (auto &w : workers)
+            if (w->s) {
+                (void)hipStreamSynchronize(w->s);
+                (void)hipStreamDestroy(w->s);
+            }
+        for (auto &e : ev)
+            if (e)
+                (void)hipEventDestroy(e);
+        if (stream)
+            (void)hipStreamDestroy(stream);
+    }
+};
+
+BatchEngine *batch_create(const msj_params &p)
+{
+    std::unique_ptr<BatchEngine> b(new BatchEngine());
+    b->prm = p;
+    MSJ_HIP(hipStreamCreate(&b->stream));
+    for (auto &e : b->ev)
+        MSJ_HIP(hipEventCreate(&e));
+    int cus = 0;
+    MSJ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p.device));
+    b->cus = cus;
+    const uint32_t W = 8;
+    for (uint32_t w = 0; w < W; ++w) {
+        b->workers.emplace_back(new Worker());
+        MSJ_HIP(hipStreamCreate(&b->workers.back()->s));
+    }
+    b->queue.ensure(1);
+    return b.release();
+}
+
+void batch_destroy(BatchEngine *b) { delete b; }
+
+void batch_load(BatchEngine *b, const msj_job *jobs, uint32_t njobs)
+{
+    const bool f32 = b->prm.storage == MSJ_STORAGE_F32;
+    for (uint32_t j = 0; j < njobs; ++j) {
+        const msj_job &J = jobs[j];
+        if (J.d == 0 || J.n == 0)
+            throw Error(MSJ_E_INVAL, "every job needs n >= 1 and d >= 1");
+        if (!J.data)
+            throw Error(MSJ_E_INVAL, "job data is NULL");
+        if (!(J.eps > 0.0) || !std::isfinite(J.eps))
+            throw Error(MSJ_E_INVAL, "job eps must be positive and finite");
+        if ((uint64_t)J.n * J.d >= (1ull << 32))
+            throw Error(MSJ_E_OVERFLOW, "n*d too large");
+    }
+    b->jobs.clear();
+    b->epi.clear();
+    b->wg_per_cu = 0;
+    for (uint32_t j = 0; j < njobs; ++j) {
+        b->jobs.emplace_back(new BatchJob());
+        b->jobs.back()->n = jobs[j].n;
+        b->jobs.back()->d = jobs[j].d;
+        b->jobs.back()->eps = jobs[j].eps;
+    }
+    b->parallel([&](uint32_t j, Worker &W) {
+        BatchJob &B = *b->jobs[j];
+        load_soa(jobs[j].data, B.n, B.d, f32, B.xs, W.staging, W.s);
+    });
+    for (auto &w : b->workers)
+        w->staging.release();
+    b->rcounts.ensure(std::max<uint32_t>(njobs, 1));
+}
+
+void batch_run(BatchEngine *b, msj_result *outs, msj_batch_stats *st)
+{
+    const double t0 = now_ms();
+    msj_batch_stats S;
+    std::memset(&S, 0, sizeof(S));
+    const uint32_t nj = (uint32_t)b->jobs.size();
+    const bool f32 = b->prm.storage == MSJ_STORAGE_F32;
+    const bool dev_out = b->prm.output == MSJ_OUT_DEVICE;
+    S.njobs = nj;
+    S.workers = (uint32_t)b->workers.size();
+    for (uint32_t j = 0; j < nj; ++j)
+        std::memset(&outs[j], 0, sizeof(msj_result));
+
+    // 1. node nets, join orders, rows, items
+    b->parallel([&](uint32_t j, Worker &W) {
+        BatchJob &B = *b->jobs[j];
+        const double a = now_ms();
+        B.ls = LayerStats();
+        W.builder.build(B.xs.p, f32, B.n, B.d, B.eps, b->prm, B.ix, B.ls, W.s);
+        B.join.order(B.ix, B.xs.p, f32, B.n, B.d, W.s);
+        B.xr.ensure((size_t)B.n * padded_dim(B.d));
+        gather_node_rows(B.xs.p, B.xr.p, f32, B.join.jperm.p, B.n, B.d, W.s);
+        const auto it = b->epi.find(padded_dim(B.d));
+        B.join.plan(B.ix, B.n, B.d, W.s, true, it == b->epi.end() ? 0 : it->second);
+        B.join.counter.ensure(1);
+        const uint64_t cap = std::max<uint64_t>(B.last_R + B.last_R / 8, 64ull * B.n + 4096);
+        if (B.join.pairs.cap < cap)
+            B.join.pairs.ensure(cap);
+        MSJ_HIP(hipStreamSynchronize(W.s));
+        B.ms_index = (float)(now_ms() - a);
+    });
+    const double t1 = now_ms();
+    S.ms_index = t1 - t0;
+
+    // autotuning on the first run of these jobs
+    if (b->wg_per_cu == 0) {
+        b->tune();
+        b->parallel([&](uint32_t j, Worker &W) {
+            BatchJob &B = *b->jobs[j];
+            B.join.plan(B.ix, B.n, B.d, W.s, true, b->epi[padded_dim(B.d)]);
+        });
+        S.ms_tune = now_ms() - t1;
+    }
+
+    // 2. the persistent join over all jobs, from the captured graph
+    std::vector<uint32_t> all(nj);
+    for (uint32_t j = 0; j < nj; ++j)
+        all[j] = j;
+    b->upload_table(all, b->stream);
+    const uint32_t grid = b->grid_for_wg(b->wg_per_cu);
+    S.ms_join = b->launch_graph(nj, grid);
+    S.graph_replays = 1;
+    S.grid_blocks = grid;
+    std::vector<unsigned long long> R(nj);
+    {
+        hipStream_t s = b->stream;
+        LAUNCH(k_batch_counts, grid_for(std::max<uint32_t>(nj, 1), 64), 64, b->views.p, nj,
+               b->rcounts.p);
+        if (nj)
+            MSJ_HIP(hipMemcpyAsync(R.data(), b->rcounts.p, nj * sizeof(unsigned long long),
+                                   hipMemcpyDeviceToHost, s));
+        MSJ_HIP(hipStreamSynchronize(s));
+    }
+
+    // 3. outputs (a job whose pair buffer overflowed is rerun alone with
+    // an exact buffer)
+    const double t2 = now_ms();
+    b->parallel([&](uint32_t j, Worker &W) {
+        BatchJob &B = *b->jobs[j];
+        B.R = R[j];
+        if (B.R > B.join.pairs.cap) {
+            B.join.pairs.ensure(B.R);
+            hipEvent_t e0, e1;
+            MSJ_HIP(hipEventCreate(&e0));
+            MSJ_HIP(hipEventCreate(&e1));
+            uint64_t launches = 0;
+            float kms = 0.f;
+            B.R = B.join.run_plain(B.ix, B.xr.p, B.n, B.d, B.eps, B.R, W.s, e0, e1, &launches,
+                                   &kms, nullptr);
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+        }
+        B.last_R = B.R;
+        B.counts.ensure(B.n);
+        B.offsets.ensure((size_t)B.n + 1);
+        B.nbrs.ensure(std::max<uint64_t>(B.R, 1));
+        B.join.finish(B.n, B.R, B.counts.p, B.offsets.p, B.nbrs.p, W.s);
+        msj_result &o = outs[j];
+        o.n = B.n;
+        o.total_pairs = B.R;
+        o.stats.num_nodes = B.ix.M;
+        o.stats.num_layers = B.ls.layers;
+        o.stats.num_candidates = B.ls.ncand;
+        o.stats.num_calcs = B.ix.num_calcs;
+        o.stats.total_pairs = B.R;
+        for (uint32_t l = 0; l < B.ls.layers; ++l) {
+            o.stats.chosen[l] = B.ls.chosen[l];
+            o.stats.nodes_per_layer[l] = B.ls.nodes[l];
+            o.stats.calcs_per_layer[l] = B.ls.calcs[l];
+            o.stats.sse[l] = B.ls.sse[l];
+        }
+        o.stats.ms_index = B.ms_index;
+        o.stats.ms_join = S.ms_join;
+        o.stats.ms_join_kernel = S.ms_join;
+        o.stats.join_launches = 1;
+        o.stats.form_used = MSJ_FORM_PLAIN;
+        if (dev_out) {
+            o.on_device = 1;
+            o.counts = B.counts.p;
+            o.offsets = B.offsets.p;
+            o.nbrs = B.nbrs.p;
+            o.node_order = B.ix.perm.p;
+        } else {
+            o.counts = (uint32_t *)std::malloc(sizeof(uint32_t) * B.n);
+            o.offsets = (uint64_t *)std::malloc(sizeof(uint64_t) * ((size_t)B.n + 1));
+            o.nbrs = (uint32_t *)std::malloc(sizeof(uint32_t) * std::max<uint64_t>(B.R, 1));
+            o.node_order = (uint32_t *)std::malloc(sizeof(uint32_t) * B.n);
+            if (!o.counts || !o.offsets || !o.nbrs || !o.node_order)
+                throw std::bad_alloc();
+            MSJ_HIP(hipMemcpyAsync(o.counts, B.counts.p, sizeof(uint32_t) * B.n,
+                                   hipMemcpyDeviceToHost, W.s));
+            MSJ_HIP(hipMemcpyAsync(o.offsets, B.offsets.p, sizeof(uint64_t) * ((size_t)B.n + 1),
+                                   hipMemcpyDeviceToHost, W.s));
+            if (B.R)
+                MSJ_HIP(hipMemcpyAsync(o.nbrs, B.nbrs.p, sizeof(uint32_t) * B.R,
+                                       hipMemcpyDeviceToHost, W.s));
+            MSJ_HIP(hipMemcpyAsync(o.node_order, B.ix.perm.p, sizeof(uint32_t) * B.n,
+                                   hipMemcpyDeviceToHost, W.s));
+            MSJ_HIP(hipStreamSynchronize(W.s));
+        }
+    });
+    S.ms_output = now_ms() - t2;
+    for (uint32_t j = 0; j < nj; ++j) {
+        S.total_pairs += b->jobs[j]->R;
+        S.num_calcs += b->jobs[j]->ix.num_calcs;
+        S.work_items += b->jobs[j]->join.nitems;
+    }
+    uint32_t k = 0;
+    for (auto &kv : b->epi) {
+        if (k == 8)
+            break;
+        S.class_ds[k] = kv.first;
+        S.class_epi_log2[k] = (uint32_t)bits_for(kv.second) - 1;
+        ++k;
+    }
+    S.classes = k;
+    S.ms_total = now_ms() - t0;
+    for (uint32_t j = 0; j < nj; ++j)
+        outs[j].stats.ms_total = S.ms_total;
+    if (st)
+        *st = S;
+}
+
+}  // namespace msj

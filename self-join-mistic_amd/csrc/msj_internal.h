@@ -176,6 +176,19 @@ struct JoinItem {
     uint32_t x, qbeg, qend, cbeg, cend;
 };
 
+// Everything a wave needs to run the work items of one join (plain form).
+struct JobView {
+    const double *xr;            // FP64 rows in join order, ds doubles each
+    uint32_t d, ds;
+    double eps2;
+    const uint32_t *nstart, *uoff, *uadj, *perm;  // perm: join position -> row id
+    const JoinItem *items;
+    uint32_t nitems;
+    int qbits;
+    unsigned long long *out, *counter;
+    unsigned long long cap;
+};
+
 struct JoinRunner {
     DevBuf<uint32_t> itcnt, itoff, uoff, uadj;
     DevBuf<uint64_t> ulen, tot64;
@@ -193,7 +206,10 @@ struct JoinRunner {
     void order(const Index &ix, const void *xs, bool f32, uint32_t n, uint32_t d, hipStream_t s);
     // Upper neighbour lists and work items of the final node net.
     // plain_items = false: upper lists only (the MFMA form has its own items).
-    void plan(const Index &ix, uint32_t n, uint32_t d, hipStream_t s, bool plain_items = true);
+    void plan(const Index &ix, uint32_t n, uint32_t d, hipStream_t s, bool plain_items = true,
+              uint64_t evals_per_item = 0);
+    // Descriptor of this join for the kernels (after plan()).
+    JobView view(const Index &ix, const double *xr, uint32_t n, uint32_t d, double eps) const;
     // Runs the plain FP64 distance kernel on the node-ordered row-major FP64
     // copy xr; returns the number of pairs R.  Grows `pairs` and reruns when
     // R exceeds the capacity.  With dims_evaluated, a counting build of the
@@ -266,6 +282,17 @@ struct MfmaRunner {
 
 // The filter's bounds hold for 2^-50 <= eps^2 <= 2^60, n < 2^30, d <= 4096.
 bool mfma_filter_applies(double eps, uint32_t n, uint32_t d);
+
+// Row-major host data (n x d, 4- or 8-byte values) -> SoA device copy xs.
+void load_soa(const void *data, uint32_t n, uint32_t d, bool f32, DevBuf<unsigned char> &xs,
+              DevBuf<unsigned char> &staging, hipStream_t s);
+
+// Batching form (msj_batch.hip); the C entry points are in msj_api.hip.
+struct BatchEngine;
+BatchEngine *batch_create(const msj_params &p);
+void batch_destroy(BatchEngine *b);
+void batch_load(BatchEngine *b, const msj_job *jobs, uint32_t njobs);
+void batch_run(BatchEngine *b, msj_result *outs, msj_batch_stats *st);
 
 // Row width of the node-ordered FP64 copy: d rounded up to a multiple of 4.
 inline uint32_t padded_dim(uint32_t d) { return (d + 3u) & ~3u; }

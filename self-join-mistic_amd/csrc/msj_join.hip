@@ -31,7 +31,7 @@
 // CSR offsets.
 #include <algorithm>
 
-#include "msj_internal.h"
+#include "msj_join_items.h"
 
 #pragma clang fp contract(off)
 
@@ -39,10 +39,8 @@ namespace msj {
 
 namespace {
 
-constexpr uint64_t EVALS_PER_ITEM = 262144;  // lane evaluations per work item
-constexpr uint32_t WPB = 4;                  // waves per block
-constexpr uint32_t LDS_WAVE_BYTES = 8192;    // staged query rows per wave
-constexpr uint32_t HITS = 256;               // per-wave LDS buffer of result keys
+constexpr uint64_t EVALS_PER_ITEM = 262144;  // lane evaluations per work item (default)
+
 __device__ inline double wave_min(double v)
 {
 #pragma unroll
@@ -62,377 +60,56 @@ __device__ inline double wave_max(double v)
     return v;
 }
 
-// Queries per work item for a row width DS (in-register dimensions): as many
-// FP64 rows as fit LDS_WAVE_BYTES, at most 64.
-__host__ __device__ constexpr uint32_t qmax_for(uint32_t DS)
+__device__ inline void add_dims(uint64_t dims, unsigned long long *work)
 {
-    return (LDS_WAVE_BYTES / (8u * DS)) < 64u ? (LDS_WAVE_BYTES / (8u * DS)) : 64u;
-}
-
-// Wave-wide vote on a lane predicate, kept in SGPRs (no VALU round trip).
-__device__ inline uint64_t wave_vote(bool v) { return __builtin_amdgcn_ballot_w64(v); }
-
-// Rows of xr are ds = roundup(d, 4) doubles, zero-padded: a padded
-// coordinate contributes s + (0 - 0)^2 = s exactly, so the chunk loop needs
-// no per-dimension guard.  DS is ds when ds <= 32 (candidate rows live in
-// VGPRs); otherwise DS = 32 and the remaining chunks come from memory.
-//
-// A work item's query rows are contiguous in xr (node order); the wave
-// copies them into its LDS slice once, then every query of the inner loop
-// is a wave-uniform LDS address (a broadcast read) instead of a scalar load
-// that would miss the small scalar cache.
-//
-// Result keys go to a per-wave LDS buffer; the wave reserves global space
-// with one atomic per flush (at most HITS keys) and copies the buffer out
-// with coalesced stores.  A global atomic per accepting step would
-// serialise on the single counter.
-template <int DS, bool TAIL, bool COUNT>
-__global__ __launch_bounds__(256) void k_join_sym(
-    const double *__restrict__ xr, uint32_t d, uint32_t ds, double eps2,
-    const uint32_t *__restrict__ nstart, const uint32_t *__restrict__ uoff,
-    const uint32_t *__restrict__ uadj, const uint32_t *__restrict__ perm,
-    const JoinItem *__restrict__ items, uint32_t nitems, int qbits,
-    unsigned long long *__restrict__ out, unsigned long long *__restrict__ counter,
-    unsigned long long cap, unsigned long long *__restrict__ work)
-{
-    constexpr uint32_t QM = qmax_for(DS);
-    __shared__ double qsh[WPB][QM * DS];
-    __shared__ unsigned long long hsh[WPB][HITS];
-    const uint32_t lane = threadIdx.x & (WAVE - 1);
-    const uint32_t wv = threadIdx.x / WAVE;
-    const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (WAVE - lane));
-    unsigned long long *hb = hsh[wv];
-    uint32_t fill = 0;  // wave-uniform
-    uint64_t dims = 0;
-    // copies the buffered keys out: one atomic for the whole buffer
-    auto flush = [&]() {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        unsigned long long base = 0;
-        if (lane == 0)
-            base = atomicAdd(counter, (unsigned long long)fill);
-        base = __shfl(base, 0, WAVE);
-        for (uint32_t e = lane; e < fill; e += WAVE)
-            if (base + e < cap)
-                out[base + e] = hb[e];
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        fill = 0;
-    };
-    // grid-stride over groups of WPB items (a launch stays under 2^32 lanes)
-    for (uint64_t it0 = (uint64_t)blockIdx.x * WPB; it0 < nitems;
-         it0 += (uint64_t)gridDim.x * WPB) {
-        // wave-uniform by construction; readfirstlane lets the compiler keep the
-        // item and the loop counters in SGPRs
-        const bool have = it0 + wv < nitems;
-        const uint32_t it = __builtin_amdgcn_readfirstlane(have ? (uint32_t)(it0 + wv) : 0u);
-        JoinItem w = {0, 0, 0, 0, 0};
-        if (have)
-            w = items[it];
-        const uint32_t X = w.x;
-        const uint32_t x0 = have ? nstart[X] : 0u;
-        const uint32_t stride = TAIL ? ds : (uint32_t)DS;
-        const uint32_t qn = w.qend - w.qbeg;
-        // stage the item's query rows (first DS dimensions) in LDS
-        double *qs = qsh[wv];
-        for (uint32_t e = lane; e < qn * DS; e += WAVE) {
-            const uint32_t r = e / DS, k = e - r * DS;
-            qs[e] = xr[(size_t)(x0 + w.qbeg + r) * stride + k];
-        }
-        __syncthreads();
-        if (have) {
-            const uint64_t qall = qn >= WAVE ? ~0ull : ((1ull << qn) - 1ull);
-
-            // per-lane cursor into X's upper neighbour list
-            uint32_t a = uoff[X];
-            uint32_t Y = uadj[a];
-            uint32_t ybase = 0;
-            uint32_t ystart = nstart[Y];
-            uint32_t ysz = nstart[Y + 1] - ystart;
-
-            for (uint32_t cs = w.cbeg; cs < w.cend; cs += WAVE) {
-                const uint32_t ci = cs + lane;
-                const bool cvalid = ci < w.cend;
-                uint32_t c = 0;
-                if (cvalid) {
-                    while (ci >= ybase + ysz) {
-                        ybase += ysz;
-                        ++a;
-                        Y = uadj[a];
-                        ystart = nstart[Y];
-                        ysz = nstart[Y + 1] - ystart;
-                    }
-                    c = ystart + (ci - ybase);
-                }
-                const bool self_node = cvalid && Y == X;
-                const double *crow = xr + (size_t)c * stride;  // c = 0 for idle lanes
-                double cc[DS];
-        #pragma unroll
-                for (int k = 0; k < DS; ++k)
-                    cc[k] = crow[k];
-                const unsigned long long corig = perm[c];
-
-                uint64_t qmask = qall;
-                while (qmask) {
-                    const uint32_t qi = (uint32_t)__builtin_ctzll(qmask);
-                    qmask &= qmask - 1ull;
-                    const uint32_t p = x0 + w.qbeg + qi;
-                    bool alive = cvalid && (!self_node || p <= c);
-                    if (!wave_vote(alive))
-                        continue;
-                    const double *qrow = qs + qi * DS;  // wave-uniform LDS address
-                    double s = 0.0;
-        #pragma unroll
-                    for (int l = 0; l < DS; l += 4) {
-                        if (l > 0 && !wave_vote(alive))
-                            break;
-                        if (COUNT && alive && (uint32_t)l < d)
-                            dims += min(4u, d - (uint32_t)l);
-                        const double2 qa = *(const double2 *)(qrow + l);
-                        const double2 qb = *(const double2 *)(qrow + l + 2);
-                        double t0 = qa.x - cc[l];
-                        s = fma(t0, t0, s);
-                        double t1 = qa.y - cc[l + 1];
-                        double q1 = t1 * t1;
-                        s = s + q1;
-                        double t2 = qb.x - cc[l + 2];
-                        double q2 = t2 * t2;
-                        s = s + q2;
-                        double t3 = qb.y - cc[l + 3];
-                        double q3 = t3 * t3;
-                        s = s + q3;
-                        alive = alive && !(s > eps2);
-                    }
-                    if (TAIL) {
-                        const double *grow = xr + (size_t)p * stride;
-                        for (uint32_t l = DS; l < ds && wave_vote(alive); l += 4) {
-                            if (COUNT && alive && l < d)
-                                dims += min(4u, d - l);
-                            double t0 = grow[l] - crow[l];
-                            s = fma(t0, t0, s);
-                            double t1 = grow[l + 1] - crow[l + 1];
-                            double q1 = t1 * t1;
-                            s = s + q1;
-                            double t2 = grow[l + 2] - crow[l + 2];
-                            double q2 = t2 * t2;
-                            s = s + q2;
-                            double t3 = grow[l + 3] - crow[l + 3];
-                            double q3 = t3 * t3;
-                            s = s + q3;
-                            alive = alive && !(s > eps2);
-                        }
-                    }
-                    const uint64_t m1 = wave_vote(alive);
-                    if (m1) {
-                        const bool two = alive && c != p;
-                        const uint64_t m2 = wave_vote(two);
-                        const uint32_t tot = (uint32_t)(__popcll(m1) + __popcll(m2));
-                        if (fill + tot > HITS)
-                            flush();
-                        if (alive) {
-                            const unsigned long long porig = perm[p];
-                            const uint32_t idx =
-                                fill + (uint32_t)(__popcll(m1 & lt) + __popcll(m2 & lt));
-                            hb[idx] = (porig << qbits) | corig;
-                            if (two)
-                                hb[idx + 1] = (corig << qbits) | porig;
-                        }
-                        fill += tot;
-                    }
-                }
-            }
-        }
-        __syncthreads();  // the next group's staging reuses qsh
-    }
-    if (fill)
-        flush();
-    if (COUNT) {
-        unsigned long long v = dims;
+    unsigned long long v = dims;
 #pragma unroll
-        for (int o = WAVE / 2; o > 0; o >>= 1)
-            v += __shfl_down(v, o, WAVE);
-        if (lane == 0)
-            atomicAdd(work, v);
-    }
+    for (int o = WAVE / 2; o > 0; o >>= 1)
+        v += __shfl_down(v, o, WAVE);
+    if ((threadIdx.x & (WAVE - 1)) == 0 && v)
+        atomicAdd(work, v);
 }
 
-// High-dimensional rows (ds > 32).  Slab-major order: a wave holds the
-// 64 candidates' next HSL dimensions in VGPRs (each loaded once) and
-// advances the partial sums of all of the item's queries (up to 32, one
-// FP64 sum per lane and query in VGPRs) through that slab, chunk by chunk in
-// the reference's order, with the same early-exit test; a query row slab is
-// staged in LDS and read as a broadcast.  Decisions are those of the
-// per-pair sequence: the chunks of every pair run in order, each followed by
-// the s > eps^2 test.
-constexpr int HSL = 16;  // dimensions per slab
-constexpr int HQ = 32;   // queries per item
+// One wave per work item, grid-stride over the items (a launch stays under
+// 2^32 work-items); DS = ds <= 32.
+template <int DS, bool COUNT>
+__global__ __launch_bounds__(256) void k_join_sym(JobView J, unsigned long long *__restrict__ work)
+{
+    __shared__ double qsh[WPB][LDS_WAVE_BYTES / 8];
+    __shared__ unsigned long long hsh[WPB][HITS];
+    const uint32_t lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+    WaveHits H(hsh[wv], lane);
+    uint64_t dims = 0;
+    const uint64_t stride = (uint64_t)gridDim.x * WPB;
+    for (uint64_t it = (uint64_t)blockIdx.x * WPB + wv; it < J.nitems; it += stride) {
+        // readfirstlane keeps the item and the loop counters in SGPRs
+        const JoinItem w = J.items[__builtin_amdgcn_readfirstlane((uint32_t)it)];
+        item_sym<DS, COUNT>(J, w, qsh[wv], H, dims);
+    }
+    if (H.fill)
+        H.flush(J);
+    if (COUNT)
+        add_dims(dims, work);
+}
 
+// The same for ds > 32 (slab-major variant).
 template <bool COUNT>
-__global__ __launch_bounds__(256) void k_join_hd(
-    const double *__restrict__ xr, uint32_t d, uint32_t ds, double eps2,
-    const uint32_t *__restrict__ nstart, const uint32_t *__restrict__ uoff,
-    const uint32_t *__restrict__ uadj, const uint32_t *__restrict__ perm,
-    const JoinItem *__restrict__ items, uint32_t nitems, int qbits,
-    unsigned long long *__restrict__ out, unsigned long long *__restrict__ counter,
-    unsigned long long cap, unsigned long long *__restrict__ work)
+__global__ __launch_bounds__(256) void k_join_hd(JobView J, unsigned long long *__restrict__ work)
 {
     __shared__ double qsh[WPB][HQ * HSL];
     __shared__ unsigned long long hsh[WPB][HITS];
-    const uint32_t lane = threadIdx.x & (WAVE - 1);
-    const uint32_t wv = threadIdx.x / WAVE;
-    const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (WAVE - lane));
-    unsigned long long *hb = hsh[wv];
-    double *qs = qsh[wv];
-    uint32_t fill = 0;  // wave-uniform
+    const uint32_t lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+    WaveHits H(hsh[wv], lane);
     uint64_t dims = 0;
-    auto wave_sync = [&]() {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    };
-    auto flush = [&]() {
-        wave_sync();
-        unsigned long long base = 0;
-        if (lane == 0)
-            base = atomicAdd(counter, (unsigned long long)fill);
-        base = __shfl(base, 0, WAVE);
-        for (uint32_t e = lane; e < fill; e += WAVE)
-            if (base + e < cap)
-                out[base + e] = hb[e];
-        wave_sync();
-        fill = 0;
-    };
-    // one wave per item, grid-stride (no block-wide synchronisation here)
-    for (uint64_t it0 = (uint64_t)blockIdx.x * WPB + wv; it0 < nitems;
-         it0 += (uint64_t)gridDim.x * WPB) {
-        const uint32_t it = __builtin_amdgcn_readfirstlane((uint32_t)it0);
-        const JoinItem w = items[it];
-        const uint32_t X = w.x;
-        const uint32_t x0 = nstart[X];
-        const uint32_t qn = w.qend - w.qbeg;  // <= HQ
-        const uint32_t q0 = x0 + w.qbeg;      // first query (node order)
-        uint32_t a = uoff[X];
-        uint32_t Y = uadj[a];
-        uint32_t ybase = 0;
-        uint32_t ystart = nstart[Y];
-        uint32_t ysz = nstart[Y + 1] - ystart;
-        for (uint32_t cs = w.cbeg; cs < w.cend; cs += WAVE) {
-            const uint32_t ci = cs + lane;
-            const bool cvalid = ci < w.cend;
-            uint32_t c = 0;
-            if (cvalid) {
-                while (ci >= ybase + ysz) {
-                    ybase += ysz;
-                    ++a;
-                    Y = uadj[a];
-                    ystart = nstart[Y];
-                    ysz = nstart[Y + 1] - ystart;
-                }
-                c = ystart + (ci - ybase);
-            }
-            const bool self_node = cvalid && Y == X;
-            uint32_t alive = 0;  // bit q: pair (query q, this lane's candidate) undecided
-            for (uint32_t q = 0; q < qn; ++q)
-                if (cvalid && (!self_node || q0 + q <= c))
-                    alive |= 1u << q;
-            double sq[HQ];
-#pragma unroll
-            for (int q = 0; q < HQ; ++q)
-                sq[q] = 0.0;
-            const double *crow = xr + (size_t)c * ds;
-            for (uint32_t l0 = 0; l0 < ds; l0 += HSL) {
-                if (!wave_vote(alive != 0))
-                    break;
-                // query slab -> LDS (qn rows of HSL dims), candidate slab -> VGPRs
-                wave_sync();
-                for (uint32_t e = lane; e < HQ * HSL; e += WAVE) {
-                    const uint32_t q = e / HSL, j = e % HSL;
-                    qs[e] = (q < qn && l0 + j < ds) ? xr[(size_t)(q0 + q) * ds + l0 + j] : 0.0;
-                }
-                double cc[HSL];
-#pragma unroll
-                for (int j = 0; j < HSL; j += 2) {
-                    if (l0 + j < ds) {
-                        const double2 v = *(const double2 *)(crow + l0 + j);
-                        cc[j] = v.x;
-                        cc[j + 1] = v.y;
-                    } else {
-                        cc[j] = 0.0;
-                        cc[j + 1] = 0.0;
-                    }
-                }
-                wave_sync();
-                // fully unrolled over q (sq[] stays in VGPRs): no break or
-                // continue; a decided query or chunk is skipped by its vote
-#pragma unroll
-                for (int q = 0; q < HQ; ++q) {
-                    bool al = (alive >> q) & 1u;  // 0 for q >= qn
-                    if (!wave_vote(al))
-                        continue;
-                    const double *qrow = qs + q * HSL;  // wave-uniform LDS address
-                    double s = sq[q];
-#pragma unroll
-                    for (int k = 0; k < HSL; k += 4) {
-                        if (l0 + k >= ds || !wave_vote(al))
-                            continue;
-                        if (COUNT && al && l0 + k < d)
-                            dims += min(4u, d - (l0 + k));
-                        const double2 qa = *(const double2 *)(qrow + k);
-                        const double2 qb = *(const double2 *)(qrow + k + 2);
-                        double t0 = qa.x - cc[k];
-                        s = fma(t0, t0, s);
-                        double t1 = qa.y - cc[k + 1];
-                        double p1 = t1 * t1;
-                        s = s + p1;
-                        double t2 = qb.x - cc[k + 2];
-                        double p2 = t2 * t2;
-                        s = s + p2;
-                        double t3 = qb.y - cc[k + 3];
-                        double p3 = t3 * t3;
-                        s = s + p3;
-                        al = al && !(s > eps2);
-                    }
-                    sq[q] = s;
-                    if (!al)
-                        alive &= ~(1u << q);
-                }
-            }
-            // emission: the pairs still undecided after every slab are results
-            const unsigned long long corig = perm[c];
-            for (uint32_t q = 0; q < qn; ++q) {
-                const bool acc = (alive >> q) & 1u;
-                const uint64_t m1 = wave_vote(acc);
-                if (!m1)
-                    continue;
-                const uint32_t p = q0 + q;
-                const bool two = acc && c != p;
-                const uint64_t m2 = wave_vote(two);
-                const uint32_t tot = (uint32_t)(__popcll(m1) + __popcll(m2));
-                if (fill + tot > HITS)
-                    flush();
-                if (acc) {
-                    const unsigned long long porig = perm[p];
-                    const uint32_t idx = fill + (uint32_t)(__popcll(m1 & lt) + __popcll(m2 & lt));
-                    hb[idx] = (porig << qbits) | corig;
-                    if (two)
-                        hb[idx + 1] = (corig << qbits) | porig;
-                }
-                fill += tot;
-            }
-        }
+    const uint64_t stride = (uint64_t)gridDim.x * WPB;
+    for (uint64_t it = (uint64_t)blockIdx.x * WPB + wv; it < J.nitems; it += stride) {
+        const JoinItem w = J.items[__builtin_amdgcn_readfirstlane((uint32_t)it)];
+        item_hd<COUNT>(J, w, qsh[wv], H, dims);
     }
-    if (fill)
-        flush();
-    if (COUNT) {
-        unsigned long long v = dims;
-#pragma unroll
-        for (int o = WAVE / 2; o > 0; o >>= 1)
-            v += __shfl_down(v, o, WAVE);
-        if (lane == 0)
-            atomicAdd(work, v);
-    }
+    if (H.fill)
+        H.flush(J);
+    if (COUNT)
+        add_dims(dims, work);
 }
 
 // Upper neighbour lists: entries Y >= X of X's adjacency, and their sizes.
@@ -473,13 +150,13 @@ __global__ __launch_bounds__(256) void k_upper_write(const uint32_t *__restrict_
 }
 
 // Items per node: query ranges of up to qmax points x candidate ranges sized
-// so one item is about EVALS_PER_ITEM lane evaluations (multiples of 64).
-__device__ inline void item_shape(uint32_t sz, uint64_t L, uint32_t qmax, uint32_t *qper,
-                                  uint32_t *nq, uint32_t *cper, uint32_t *nc)
+// so one item is about `epi` lane evaluations (multiples of 64).
+__device__ inline void item_shape(uint32_t sz, uint64_t L, uint32_t qmax, uint64_t epi,
+                                  uint32_t *qper, uint32_t *nq, uint32_t *cper, uint32_t *nc)
 {
     *qper = min(qmax, sz);
     *nq = (sz + *qper - 1) / *qper;
-    uint64_t cp = EVALS_PER_ITEM / *qper;
+    uint64_t cp = epi / *qper;
     cp = (cp + WAVE - 1) / WAVE * WAVE;
     *cper = (uint32_t)cp;
     uint64_t k = (L + cp - 1) / cp;
@@ -488,21 +165,22 @@ __device__ inline void item_shape(uint32_t sz, uint64_t L, uint32_t qmax, uint32
 
 __global__ __launch_bounds__(256) void k_item_count(const uint32_t *__restrict__ nstart,
                                                     const uint64_t *__restrict__ ulen, uint32_t M,
-                                                    uint32_t qmax, uint32_t *__restrict__ cnt)
+                                                    uint32_t qmax, uint64_t epi,
+                                                    uint32_t *__restrict__ cnt)
 {
     uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= M)
         return;
     uint32_t qper, nq, cper, nc;
-    item_shape(nstart[x + 1] - nstart[x], ulen[x], qmax, &qper, &nq, &cper, &nc);
+    item_shape(nstart[x + 1] - nstart[x], ulen[x], qmax, epi, &qper, &nq, &cper, &nc);
     cnt[x] = nq * nc;
 }
 
 __global__ __launch_bounds__(256) void k_item_fill(const uint32_t *__restrict__ nstart,
                                                    const uint64_t *__restrict__ ulen,
                                                    const uint32_t *__restrict__ itoff, uint32_t M,
-                                                   uint32_t qmax, uint32_t nitems,
-                                                   JoinItem *__restrict__ items)
+                                                   uint32_t qmax, uint64_t epi,
+                                                   uint32_t nitems, JoinItem *__restrict__ items)
 {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nitems)
@@ -519,7 +197,7 @@ __global__ __launch_bounds__(256) void k_item_fill(const uint32_t *__restrict__ 
     const uint32_t sz = nstart[x + 1] - nstart[x];
     const uint64_t L = ulen[x];
     uint32_t qper, nq, cper, nc;
-    item_shape(sz, L, qmax, &qper, &nq, &cper, &nc);
+    item_shape(sz, L, qmax, epi, &qper, &nq, &cper, &nc);
     const uint32_t local = k - itoff[x];
     const uint32_t qi = local / nc, ck = local % nc;
     JoinItem w;
@@ -665,50 +343,31 @@ __global__ void k_zero(T *p, size_t n)
         MSJ_HIP(hipGetLastError());                                                      \
     } while (0)
 
-template <int DS, bool TAIL>
-void launch_sym(const JoinRunner &J, const Index &ix, const double *xr, uint32_t d, uint32_t ds,
-                double eps2, int qbits, unsigned long long *out, unsigned long long cap,
-                unsigned long long *work, hipStream_t s)
+template <int DS>
+void launch_sym(const JobView &V, unsigned long long *work, hipStream_t s)
 {
-    const unsigned wpb = WPB;
     if (work)
-        LAUNCH((k_join_sym<DS, TAIL, true>), grid_stride(J.nitems, wpb), WAVE * wpb, xr, d, ds,
-               eps2, ix.nstart.p, J.uoff.p, J.uadj.p, J.jperm.p, J.items.p, J.nitems, qbits, out,
-               J.counter.p, cap, work);
+        LAUNCH((k_join_sym<DS, true>), grid_stride(V.nitems, WPB), WAVE * WPB, V, work);
     else
-        LAUNCH((k_join_sym<DS, TAIL, false>), grid_stride(J.nitems, wpb), WAVE * wpb, xr, d, ds,
-               eps2, ix.nstart.p, J.uoff.p, J.uadj.p, J.jperm.p, J.items.p, J.nitems, qbits, out,
-               J.counter.p, cap, work);
+        LAUNCH((k_join_sym<DS, false>), grid_stride(V.nitems, WPB), WAVE * WPB, V, work);
 }
 
-void dispatch_sym(const JoinRunner &J, const Index &ix, const double *xr, uint32_t d, double eps2,
-                  int qbits, unsigned long long *out, unsigned long long cap,
-                  unsigned long long *work, hipStream_t s)
+void dispatch_plain(const JobView &V, unsigned long long *work, hipStream_t s)
 {
-    const uint32_t ds = padded_dim(d);
-    switch (ds) {
-#define MSJ_DS(v)                                                                   \
-    case v:                                                                         \
-        launch_sym<v, false>(J, ix, xr, d, ds, eps2, qbits, out, cap, work, s);       \
-        return;
-        MSJ_DS(4)
-        MSJ_DS(8)
-        MSJ_DS(12)
-        MSJ_DS(16)
-        MSJ_DS(20)
-        MSJ_DS(24)
-        MSJ_DS(28)
-        MSJ_DS(32)
-#undef MSJ_DS
+    switch (V.ds) {
+    case 4: launch_sym<4>(V, work, s); return;
+    case 8: launch_sym<8>(V, work, s); return;
+    case 12: launch_sym<12>(V, work, s); return;
+    case 16: launch_sym<16>(V, work, s); return;
+    case 20: launch_sym<20>(V, work, s); return;
+    case 24: launch_sym<24>(V, work, s); return;
+    case 28: launch_sym<28>(V, work, s); return;
+    case 32: launch_sym<32>(V, work, s); return;
     default:
         if (work)
-            LAUNCH(k_join_hd<true>, grid_stride(J.nitems, WPB), WAVE * WPB, xr, d, ds, eps2,
-                   ix.nstart.p, J.uoff.p, J.uadj.p, J.jperm.p, J.items.p, J.nitems, qbits, out,
-                   J.counter.p, cap, work);
+            LAUNCH(k_join_hd<true>, grid_stride(V.nitems, WPB), WAVE * WPB, V, work);
         else
-            LAUNCH(k_join_hd<false>, grid_stride(J.nitems, WPB), WAVE * WPB, xr, d, ds, eps2,
-                   ix.nstart.p, J.uoff.p, J.uadj.p, J.jperm.p, J.items.p, J.nitems, qbits, out,
-                   J.counter.p, cap, work);
+            LAUNCH(k_join_hd<false>, grid_stride(V.nitems, WPB), WAVE * WPB, V, work);
     }
 }
 
@@ -746,11 +405,34 @@ void JoinRunner::order(const Index &ix, const void *xs, bool f32, uint32_t n, ui
     sorter.sort(jkeys.p, jperm.p, n, bits_for(ix.M ? ix.M - 1 : 0) + 30, s);
 }
 
-void JoinRunner::plan(const Index &ix, uint32_t n, uint32_t d, hipStream_t s, bool plain_items)
+JobView JoinRunner::view(const Index &ix, const double *xr, uint32_t n, uint32_t d,
+                         double eps) const
+{
+    JobView V;
+    V.xr = xr;
+    V.d = d;
+    V.ds = padded_dim(d);
+    V.eps2 = eps * eps;  // launcher.cu:3200
+    V.nstart = ix.nstart.p;
+    V.uoff = uoff.p;
+    V.uadj = uadj.p;
+    V.perm = jperm.p;
+    V.items = items.p;
+    V.nitems = nitems;
+    V.qbits = std::max(1, bits_for(n ? n - 1 : 0));
+    V.out = (unsigned long long *)pairs.p;
+    V.counter = counter.p;
+    V.cap = pairs.cap;
+    return V;
+}
+
+void JoinRunner::plan(const Index &ix, uint32_t n, uint32_t d, hipStream_t s, bool plain_items,
+                      uint64_t evals_per_item)
 {
     const uint32_t M = ix.M;
     const uint32_t ds = padded_dim(d);
-    const uint32_t qmax = ds <= 32 ? qmax_for(ds) : (uint32_t)HQ;
+    const uint32_t qmax = item_queries(ds);
+    const uint64_t epi = evals_per_item ? evals_per_item : EVALS_PER_ITEM;
     itcnt.ensure(M);
     itoff.ensure((size_t)M + 1);
     ulen.ensure(M);
@@ -769,7 +451,7 @@ void JoinRunner::plan(const Index &ix, uint32_t n, uint32_t d, hipStream_t s, bo
         return;
     }
 
-    LAUNCH(k_item_count, grid_for(M, 256), 256, ix.nstart.p, ulen.p, M, qmax, itcnt.p);
+    LAUNCH(k_item_count, grid_for(M, 256), 256, ix.nstart.p, ulen.p, M, qmax, epi, itcnt.p);
     scan.excl_u32_to_u64(itcnt.p, tot64.p, M, s);
     uint64_t total = 0;
     MSJ_HIP(hipMemcpyAsync(&total, tot64.p + M, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
@@ -779,8 +461,8 @@ void JoinRunner::plan(const Index &ix, uint32_t n, uint32_t d, hipStream_t s, bo
     nitems = (uint32_t)total;
     scan.excl_u32_to_u32(itcnt.p, itoff.p, M, s);
     items.ensure(std::max<uint32_t>(nitems, 1));
-    LAUNCH(k_item_fill, grid_for(nitems, 256), 256, ix.nstart.p, ulen.p, itoff.p, M, qmax, nitems,
-           items.p);
+    LAUNCH(k_item_fill, grid_for(nitems, 256), 256, ix.nstart.p, ulen.p, itoff.p, M, qmax, epi,
+           nitems, items.p);
 }
 
 uint64_t JoinRunner::run_plain(const Index &ix, const double *xr, uint32_t n, uint32_t d,
@@ -788,8 +470,6 @@ uint64_t JoinRunner::run_plain(const Index &ix, const double *xr, uint32_t n, ui
                                hipEvent_t k1, uint64_t *launches, float *kernel_ms,
                                uint64_t *dims_evaluated)
 {
-    const double eps2 = eps * eps;  // launcher.cu:3200
-    const int qbits = std::max(1, bits_for(n ? n - 1 : 0));
     counter.ensure(1);
     if (pairs.cap < cap_hint)
         pairs.ensure(cap_hint);
@@ -805,8 +485,7 @@ uint64_t JoinRunner::run_plain(const Index &ix, const double *xr, uint32_t n, ui
             LAUNCH(k_zero<unsigned long long>, 1, 64, work, (size_t)1);
         MSJ_HIP(hipEventRecord(k0, s));
         if (nitems) {
-            dispatch_sym(*this, ix, xr, d, eps2, qbits, (unsigned long long *)pairs.p, pairs.cap,
-                         work, s);
+            dispatch_plain(view(ix, xr, n, d, eps), work, s);
             ++*launches;
         }
         MSJ_HIP(hipEventRecord(k1, s));

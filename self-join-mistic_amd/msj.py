@@ -23,7 +23,8 @@ OUT_HOST, OUT_DEVICE = 0, 1
 EXPORTS = (
     "msj_version", "msj_strerror", "msj_last_error", "msj_default_params",
     "msj_self_join", "msj_result_free", "msj_ctx_create", "msj_ctx_destroy",
-    "msj_ctx_load", "msj_ctx_join", "msj_dim_order",
+    "msj_ctx_load", "msj_ctx_join", "msj_dim_order", "msj_batch_create", "msj_batch_destroy",
+    "msj_batch_load", "msj_batch_run", "msj_join_batch",
 )
 
 
@@ -81,6 +82,36 @@ class Stats(C.Structure):
     ]
 
 
+class Job(C.Structure):
+    _fields_ = [
+        ("data", C.c_void_p),
+        ("n", C.c_uint32),
+        ("d", C.c_uint32),
+        ("eps", C.c_double),
+    ]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [
+        ("njobs", C.c_uint32),
+        ("workers", C.c_uint32),
+        ("grid_blocks", C.c_uint32),
+        ("classes", C.c_uint32),
+        ("class_ds", C.c_uint32 * 8),
+        ("class_epi_log2", C.c_uint32 * 8),
+        ("work_items", C.c_uint64),
+        ("total_pairs", C.c_uint64),
+        ("num_calcs", C.c_uint64),
+        ("ms_index", C.c_double),
+        ("ms_join", C.c_double),
+        ("ms_output", C.c_double),
+        ("ms_tune", C.c_double),
+        ("ms_total", C.c_double),
+        ("graph_replays", C.c_uint32),
+        ("pad0", C.c_uint32),
+    ]
+
+
 class Result(C.Structure):
     _fields_ = [
         ("n", C.c_uint32),
@@ -126,6 +157,12 @@ def lib():
         L.msj_ctx_join.argtypes = [C.c_void_p, C.c_double, C.POINTER(Result)]
         L.msj_dim_order.argtypes = [C.POINTER(C.c_double), C.c_uint32, C.c_uint32,
                                     C.POINTER(C.c_uint32), C.POINTER(C.c_double)]
+        L.msj_batch_create.argtypes = [C.POINTER(Params), C.POINTER(C.c_void_p)]
+        L.msj_batch_destroy.argtypes = [C.c_void_p]
+        L.msj_batch_load.argtypes = [C.c_void_p, C.POINTER(Job), C.c_uint32]
+        L.msj_batch_run.argtypes = [C.c_void_p, C.POINTER(Result), C.POINTER(BatchStats)]
+        L.msj_join_batch.argtypes = [C.POINTER(Job), C.c_uint32, C.POINTER(Params),
+                                     C.POINTER(Result)]
         _lib = L
     return _lib
 
@@ -260,6 +297,78 @@ class Context:
     def close(self):
         if self.h:
             lib().msj_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass
+
+
+def _jobs(datas, eps, storage):
+    arrs = [_as_input(x, storage) for x in datas]
+    jobs = (Job * max(1, len(arrs)))()
+    for k, (x, e) in enumerate(zip(arrs, eps)):
+        jobs[k].data = x.ctypes.data
+        jobs[k].n, jobs[k].d = x.shape
+        jobs[k].eps = float(e)
+    return arrs, jobs
+
+
+def join_batch(datas, eps, **kw):
+    """Batching form, one shot: a list of (n_k, d_k) arrays and their eps."""
+    p = default_params(**kw)
+    arrs, jobs = _jobs(datas, eps, p.storage)
+    outs = (Result * max(1, len(arrs)))()
+    rc = lib().msj_join_batch(jobs, len(arrs), C.byref(p), outs)
+    try:
+        _check(rc)
+        return [JoinResult(outs[k]) for k in range(len(arrs))]
+    finally:
+        for k in range(len(arrs)):
+            lib().msj_result_free(C.byref(outs[k]))
+
+
+class Batch:
+    """Batching form with the jobs resident in HBM across runs (msj_batch_*)."""
+
+    def __init__(self, **kw):
+        self.params = default_params(**kw)
+        h = C.c_void_p()
+        _check(lib().msj_batch_create(C.byref(self.params), C.byref(h)))
+        self.h = h
+        self.njobs = 0
+
+    def load(self, datas, eps):
+        arrs, jobs = _jobs(datas, eps, self.params.storage)
+        _check(lib().msj_batch_load(self.h, jobs, len(arrs)))
+        self.njobs = len(arrs)
+
+    def run(self, copy_arrays=True):
+        outs = (Result * max(1, self.njobs))()
+        st = BatchStats()
+        rc = lib().msj_batch_run(self.h, outs, C.byref(st))
+        try:
+            _check(rc)
+            res = [JoinResult(outs[k], copy_arrays=copy_arrays) for k in range(self.njobs)]
+        finally:
+            for k in range(self.njobs):
+                lib().msj_result_free(C.byref(outs[k]))
+        stats = {f[0]: getattr(st, f[0]) for f in BatchStats._fields_ if f[0] != "pad0"}
+        stats["class_ds"] = list(st.class_ds)[:st.classes]
+        stats["class_epi_log2"] = list(st.class_epi_log2)[:st.classes]
+        return res, stats
+
+    def close(self):
+        if self.h:
+            lib().msj_batch_destroy(self.h)
             self.h = None
 
     def __enter__(self):

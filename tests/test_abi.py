@@ -38,6 +38,8 @@ def test_struct_layout_matches_c(tmp_path, msj_mod):
         "msj_params": [f[0] for f in msj_mod.Params._fields_],
         "msj_stats": [f[0] for f in msj_mod.Stats._fields_],
         "msj_result": [f[0] for f in msj_mod.Result._fields_],
+        "msj_job": [f[0] for f in msj_mod.Job._fields_],
+        "msj_batch_stats": [f[0] for f in msj_mod.BatchStats._fields_],
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "msj.h"', "int main(void){"]
     for st, fl in fields.items():
@@ -51,7 +53,8 @@ def test_struct_layout_matches_c(tmp_path, msj_mod):
                     str(exe)], check=True)
     got = dict(line.split() for line in subprocess.run(
         [str(exe)], capture_output=True, text=True, check=True).stdout.splitlines())
-    py = {"msj_params": msj_mod.Params, "msj_stats": msj_mod.Stats, "msj_result": msj_mod.Result}
+    py = {"msj_params": msj_mod.Params, "msj_stats": msj_mod.Stats, "msj_result": msj_mod.Result,
+          "msj_job": msj_mod.Job, "msj_batch_stats": msj_mod.BatchStats}
     for st, cls in py.items():
         assert int(got[st]) == C.sizeof(cls), st
         for f in fields[st]:
