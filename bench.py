@@ -45,6 +45,26 @@ FP64_PEAK_TFLOPS = 78.6
 BF16_PEAK_TFLOPS = 2500.0  # dense MFMA
 FORMS = {"plain": 0, "bf16": 1, "split": 2}
 
+# c5: eps of the 64 instances of datagen.c5_specs(), from `--config c5 --calibrate`
+C5_EPS_HEX = [
+    "3fec838005945f0a", "3fcea8991317beb3", "3fb08267cd523431", "3fe918b28fcf9517",
+    "3fdbbe4b2f08cde6", "3fa959684580808c", "3fe2adde4ab2fbc4", "3fe5eb7510578640",
+    "3fda5e8a0d1c4b2f", "3fd9798454f8d1db", "3fc76122ed76f7a9", "3fe14f23fafd37a8",
+    "3fd78fed1bb43427", "3fdd0cd02d0eee3b", "3fd80bdebfa016c5", "3fd86caf0789b6a7",
+    "3fe3f994ccd6451a", "3fe4814ef3a652fa", "3fd5a429bd4e2c62", "3fe277ad5c37c067",
+    "3fc3aa0df825adea", "3fe055f1819d25cb", "3fc4030ec26e59a7", "3fe1e9c9129c71d1",
+    "3fd36a689aab73c6", "3fb86f229e3c2890", "3fca304e49b95a3b", "3fece1126e3e6031",
+    "3fe5a6a5c9bb8edb", "3fc5a5ba8d33ec4a", "3fd95c243a061b78", "3fe7810f73a817b3",
+    "3fb4d86d0fe8e46e", "3fe0dff264f47d3e", "3fe2c0ffe65f993c", "3fb87d168c9fe1c4",
+    "3fb20d42b0738b13", "3fba4af6377eddec", "3fe54aa46ac5e1b3", "3fe60869db787f86",
+    "3fe0e027ddbe6dbf", "3fdc7f9f1d9556df", "3fe2a1d939173411", "3fbf6c53ba02da1d",
+    "3fda800529596860", "3fe57889421cddff", "3fb0fd46708eafc8", "3fd1b0946b0da8ce",
+    "3fd21a5fe00f761f", "3fdc3df694df6d9a", "3fbbd2032fdda971", "3feabbff3edc3ef5",
+    "3fe8dc7934c34397", "3fe4fdca8f82c06c", "3fec87ac0f716ec8", "3fe593f2d0478bd3",
+    "3fe97faaa1c9ca9d", "3fe3341d2a99bd31", "3fe38a6fd6b86bc9", "3fc113ffdf8c9987",
+    "3fdcd49b67dc3c3d", "3fb5814c5caf8ead", "3fe288f4ffa36e1e", "3fd23697d99e4faf",
+]
+
 CONFIGS = {
     "c1": dict(n=65536, d=2, kind="uniform", seed=1001, eps=0.01,
                workload="plain FP64 join, 65,536 x 2 uniform [0,1), eps 0.01"),
@@ -76,6 +96,27 @@ eps = datagen.calibrate_eps(x, int(sys.argv[3]), sample=int(sys.argv[5]), seed=i
                             log=lambda m: print("[bench] " + m, file=sys.stderr, flush=True))
 print(json.dumps({"eps": eps}))
 """
+
+
+_CALIBRATE_C5 = r"""
+import sys, json
+sys.path.insert(0, sys.argv[1])
+import datagen
+out = []
+for k, (n, d, seed, cal_seed, target) in enumerate(datagen.c5_specs()):
+    x = datagen.mixture(n, d, seed)
+    out.append(datagen.calibrate_eps(x, target, sample=min(2000, n), seed=cal_seed,
+                                     device="cuda:0"))
+    print("[bench] c5 instance %d/64 calibrated" % (k + 1), file=sys.stderr, flush=True)
+print(json.dumps({"eps": out}))
+"""
+
+
+def calibrate_c5():
+    out = subprocess.run([sys.executable, "-c", _CALIBRATE_C5,
+                          os.path.join(ROOT, "self-join-mistic_amd")],
+                         stdout=subprocess.PIPE, check=True, text=True).stdout
+    return json.loads(out.strip().splitlines()[-1])["eps"]
 
 
 def calibrate(x, cfg):
@@ -157,7 +198,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c5"])
     ap.add_argument("--form", default="plain", choices=sorted(FORMS))
     ap.add_argument("--eps", type=float, default=None, help="override the calibrated eps")
     ap.add_argument("--calibrate", action="store_true",
@@ -171,6 +212,8 @@ def main():
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
+    if args.config == "c5":
+        return run_c5(args)
     cfg = CONFIGS[args.config]
     log("[bench] generating %s: n=%d d=%d" % (args.config, cfg["n"], cfg["d"]))
     x = make_data(cfg)
@@ -279,6 +322,125 @@ def main():
         line["cpu_baseline"] = cb
         line["mismatch"] = mism
         log("[bench] CPU %.3g evals/s; sampled-row mismatch %d" % (cb["value"], mism))
+    print(json.dumps(line), flush=True)
+
+
+def run_c5(args):
+    if args.calibrate or not C5_EPS_HEX:
+        log("[bench] calibrating the 64 instances of c5 on the GPU")
+        eps = calibrate_c5()
+        hexes = [struct.pack(">d", e).hex() for e in eps]
+        print(json.dumps({"config": "c5", "eps_hex": hexes}), flush=True)
+        return
+    specs = datagen.c5_specs()
+    eps = [struct.unpack(">d", bytes.fromhex(h))[0] for h in C5_EPS_HEX]
+    log("[bench] generating c5: %d instances, %d points" % (len(specs), sum(p[0] for p in specs)))
+    datas = []
+    for (n, d, seed, _, _) in specs:
+        x = datagen.mixture(n, d, seed)
+        order, _ = msj.dim_order(x)
+        datas.append(datagen.reorder_dims(x, order))
+    # counting pass (untimed): FP64 work of the plain kernel, per job
+    dims_eval, calcs = 0, 0
+    for x, e in zip(datas, eps):
+        r = msj.self_join(x, e, count_work=1)
+        dims_eval += r.stats["dims_evaluated"]
+        calcs += r.stats["num_calcs"]
+    log("[bench] counting pass: candidates=%d dims=%d" % (calcs, dims_eval))
+
+    with msj.Batch(output=msj.OUT_DEVICE) as b:
+        b.load(datas, eps)
+        for w in range(max(1, args.warmup)):
+            _, st = b.run(copy_arrays=False)
+            log("[bench] warmup %d: %.1f ms (tune %.1f ms, classes %s, epi log2 %s, grid %d)" % (
+                w, st["ms_total"], st["ms_tune"], st["class_ds"], st["class_epi_log2"],
+                st["grid_blocks"]))
+        stage = []
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            _, st = b.run(copy_arrays=False)
+            stage.append((st["ms_index"], st["ms_join"], st["ms_output"]))
+        wall = time.perf_counter() - t0
+    ms_step = wall * 1e3 / args.steps
+    ms_index, ms_join, ms_output = (float(v) for v in np.mean(stage, axis=0))
+    log("[bench] batch: %.2f ms/step (index %.2f, persistent join %.2f, output %.2f ms)" % (
+        ms_step, ms_index, ms_join, ms_output))
+
+    # the same 64 joins as sequential plain launches, data resident
+    ctxs = []
+    try:
+        for x in datas:
+            c = msj.Context(output=msj.OUT_DEVICE)
+            c.load(x)
+            ctxs.append(c)
+        for c, e in zip(ctxs, eps):  # warm
+            c.join(e, copy_arrays=False)
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            for c, e in zip(ctxs, eps):
+                c.join(e, copy_arrays=False)
+        seq_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    finally:
+        for c in ctxs:
+            c.close()
+    log("[bench] 64 sequential plain joins: %.2f ms/step" % seq_ms)
+
+    # parity: batch (host output) against the sequential single joins
+    log("[bench] parity: batch with host output vs 64 single joins")
+    res = msj.join_batch(datas, eps)
+    mism = 0
+    for k, (x, e, r) in enumerate(zip(datas, eps, res)):
+        s1 = msj.self_join(x, e)
+        if not (r.total_pairs == s1.total_pairs and np.array_equal(r.counts, s1.counts)
+                and np.array_equal(r.nbrs, s1.nbrs)):
+            mism += 1
+        if k % 16 == 15:
+            log("[bench] parity: %d/64 jobs compared, %d mismatching" % (k + 1, mism))
+    R = sum(r.total_pairs for r in res)
+    flops = 3.0 * dims_eval
+    achieved = flops / (ms_join / 1e3) / 1e12
+    line = {
+        "metric": METRIC, "value": calcs / (ms_step / 1e3), "unit": "distance evals/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded, generated in-process)",
+        "config": {"workload": "batching form: 64 independent plain FP64 joins, sizes "
+                               "log-uniform in [1e3, 1e6], d in {8, 18, 57, 90}, mixture data, "
+                               "mean 16-64 neighbours", "jobs": len(datas),
+                   "total_points": int(sum(p[0] for p in specs)), "form": "batch",
+                   "parallelism": "none (1 GPU)"},
+        "join_time_s": ms_step / 1e3, "pairs_per_s": R / (ms_step / 1e3), "total_pairs": R,
+        "candidate_evals": calcs, "ms_index": ms_index, "ms_join": ms_join,
+        "ms_output": ms_output, "sequential_ms_per_step": seq_ms,
+        "speedup_vs_sequential": seq_ms / ms_step, "mismatch_jobs": mism,
+        "tuned": {"class_ds": st["class_ds"], "epi_log2": st["class_epi_log2"],
+                  "grid_blocks": st["grid_blocks"], "work_items": st["work_items"]},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "k_join_batch (persistent, FP64 vector ALU; MI355X FP64 peak)",
+                     "kernel_ms": ms_join, "algorithmic_flops": flops},
+    }
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle  # noqa: WPS433 - the CPU baseline
+        # about 2e8 candidate evaluations/s on the box's cores at d = 57..90
+        budget = max(1.0, 2e8 * args.cpu_seconds)
+        every = max(1, int(calcs // budget))
+        done, tj = 0.0, 0.0
+        log("[bench] CPU baseline (oracle, %d threads): every %d-th query" % (
+            args.cpu_threads, every))
+        for k, (x, e) in enumerate(zip(datas, eps)):
+            r = oracle.self_join(x, e, sample_every=every, nthreads=args.cpu_threads)
+            done += r.evals_done
+            tj += r.t_join
+            if k % 8 == 7:
+                log("[bench] CPU baseline: %d/64 instances, %.1f s of join" % (k + 1, tj))
+        line["cpu_baseline"] = {
+            "value": done / tj if tj > 0 else 0.0, "unit": "distance evals/s",
+            "cores": args.cpu_threads, "kind": "port",
+            "sample": "oracle (C/OpenMP restatement) on all 64 instances: node nets + join of "
+                      "every %d-th query point in node order (%.3g candidate evaluations, "
+                      "%.1f s of join)" % (every, done, tj)}
     print(json.dumps(line), flush=True)
 
 

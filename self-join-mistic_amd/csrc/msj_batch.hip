@@ -44,55 +44,79 @@ namespace msj {
 
 namespace {
 
-constexpr uint32_t CHUNK = 1;  // items per queue ticket
-
-// Persistent kernel: waves pop items of any job from the queue.
-__global__ __launch_bounds__(256) void k_join_batch(const JobView *__restrict__ views,
-                                                    const uint64_t *__restrict__ job_off,
-                                                    uint32_t njobs,
-                                                    unsigned long long *__restrict__ queue)
+// Persistent kernel of one row-width class (DS = ds <= 32, or 0 for the
+// slab-major variant of ds > 32): its waves pop tickets from the class's
+// queue; ticket t is item job_off[jb] + t of the table, i.e. an item of one
+// of the class's jobs [jb, je).  At most `limit` tickets are served (the
+// autotuner times prefixes of the queue).
+template <int DS>
+__device__ inline void batch_loop(const JobView *__restrict__ views,
+                                  const uint64_t *__restrict__ job_off, uint32_t jb, uint32_t je,
+                                  unsigned long long *__restrict__ queue, uint64_t limit)
 {
     __shared__ double qsh[WPB][LDS_WAVE_BYTES / 8];
     __shared__ unsigned long long hsh[WPB][HITS];
     const uint32_t lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
     WaveHits H(hsh[wv], lane);
     uint64_t dims = 0;
-    const uint64_t total = job_off[njobs];
+    const uint64_t base = job_off[jb];
+    const uint64_t total = min(job_off[je] - base, limit);
     // every wave leaves the loop once the queue is drained
     while (true) {
         unsigned long long t = 0;
         if (lane == 0)
-            t = atomicAdd(queue, (unsigned long long)CHUNK);
+            t = atomicAdd(queue, 1ull);
         t = __shfl(t, 0, WAVE);
         if (t >= total)
             break;
-        const uint64_t tend = min((uint64_t)t + CHUNK, total);
-        for (uint64_t i = t; i < tend; ++i) {
-            uint32_t lo = 0, hi = njobs;  // job with job_off[j] <= i < job_off[j+1]
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) / 2;
-                if (job_off[mid] <= i)
-                    lo = mid;
-                else
-                    hi = mid;
-            }
-            const JobView J = views[lo];
-            const JoinItem w = J.items[i - job_off[lo]];
-            item_any<false>(J, w, qsh[wv], H, dims);
-            if (H.fill)
-                H.flush(J);  // the keys belong to this job
+        const uint64_t i = base + t;
+        uint32_t lo = jb, hi = je;  // job with job_off[j] <= i < job_off[j+1]
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (job_off[mid] <= i)
+                lo = mid;
+            else
+                hi = mid;
         }
+        const JobView J = views[lo];
+        const JoinItem w = J.items[i - job_off[lo]];
+        if (DS > 0)
+            item_sym<(DS > 0 ? DS : 4), false>(J, w, qsh[wv], H, dims);
+        else
+            item_hd<false>(J, w, qsh[wv], H, dims);
+        if (H.fill)
+            H.flush(J);  // the keys belong to this job
     }
 }
 
+template <int DS>
+__global__ __launch_bounds__(256) void k_join_batch(const JobView *__restrict__ views,
+                                                    const uint64_t *__restrict__ job_off,
+                                                    uint32_t jb, uint32_t je,
+                                                    unsigned long long *__restrict__ queue,
+                                                    uint64_t limit)
+{
+    batch_loop<DS>(views, job_off, jb, je, queue, limit);
+}
+
+// ds > 32: three waves per SIMD (the slab-major code fits 168 VGPRs).
+__global__ __launch_bounds__(256, 3) void k_join_batch_hd(const JobView *__restrict__ views,
+                                                          const uint64_t *__restrict__ job_off,
+                                                          uint32_t jb, uint32_t je,
+                                                          unsigned long long *__restrict__ queue,
+                                                          uint64_t limit)
+{
+    batch_loop<0>(views, job_off, jb, je, queue, limit);
+}
+
 __global__ void k_batch_reset(const JobView *__restrict__ views, uint32_t njobs,
-                              unsigned long long *__restrict__ queue)
+                              unsigned long long *__restrict__ queues, uint32_t nq)
 {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j < njobs)
         *views[j].counter = 0ull;
-    if (j == 0)
-        *queue = 0ull;
+    if (j < nq)
+        queues[j] = 0ull;
 }
 
 __global__ void k_batch_counts(const JobView *__restrict__ views, uint32_t njobs,
@@ -138,6 +162,43 @@ double now_ms()
         .count();
 }
 
+
+// One class: the jobs of one padded row width, [jb, je) of the sorted table.
+struct Cls {
+    uint32_t ds, jb, je;
+    uint64_t epi;   // tuned lane evaluations per item
+    uint32_t wg;    // tuned workgroups per CU
+};
+
+constexpr uint32_t MAX_CLASSES = 8;  // padded widths 4..32 step 4, and ds > 32
+
+void launch_class(const Cls &c, uint32_t grid, const JobView *views, const uint64_t *job_off,
+                  unsigned long long *queue, uint64_t limit, hipStream_t s)
+{
+    check_grid(grid, WAVE * WPB);
+    const dim3 g(grid), blk(WAVE * WPB);
+    switch (c.ds) {
+#define MSJ_C(v)                                                                            \
+    case v:                                                                                 \
+        hipLaunchKernelGGL(k_join_batch<v>, g, blk, 0, s, views, job_off, c.jb, c.je, queue, \
+                           limit);                                                          \
+        break;
+        MSJ_C(4)
+        MSJ_C(8)
+        MSJ_C(12)
+        MSJ_C(16)
+        MSJ_C(20)
+        MSJ_C(24)
+        MSJ_C(28)
+        MSJ_C(32)
+#undef MSJ_C
+    default:
+        hipLaunchKernelGGL(k_join_batch_hd, g, blk, 0, s, views, job_off, c.jb, c.je, queue,
+                           limit);
+    }
+    MSJ_HIP(hipGetLastError());
+}
+
 }  // namespace
 
 struct BatchEngine {
@@ -145,17 +206,18 @@ struct BatchEngine {
     std::vector<std::unique_ptr<BatchJob>> jobs;
     std::vector<std::unique_ptr<Worker>> workers;
     hipStream_t stream = nullptr;
-    hipEvent_t ev[2] = {};
+    hipStream_t cstream[MAX_CLASSES] = {};  // one per class: the branches of the graph
+    hipEvent_t ev[2] = {}, fork = nullptr, join_ev[MAX_CLASSES] = {};
     DevBuf<JobView> views;
     DevBuf<uint64_t> job_off;
-    DevBuf<unsigned long long> queue, rcounts;
-    std::map<uint32_t, uint64_t> epi;  // padded row width -> evaluations per item
-    uint32_t wg_per_cu = 0;            // 0: not tuned yet
+    DevBuf<unsigned long long> queues, rcounts;
+    std::vector<uint32_t> order;  // table position -> job (sorted by padded width)
+    std::vector<Cls> cls;
+    std::map<uint32_t, std::pair<uint64_t, uint32_t>> tuned;  // ds -> (epi, wg)
     int cus = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
-    uint32_t graph_njobs = 0, graph_grid = 0;
-    const void *graph_views = nullptr;
+    std::vector<uint64_t> graph_key;
 
     // Runs f(job index, worker) for all jobs, job j on worker j % W.
     template <class F>
@@ -182,121 +244,165 @@ struct BatchEngine {
                 std::rethrow_exception(e);
     }
 
-    void upload_table(const std::vector<uint32_t> &sel, hipStream_t s)
+    // Classes and the job table (sorted by padded width) from the plans.
+    void upload_table()
     {
+        order.resize(jobs.size());
+        for (uint32_t j = 0; j < order.size(); ++j)
+            order[j] = j;
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+            return padded_dim(jobs[a]->d) < padded_dim(jobs[b]->d);
+        });
+        std::vector<Cls> nc;
         std::vector<JobView> hv;
         std::vector<uint64_t> off(1, 0);
-        for (uint32_t j : sel) {
-            BatchJob &B = *jobs[j];
+        for (uint32_t k = 0; k < order.size(); ++k) {
+            BatchJob &B = *jobs[order[k]];
+            const uint32_t ds = padded_dim(B.d);
+            const uint32_t key = ds <= 32 ? ds : 0u;  // every ds > 32 shares one kernel
+            if (nc.empty() || (nc.back().ds <= 32 ? nc.back().ds : 0u) != key)
+                nc.push_back(Cls{ds, k, k, 0, 0});
+            nc.back().je = k + 1;
             hv.push_back(B.join.view(B.ix, B.xr.p, B.n, B.d, B.eps));
             off.push_back(off.back() + B.join.nitems);
         }
+        if (nc.size() > MAX_CLASSES)
+            throw Error(MSJ_E_UNSUPPORTED, "too many row-width classes");
+        for (auto &c : nc) {
+            const auto it = tuned.find(c.ds <= 32 ? c.ds : 0u);
+            c.epi = it == tuned.end() ? 0 : it->second.first;
+            c.wg = it == tuned.end() ? 2 : it->second.second;
+        }
+        cls = nc;
         views.ensure(std::max<size_t>(hv.size(), 1));
         job_off.ensure(off.size());
         if (!hv.empty())
             MSJ_HIP(hipMemcpyAsync(views.p, hv.data(), hv.size() * sizeof(JobView),
-                                   hipMemcpyHostToDevice, s));
+                                   hipMemcpyHostToDevice, stream));
         MSJ_HIP(hipMemcpyAsync(job_off.p, off.data(), off.size() * sizeof(uint64_t),
-                               hipMemcpyHostToDevice, s));
-        MSJ_HIP(hipStreamSynchronize(s));  // the host vectors go out of scope
+                               hipMemcpyHostToDevice, stream));
+        MSJ_HIP(hipStreamSynchronize(stream));  // the host vectors go out of scope
     }
 
     uint32_t grid_for_wg(uint32_t wg) const { return (uint32_t)std::max(1, cus) * wg; }
 
-    // Direct launch over the uploaded table of `nj` jobs; returns device ms.
-    float launch_direct(uint32_t nj, uint32_t grid)
+    // Reset, then one persistent kernel per class on its own stream (the
+    // classes run concurrently), joined back on `stream`.
+    void enqueue_all(uint64_t limit, int only)
     {
         hipStream_t s = stream;
-        MSJ_HIP(hipEventRecord(ev[0], s));
-        LAUNCH(k_batch_reset, grid_for(std::max<uint32_t>(nj, 1), 64), 64, views.p, nj, queue.p);
-        LAUNCH(k_join_batch, grid, WAVE * WPB, views.p, job_off.p, nj, queue.p);
-        MSJ_HIP(hipEventRecord(ev[1], s));
+        const uint32_t nj = (uint32_t)jobs.size();
+        LAUNCH(k_batch_reset, grid_for(std::max<uint32_t>(nj, MAX_CLASSES), 64), 64, views.p, nj,
+               queues.p, MAX_CLASSES);
+        MSJ_HIP(hipEventRecord(fork, s));
+        for (uint32_t c = 0; c < cls.size(); ++c) {
+            if (only >= 0 && (int)c != only)
+                continue;
+            MSJ_HIP(hipStreamWaitEvent(cstream[c], fork, 0));
+            launch_class(cls[c], grid_for_wg(cls[c].wg), views.p, job_off.p, queues.p + c, limit,
+                         cstream[c]);
+            MSJ_HIP(hipEventRecord(join_ev[c], cstream[c]));
+            MSJ_HIP(hipStreamWaitEvent(s, join_ev[c], 0));
+        }
+    }
+
+    float timed(uint64_t limit, int only)
+    {
+        MSJ_HIP(hipEventRecord(ev[0], stream));
+        enqueue_all(limit, only);
+        MSJ_HIP(hipEventRecord(ev[1], stream));
         MSJ_HIP(hipEventSynchronize(ev[1]));
         float ms = 0.f;
         MSJ_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
         return ms;
     }
 
-    // The same through the captured graph (captured again when the job
-    // count, the grid or the table's address changes).
-    float launch_graph(uint32_t nj, uint32_t grid)
+    // The whole join through the captured graph (captured again when the
+    // table's address, the jobs, the classes or their grids change).
+    float launch_graph()
     {
-        hipStream_t s = stream;
-        if (!gexec || graph_njobs != nj || graph_grid != grid || graph_views != views.p) {
+        std::vector<uint64_t> key = {(uint64_t)(uintptr_t)views.p, (uint64_t)(uintptr_t)job_off.p,
+                                     jobs.size()};
+        for (auto &c : cls) {
+            key.push_back(c.ds);
+            key.push_back(c.jb);
+            key.push_back(c.je);
+            key.push_back(c.wg);
+        }
+        if (!gexec || key != graph_key) {
             if (gexec)
                 (void)hipGraphExecDestroy(gexec);
             if (graph)
                 (void)hipGraphDestroy(graph);
             gexec = nullptr;
             graph = nullptr;
-            MSJ_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            hipLaunchKernelGGL(k_batch_reset, dim3(grid_for(std::max<uint32_t>(nj, 1), 64)),
-                               dim3(64), 0, s, views.p, nj, queue.p);
-            hipLaunchKernelGGL(k_join_batch, dim3(grid), dim3(WAVE * WPB), 0, s, views.p,
-                               job_off.p, nj, queue.p);
-            MSJ_HIP(hipStreamEndCapture(s, &graph));
+            MSJ_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            enqueue_all(~0ull, -1);
+            MSJ_HIP(hipStreamEndCapture(stream, &graph));
             MSJ_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
-            graph_njobs = nj;
-            graph_grid = grid;
-            graph_views = views.p;
+            graph_key = key;
         }
-        MSJ_HIP(hipEventRecord(ev[0], s));
-        MSJ_HIP(hipGraphLaunch(gexec, s));
-        MSJ_HIP(hipEventRecord(ev[1], s));
+        MSJ_HIP(hipEventRecord(ev[0], stream));
+        MSJ_HIP(hipGraphLaunch(gexec, stream));
+        MSJ_HIP(hipEventRecord(ev[1], stream));
         MSJ_HIP(hipEventSynchronize(ev[1]));
         float ms = 0.f;
         MSJ_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
         return ms;
     }
 
-    void plan_job(uint32_t j, Worker &W, uint64_t e)
+    void replan(uint32_t key_ds, uint64_t e)
     {
-        BatchJob &B = *jobs[j];
-        B.join.plan(B.ix, B.n, B.d, W.s, true, e);
+        parallel([&](uint32_t j, Worker &W) {
+            BatchJob &B = *jobs[j];
+            const uint32_t ds = padded_dim(B.d);
+            if ((ds <= 32 ? ds : 0u) == key_ds)
+                B.join.plan(B.ix, B.n, B.d, W.s, true, e);
+        });
     }
 
-    // Autotuning (first run): evaluations per item per row width, then
-    // workgroups per CU for the whole batch.
+    // Autotuning of the classes not tuned yet: lane evaluations per item
+    // (2^16, 2^18, 2^20, with 2 workgroups per CU), then workgroups per CU
+    // (1, 2, 4); each trial runs the class alone over a prefix of its queue
+    // (an eighth, at least 4096 items), timed per full queue.
     void tune()
     {
-        std::map<uint32_t, std::vector<uint32_t>> cls;
-        for (uint32_t j = 0; j < jobs.size(); ++j)
-            cls[padded_dim(jobs[j]->d)].push_back(j);
-        const uint64_t cand[3] = {1ull << 16, 1ull << 18, 1ull << 20};
-        for (auto &kv : cls) {
-            if (epi.count(kv.first))
+        upload_table();
+        for (uint32_t c = 0; c < cls.size(); ++c) {
+            const uint32_t key = cls[c].ds <= 32 ? cls[c].ds : 0u;
+            if (tuned.count(key))
                 continue;
-            float best = 0.f;
-            uint64_t best_e = cand[1];
-            for (uint64_t e : cand) {
-                for (uint32_t j : kv.second)
-                    plan_job(j, *workers[0], e);
-                MSJ_HIP(hipStreamSynchronize(workers[0]->s));
-                upload_table(kv.second, stream);
-                uint32_t nj = (uint32_t)kv.second.size();
-                launch_direct(nj, grid_for_wg(2));  // warm
-                const float ms = launch_direct(nj, grid_for_wg(2));
-                if (best == 0.f || ms < best) {
-                    best = ms;
+            auto trial = [&](uint32_t wg) {
+                upload_table();
+                cls[c].wg = wg;
+                uint64_t items = 0;
+                for (uint32_t k = cls[c].jb; k < cls[c].je; ++k)
+                    items += jobs[order[k]]->join.nitems;
+                const uint64_t lim = std::min<uint64_t>(items, std::max<uint64_t>(4096, items / 8));
+                const float ms = timed(lim, (int)c);
+                return items ? ms * (double)items / (double)lim : 0.0;
+            };
+            double best = -1.0;
+            uint64_t best_e = 1ull << 18;
+            for (uint64_t e : {1ull << 16, 1ull << 18, 1ull << 20}) {
+                replan(key, e);
+                const double t = trial(2);
+                if (best < 0 || t < best) {
+                    best = t;
                     best_e = e;
                 }
             }
-            epi[kv.first] = best_e;
-        }
-        if (wg_per_cu == 0) {
-            parallel([&](uint32_t j, Worker &W) { plan_job(j, W, epi[padded_dim(jobs[j]->d)]); });
-            std::vector<uint32_t> all(jobs.size());
-            for (uint32_t j = 0; j < all.size(); ++j)
-                all[j] = j;
-            upload_table(all, stream);
-            float best = 0.f;
+            replan(key, best_e);
+            best = -1.0;
+            uint32_t best_wg = 2;
             for (uint32_t wg : {1u, 2u, 4u}) {
-                const float ms = launch_direct((uint32_t)all.size(), grid_for_wg(wg));
-                if (best == 0.f || ms < best) {
-                    best = ms;
-                    wg_per_cu = wg;
+                const double t = trial(wg);
+                if (best < 0 || t < best) {
+                    best = t;
+                    best_wg = wg;
                 }
             }
+            tuned[key] = {best_e, best_wg};
         }
     }
 
@@ -314,9 +420,17 @@ struct BatchEngine {
                 (void)hipStreamSynchronize(w->s);
                 (void)hipStreamDestroy(w->s);
             }
+        for (auto &cs : cstream)
+            if (cs)
+                (void)hipStreamDestroy(cs);
         for (auto &e : ev)
             if (e)
                 (void)hipEventDestroy(e);
+        for (auto &e : join_ev)
+            if (e)
+                (void)hipEventDestroy(e);
+        if (fork)
+            (void)hipEventDestroy(fork);
         if (stream)
             (void)hipStreamDestroy(stream);
     }
@@ -327,8 +441,13 @@ BatchEngine *batch_create(const msj_params &p)
     std::unique_ptr<BatchEngine> b(new BatchEngine());
     b->prm = p;
     MSJ_HIP(hipStreamCreate(&b->stream));
+    for (auto &cs : b->cstream)
+        MSJ_HIP(hipStreamCreate(&cs));
     for (auto &e : b->ev)
         MSJ_HIP(hipEventCreate(&e));
+    for (auto &e : b->join_ev)
+        MSJ_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    MSJ_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
     int cus = 0;
     MSJ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p.device));
     b->cus = cus;
@@ -337,7 +456,7 @@ BatchEngine *batch_create(const msj_params &p)
         b->workers.emplace_back(new Worker());
         MSJ_HIP(hipStreamCreate(&b->workers.back()->s));
     }
-    b->queue.ensure(1);
+    b->queues.ensure(MAX_CLASSES);
     return b.release();
 }
 
@@ -358,8 +477,6 @@ void batch_load(BatchEngine *b, const msj_job *jobs, uint32_t njobs)
             throw Error(MSJ_E_OVERFLOW, "n*d too large");
     }
     b->jobs.clear();
-    b->epi.clear();
-    b->wg_per_cu = 0;
     for (uint32_t j = 0; j < njobs; ++j) {
         b->jobs.emplace_back(new BatchJob());
         b->jobs.back()->n = jobs[j].n;
@@ -397,8 +514,9 @@ void batch_run(BatchEngine *b, msj_result *outs, msj_batch_stats *st)
         B.join.order(B.ix, B.xs.p, f32, B.n, B.d, W.s);
         B.xr.ensure((size_t)B.n * padded_dim(B.d));
         gather_node_rows(B.xs.p, B.xr.p, f32, B.join.jperm.p, B.n, B.d, W.s);
-        const auto it = b->epi.find(padded_dim(B.d));
-        B.join.plan(B.ix, B.n, B.d, W.s, true, it == b->epi.end() ? 0 : it->second);
+        const uint32_t ds = padded_dim(B.d);
+        const auto it = b->tuned.find(ds <= 32 ? ds : 0u);
+        B.join.plan(B.ix, B.n, B.d, W.s, true, it == b->tuned.end() ? 0 : it->second.first);
         B.join.counter.ensure(1);
         const uint64_t cap = std::max<uint64_t>(B.last_R + B.last_R / 8, 64ull * B.n + 4096);
         if (B.join.pairs.cap < cap)
@@ -409,38 +527,39 @@ void batch_run(BatchEngine *b, msj_result *outs, msj_batch_stats *st)
     const double t1 = now_ms();
     S.ms_index = t1 - t0;
 
-    // autotuning on the first run of these jobs
-    if (b->wg_per_cu == 0) {
+    // autotuning of the row widths not seen before
+    bool need = false;
+    for (auto &J : b->jobs) {
+        const uint32_t ds = padded_dim(J->d);
+        need = need || !b->tuned.count(ds <= 32 ? ds : 0u);
+    }
+    if (need) {
         b->tune();
-        b->parallel([&](uint32_t j, Worker &W) {
-            BatchJob &B = *b->jobs[j];
-            B.join.plan(B.ix, B.n, B.d, W.s, true, b->epi[padded_dim(B.d)]);
-        });
         S.ms_tune = now_ms() - t1;
     }
 
     // 2. the persistent join over all jobs, from the captured graph
-    std::vector<uint32_t> all(nj);
-    for (uint32_t j = 0; j < nj; ++j)
-        all[j] = j;
-    b->upload_table(all, b->stream);
-    const uint32_t grid = b->grid_for_wg(b->wg_per_cu);
-    S.ms_join = b->launch_graph(nj, grid);
+    b->upload_table();
+    S.ms_join = b->launch_graph();
     S.graph_replays = 1;
-    S.grid_blocks = grid;
-    std::vector<unsigned long long> R(nj);
+    for (auto &c : b->cls)
+        S.grid_blocks += b->grid_for_wg(c.wg);
+    std::vector<unsigned long long> Rs(nj);
     {
         hipStream_t s = b->stream;
         LAUNCH(k_batch_counts, grid_for(std::max<uint32_t>(nj, 1), 64), 64, b->views.p, nj,
                b->rcounts.p);
         if (nj)
-            MSJ_HIP(hipMemcpyAsync(R.data(), b->rcounts.p, nj * sizeof(unsigned long long),
+            MSJ_HIP(hipMemcpyAsync(Rs.data(), b->rcounts.p, nj * sizeof(unsigned long long),
                                    hipMemcpyDeviceToHost, s));
         MSJ_HIP(hipStreamSynchronize(s));
     }
+    std::vector<unsigned long long> R(nj);  // table order -> job order
+    for (uint32_t k = 0; k < nj; ++k)
+        R[b->order[k]] = Rs[k];
 
-    // 3. outputs (a job whose pair buffer overflowed is rerun alone with
-    // an exact buffer)
+    // 3. outputs (a job whose pair buffer overflowed is rerun alone with an
+    // exact buffer)
     const double t2 = now_ms();
     b->parallel([&](uint32_t j, Worker &W) {
         BatchJob &B = *b->jobs[j];
@@ -513,11 +632,11 @@ void batch_run(BatchEngine *b, msj_result *outs, msj_batch_stats *st)
         S.work_items += b->jobs[j]->join.nitems;
     }
     uint32_t k = 0;
-    for (auto &kv : b->epi) {
+    for (auto &c : b->cls) {
         if (k == 8)
             break;
-        S.class_ds[k] = kv.first;
-        S.class_epi_log2[k] = (uint32_t)bits_for(kv.second) - 1;
+        S.class_ds[k] = c.ds <= 32 ? c.ds : 0u;
+        S.class_epi_log2[k] = (uint32_t)bits_for(c.epi) - 1;
         ++k;
     }
     S.classes = k;

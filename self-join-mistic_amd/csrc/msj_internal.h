@@ -182,6 +182,7 @@ struct JobView {
     uint32_t d, ds;
     double eps2;
     const uint32_t *nstart, *uoff, *uadj, *perm;  // perm: join position -> row id
+    const uint32_t *ucum;        // per upper entry: first offset in the node's list
     const JoinItem *items;
     uint32_t nitems;
     int qbits;
@@ -190,7 +191,7 @@ struct JobView {
 };
 
 struct JoinRunner {
-    DevBuf<uint32_t> itcnt, itoff, uoff, uadj;
+    DevBuf<uint32_t> itcnt, itoff, uoff, uadj, ucum;
     DevBuf<uint64_t> ulen, tot64;
     DevBuf<JoinItem> items;
     DevBuf<unsigned long long> counter, work_ctr;

@@ -94,7 +94,8 @@ __global__ __launch_bounds__(256) void k_join_sym(JobView J, unsigned long long 
 
 // The same for ds > 32 (slab-major variant).
 template <bool COUNT>
-__global__ __launch_bounds__(256) void k_join_hd(JobView J, unsigned long long *__restrict__ work)
+__global__ __launch_bounds__(256, 3) void k_join_hd(JobView J,
+                                                    unsigned long long *__restrict__ work)
 {
     __shared__ double qsh[WPB][HQ * HSL];
     __shared__ unsigned long long hsh[WPB][HITS];
@@ -135,18 +136,26 @@ __global__ __launch_bounds__(256) void k_upper_count(const uint32_t *__restrict_
     len[x] = L;
 }
 
-__global__ __launch_bounds__(256) void k_upper_write(const uint32_t *__restrict__ aoff,
+// ... and, per entry, the offset of its first point inside the node's
+// concatenated upper candidate list (a work item finds its first entry by
+// binary search).
+__global__ __launch_bounds__(256) void k_upper_write(const uint32_t *__restrict__ nstart,
+                                                     const uint32_t *__restrict__ aoff,
                                                      const uint32_t *__restrict__ adj, uint32_t M,
                                                      const uint32_t *__restrict__ uoff,
-                                                     uint32_t *__restrict__ uadj)
+                                                     uint32_t *__restrict__ uadj,
+                                                     uint32_t *__restrict__ ucum)
 {
     uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= M)
         return;
-    uint32_t o = uoff[x];
+    uint32_t o = uoff[x], c = 0;
     for (uint32_t e = aoff[x]; e < aoff[x + 1]; ++e)
-        if (adj[e] >= x)
+        if (adj[e] >= x) {
+            ucum[o] = c;
+            c += nstart[adj[e] + 1] - nstart[adj[e]];
             uadj[o++] = adj[e];
+        }
 }
 
 // Items per node: query ranges of up to qmax points x candidate ranges sized
@@ -416,6 +425,7 @@ JobView JoinRunner::view(const Index &ix, const double *xr, uint32_t n, uint32_t
     V.nstart = ix.nstart.p;
     V.uoff = uoff.p;
     V.uadj = uadj.p;
+    V.ucum = ucum.p;
     V.perm = jperm.p;
     V.items = items.p;
     V.nitems = nitems;
@@ -445,7 +455,9 @@ void JoinRunner::plan(const Index &ix, uint32_t n, uint32_t d, hipStream_t s, bo
     MSJ_HIP(hipMemcpyAsync(&ue, uoff.p + M, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     MSJ_HIP(hipStreamSynchronize(s));
     uadj.ensure(std::max<uint32_t>(ue, 1));
-    LAUNCH(k_upper_write, grid_for(M, 256), 256, ix.aoff.p, ix.adj.p, M, uoff.p, uadj.p);
+    ucum.ensure(std::max<uint32_t>(ue, 1));
+    LAUNCH(k_upper_write, grid_for(M, 256), 256, ix.nstart.p, ix.aoff.p, ix.adj.p, M, uoff.p,
+           uadj.p, ucum.p);
     if (!plain_items) {
         nitems = 0;
         return;

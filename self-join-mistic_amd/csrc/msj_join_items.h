@@ -102,6 +102,21 @@ struct WaveHits {
     }
 };
 
+// The entry of X's upper list whose points hold candidate offset c (the
+// last entry starting at or before c): binary search, wave-uniform.
+__device__ inline uint32_t first_entry(const JobView &J, uint32_t X, uint32_t c)
+{
+    uint32_t lo = J.uoff[X], hi = J.uoff[X + 1];
+    while (hi - lo > 1) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (J.ucum[mid] <= c)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
 // Rows of xr are ds = roundup(d, 4) doubles, zero-padded: a padded
 // coordinate contributes s + (0 - 0)^2 = s exactly, so the chunk loop needs
 // no per-dimension guard.  DS = ds (<= 32): candidate rows live in VGPRs.
@@ -127,10 +142,11 @@ __device__ inline void item_sym(const JobView &J, const JoinItem &w, double *qs,
     wave_sync();
     const uint64_t qall = qn >= WAVE ? ~0ull : ((1ull << qn) - 1ull);
 
-    // per-lane cursor into X's upper neighbour list
-    uint32_t a = J.uoff[X];
+    // per-lane cursor into X's upper neighbour list, from the entry holding
+    // the item's first candidate
+    uint32_t a = first_entry(J, X, w.cbeg);
+    uint32_t ybase = J.ucum[a];
     uint32_t Y = J.uadj[a];
-    uint32_t ybase = 0;
     uint32_t ystart = J.nstart[Y];
     uint32_t ysz = J.nstart[Y + 1] - ystart;
 
@@ -210,9 +226,9 @@ __device__ inline void item_hd(const JobView &J, const JoinItem &w, double *qs, 
     const uint32_t x0 = J.nstart[X];
     const uint32_t qn = w.qend - w.qbeg;  // <= HQ
     const uint32_t q0 = x0 + w.qbeg;      // first query (node order)
-    uint32_t a = J.uoff[X];
+    uint32_t a = first_entry(J, X, w.cbeg);
+    uint32_t ybase = J.ucum[a];
     uint32_t Y = J.uadj[a];
-    uint32_t ybase = 0;
     uint32_t ystart = J.nstart[Y];
     uint32_t ysz = J.nstart[Y + 1] - ystart;
     for (uint32_t cs = w.cbeg; cs < w.cend; cs += WAVE) {

@@ -67,6 +67,21 @@ def clusters(n, d, seed, ncl=1000, zipf=1.1, sigma=0.05):
     return out
 
 
+def c5_specs(seed=1005, jobs=64):
+    """Config 5: `jobs` independent instances; sizes log-uniform in
+    [10^3, 10^6], d from {8, 18, 57, 90}, mixture data (Gaussian clusters +
+    uniform noise), target mean neighbours uniform in [16, 64].  Returns
+    (n, d, data seed, calibration seed, target) per instance."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(jobs):
+        n = int(round(10 ** rng.uniform(3.0, 6.0)))
+        d = int(rng.choice([8, 18, 57, 90]))
+        target = int(rng.integers(16, 65))
+        out.append((n, d, seed * 1000 + k, seed * 1000 + 500 + k, target))
+    return out
+
+
 def reorder_dims(x, order):
     """main.cu:117-122: column j of the result is column order[j] of x."""
     return np.ascontiguousarray(x[:, np.asarray(order, dtype=np.int64)])
