@@ -61,10 +61,10 @@ namespace msj {
 
 namespace {
 
-constexpr uint32_t TM = 128;     // rows per workgroup tile, A and B
+constexpr uint32_t TM = 256;     // rows per workgroup tile, A and B (== threads per block)
 constexpr uint32_t KSL = 2;      // 16-dim k-steps per LDS slab
 constexpr uint32_t SBS = 8;      // tiles per side of a super-block
-constexpr uint32_t MHITS = 256;  // per-wave LDS buffer of result keys
+constexpr uint32_t MHITS = 128;  // per-wave LDS buffer of result keys
 constexpr uint32_t KALIGN = 16 * KSL;
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -317,15 +317,23 @@ __global__ __launch_bounds__(256) void k_sblk_fill(const uint32_t *__restrict__ 
 // (count_work): class 1, a certain accept (every one is verified); class 2,
 // a certain reject of a sampled work item (verified).
 //
-// Epilogue.  The A rows' terms are staged per item and the B rows' per tile;
-// rows outside the item (padding, the next node) get +inf terms, so the
-// fast pass -- one add, one fma, one compare per pair -- rejects them
-// without a validity test.  Only a wave whose 64 x 64 block holds a pair
-// that is not a certain reject (or NaN-scored) takes the full pass, which
-// tests validity explicitly, collects certain accepts into a lane mask and
-// stages band entries in LDS.
+// Tile.  A workgroup (8 waves) computes a 256 x 256 tile; wave (wr, wc) the
+// 128 x 64 block of rows 128 wr.. and columns 64 wc.., as 4 x 2 MFMA 32x32
+// tiles (128 FP32 accumulators per lane).  Per 32-dimension slab the
+// workgroup copies 16 row blocks of fragments (8 of A, 8 of B; both planes
+// when split) from global memory straight into LDS, and each wave reads its
+// 4 + 2 fragments per k-step from there; the next slab's copy runs into the
+// other LDS buffer while the matrix cores work on the current one.
+//
+// Epilogue.  The A and B rows' terms are staged per tile; rows outside the
+// item (padding, the next node) get +inf terms, so the fast pass -- one add,
+// one fma, one compare per pair -- rejects them without a validity test.
+// Only a wave whose 128 x 64 block holds a pair that is not a certain
+// reject (or NaN-scored) takes the full pass, one 32-row block at a time:
+// it tests validity explicitly, collects certain accepts into a lane mask
+// and stages band entries in LDS.
 template <bool SPLIT, bool DIAG>
-__global__ __launch_bounds__(256, 2) void k_mfma_join(
+__global__ __launch_bounds__(512, 1) void k_mfma_join(
     const uint4 *__restrict__ fh, const uint4 *__restrict__ fl, uint32_t KS,
     const float4 *__restrict__ thr, const uint32_t *__restrict__ porig,
     const uint32_t *__restrict__ pjoin, const MItem *__restrict__ items, SlotMap sm,
@@ -335,22 +343,27 @@ __global__ __launch_bounds__(256, 2) void k_mfma_join(
     uint32_t sample_every, unsigned long long band_budget)
 {
     constexpr uint32_t NP = SPLIT ? 2 : 1;
-    constexpr uint32_t SLAB = 4 * KSL * WAVE;  // uint4 per plane per operand
-    constexpr uint32_t PER_T = SLAB / 256;     // per thread
-    constexpr uint32_t BHITS = 128;            // per-wave band staging
-    __shared__ uint4 sA[NP][SLAB];
-    __shared__ uint4 sB[NP][SLAB];
+    constexpr uint32_t RB = TM / 32;              // row blocks per tile side
+    constexpr uint32_t SLAB = RB * KSL * WAVE;    // uint4 per plane per operand
+    constexpr uint32_t UNITS = 2 * NP * RB * KSL;  // 1 KB copies per slab
+    constexpr uint32_t NW = 8;                    // waves per workgroup
+    constexpr uint32_t BHITS = 64;                // per-wave band staging
+    static_assert(UNITS % NW == 0, "every wave issues the same number of copies");
+    // two slab buffers: the next slab is copied (global -> LDS, no VGPRs)
+    // while the matrix cores work on the current one
+    __shared__ uint4 sA[2][NP][SLAB];
+    __shared__ uint4 sB[2][NP][SLAB];
     __shared__ float4 tA[TM], tB[TM];
     __shared__ uint32_t oA[TM], oB[TM], jA[TM], jB[TM];
-    __shared__ unsigned long long hsh[4][MHITS];
-    __shared__ unsigned long long bsh[4][BHITS];
-    __shared__ float ssh[4][BHITS];
+    __shared__ unsigned long long hsh[NW][MHITS];
+    __shared__ unsigned long long bsh[NW][BHITS];
+    __shared__ float ssh[NW][BHITS];
     __shared__ uint32_t s_stop;
     if (threadIdx.x == 0)
         s_stop = 0;
 
     const uint32_t tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid / WAVE;
-    const uint32_t wr = wv >> 1, wc = wv & 1, h = lane / 32;
+    const uint32_t wr = wv >> 2, wc = wv & 3, h = lane / 32;
     const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (WAVE - lane));
     const float INF = __builtin_inff();
     unsigned long long *hb = hsh[wv];
@@ -406,32 +419,28 @@ __global__ __launch_bounds__(256, 2) void k_mfma_join(
         const uint32_t sz = SBS - cls;
         const uint64_t local = have ? t - sm.slot_base[cls] : 0;
         const uint64_t it = sm.item_base[cls] + local / (sz * sz);
-        const uint32_t r = (uint32_t)(local % (sz * sz)), ta = r / sz, tb = r % sz;
+        const uint32_t r = (uint32_t)(local % (sz * sz)), ta = r / sz, tbk = r % sz;
         MItem w = {0, 0, 0, 0, 0};
         if (have)
             w = items[it];
-        const uint32_t qbeg = w.qbeg + ta * TM, cb = w.cbeg + tb * TM;
-        have = have && qbeg < w.qend && cb < w.cend && !((w.self & 0x200u) && tb < ta);
+        const uint32_t qbeg = w.qbeg + ta * TM, cb = w.cbeg + tbk * TM;
+        have = have && qbeg < w.qend && cb < w.cend && !((w.self & 0x200u) && tbk < ta);
         __syncthreads();  // the previous tile is done with tA, tB and the slabs
         // the band counter is read every 32nd slot of a workgroup: one global
         // address polled by every workgroup on every tile is a hot spot
         if (tid == 0 && band_budget != 0 && ((v / gridDim.x) & 31u) == 0)
             s_stop = __hip_atomic_load(&ctr[7], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >
                      band_budget;
-        if (have && tid < TM) {
-            const uint32_t ra_ = qbeg + tid, rb_ = cb + tid;
-            float4 t4 = thr[ra_];
-            if (ra_ >= w.qend)
+        if (have) {  // threads 0..TM-1 stage A rows, TM..2TM-1 B rows
+            const bool isb = tid >= TM;
+            const uint32_t k = isb ? tid - TM : tid;
+            const uint32_t r_ = (isb ? cb : qbeg) + k;
+            float4 t4 = thr[r_];
+            if (r_ >= (isb ? w.cend : w.qend))
                 t4 = make_float4(INF, INF, INF, INF);
-            tA[tid] = t4;
-            oA[tid] = porig[ra_];
-            jA[tid] = pjoin[ra_];
-            t4 = thr[rb_];
-            if (rb_ >= w.cend)
-                t4 = make_float4(INF, INF, INF, INF);
-            tB[tid] = t4;
-            oB[tid] = porig[rb_];
-            jB[tid] = pjoin[rb_];
+            (isb ? tB : tA)[k] = t4;
+            (isb ? oB : oA)[k] = porig[r_];
+            (isb ? jB : jA)[k] = pjoin[r_];
         }
         __syncthreads();
         if (!have || s_stop)
@@ -439,156 +448,209 @@ __global__ __launch_bounds__(256, 2) void k_mfma_join(
         const bool sampled = DIAG && (it % sample_every == 0);
         const bool self = (w.self & 0x100u) != 0;
         const uint32_t rbA = qbeg / 32, rbB = cb / 32;
-        {
-            // two uint4 per thread per plane and operand (SLAB = 2 * 256):
-            // named registers, so the prefetch stays in VGPRs
-            static_assert(PER_T == 2, "prefetch registers are written out for two per thread");
-            uint4 pa0, pa1, pb0, pb1, qa0, qa1, qb0, qb1;  // p: high plane, q: low plane
-            const uint32_t rbl0 = tid / (KSL * WAVE), rem0 = tid % (KSL * WAVE);
-            const uint32_t rbl1 = (tid + 256) / (KSL * WAVE), rem1 = (tid + 256) % (KSL * WAVE);
-#define MSJ_GLOAD(K0)                                                                  \
-    {                                                                                  \
-        const size_t ga0 = ((size_t)(rbA + rbl0) * KS + (K0)) * WAVE + rem0;           \
-        const size_t ga1 = ((size_t)(rbA + rbl1) * KS + (K0)) * WAVE + rem1;           \
-        const size_t gb0 = ((size_t)(rbB + rbl0) * KS + (K0)) * WAVE + rem0;           \
-        const size_t gb1 = ((size_t)(rbB + rbl1) * KS + (K0)) * WAVE + rem1;           \
-        pa0 = fh[ga0];                                                                 \
-        pa1 = fh[ga1];                                                                 \
-        pb0 = fh[gb0];                                                                 \
-        pb1 = fh[gb1];                                                                 \
-        if (SPLIT) {                                                                   \
-            qa0 = fl[ga0];                                                             \
-            qa1 = fl[ga1];                                                             \
-            qb0 = fl[gb0];                                                             \
-            qb1 = fl[gb1];                                                             \
-        }                                                                              \
-    }
-            MSJ_GLOAD(0u)
-            floatx16 acc[2][2];
-#pragma unroll
-            for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-                for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-                    for (int e = 0; e < 16; ++e)
-                        acc[bi][bj][e] = 0.f;
-            for (uint32_t k0 = 0; k0 < KS; k0 += KSL) {
-                __syncthreads();  // all waves are done reading the previous slab
-                sA[0][tid] = pa0;
-                sA[0][tid + 256] = pa1;
-                sB[0][tid] = pb0;
-                sB[0][tid + 256] = pb1;
-                if (SPLIT) {
-                    sA[NP - 1][tid] = qa0;
-                    sA[NP - 1][tid + 256] = qa1;
-                    sB[NP - 1][tid] = qb0;
-                    sB[NP - 1][tid + 256] = qb1;
-                }
-                __syncthreads();
-                if (k0 + KSL < KS) {
-                    MSJ_GLOAD(k0 + KSL)  // in flight during this slab's MFMAs
-                }
-#pragma unroll
-                for (uint32_t kk = 0; kk < KSL; ++kk) {
-                    bf16x8 ah[2], al[2], bh[2], bl[2];
-#pragma unroll
-                    for (int bi = 0; bi < 2; ++bi) {
-                        const uint32_t o = ((2 * wr + bi) * KSL + kk) * WAVE + lane;
-                        ah[bi] = as_bf16x8(sA[0][o]);
-                        if (SPLIT)
-                            al[bi] = as_bf16x8(sA[NP - 1][o]);
-                    }
-#pragma unroll
-                    for (int bj = 0; bj < 2; ++bj) {
-                        const uint32_t o = ((2 * wc + bj) * KSL + kk) * WAVE + lane;
-                        bh[bj] = as_bf16x8(sB[0][o]);
-                        if (SPLIT)
-                            bl[bj] = as_bf16x8(sB[NP - 1][o]);
-                    }
-#pragma unroll
-                    for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-                        for (int bj = 0; bj < 2; ++bj) {
-                            acc[bi][bj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-                                ah[bi], bh[bj], acc[bi][bj], 0, 0, 0);
-                            if (SPLIT) {
-                                acc[bi][bj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-                                    ah[bi], bl[bj], acc[bi][bj], 0, 0, 0);
-                                acc[bi][bj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-                                    al[bi], bh[bj], acc[bi][bj], 0, 0, 0);
-                            }
-                        }
-                }
-            }
 
-#undef MSJ_GLOAD
-            // ---- epilogue.  C[i][j] of MFMA tile (bi, bj) is in lane
-            // j + 32h, element e, with i = 8 (e / 4) + 4 h + e % 4.
-            const bool diag_tile = self && cb == qbeg;  // inside: pairs p <= c only
-            float4 tb[2];
+        // slab copy: unit u = ((operand * NP + plane) * RB + block) * KSL + kstep
+        // is one wave-wide 16-byte-per-lane load straight into LDS (the lane
+        // layout of a fragment block is its memory layout); wave wv issues
+        // units wv, wv + 8, ...
+        floatx16 acc[4][2];
+#pragma unroll
+        for (int bi = 0; bi < 4; ++bi)
 #pragma unroll
             for (int bj = 0; bj < 2; ++bj)
-                tb[bj] = tB[64 * wc + 32 * bj + (lane & 31)];
-            bool need = sampled;
 #pragma unroll
-            for (int bi = 0; bi < 2; ++bi)
+                for (int e = 0; e < 16; ++e)
+                    acc[bi][bj][e] = 0.f;
+        // slab copy into buffer `buf`: unit u = ((operand * NP + plane) * RB +
+        // block) * KSL + kstep is one wave-wide 16-byte-per-lane load straight
+        // into LDS; wave wv issues units wv, wv + 8, ...
+        auto copy_slab = [&](uint32_t k0, uint32_t buf) {
 #pragma unroll
-                for (int e = 0; e < 16; ++e) {
-                    const uint32_t il = 64 * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
-                    const float ay = tA[il].y;
+            for (uint32_t m = 0; m < UNITS / NW; ++m) {
+                const uint32_t u = wv + NW * m;
+                const uint32_t kk = u % KSL, blk = (u / KSL) % RB, pl = (u / (KSL * RB)) % NP;
+                const uint32_t op = u / (KSL * RB * NP);
+                const uint4 *src = (pl ? fl : fh) +
+                                   ((size_t)((op ? rbB : rbA) + blk) * KS + k0 + kk) * WAVE +
+                                   lane;
+                uint4 *dst = (op ? &sB[buf][pl][0] : &sA[buf][pl][0]) + (blk * KSL + kk) * WAVE;
+                __builtin_amdgcn_global_load_lds((const void *)src,
+                                                 (void __attribute__((address_space(3))) *)dst,
+                                                 16, 0, 0);
+            }
+        };
+        // LDS byte addresses of this lane's first A and B fragments (k-step 0,
+        // buffer 0); fragment (block, k-step) lies (block * KSL + kstep) KiB
+        // further, the low plane SLAB * 16 bytes, buffer 1 NP * SLAB * 16.
+        const uint32_t ldsA = (uint32_t)(uintptr_t)(
+            (__attribute__((address_space(3))) uint4 *)&sA[0][0][(4 * wr * KSL) * WAVE + lane]);
+        const uint32_t ldsB = (uint32_t)(uintptr_t)(
+            (__attribute__((address_space(3))) uint4 *)&sB[0][0][(2 * wc * KSL) * WAVE + lane]);
+        static_assert(KSL == 2 && SLAB * 16 == 16384, "fragment read offsets below");
+        copy_slab(0, 0);
+        for (uint32_t k0 = 0, sl = 0; k0 < KS; k0 += KSL, ++sl) {
+            const uint32_t buf = sl & 1u;
+            // this wave's copies of the slab have landed; after the barrier,
+            // everyone's have, and nobody still reads the other buffer
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            __syncthreads();
+            if (k0 + KSL < KS)
+                copy_slab(k0 + KSL, buf ^ 1u);  // in flight during this slab's MFMAs
 #pragma unroll
-                    for (int bj = 0; bj < 2; ++bj) {
-                        const bool keep = !(fmaf(-2.f, acc[bi][bj][e], ay + tb[bj].y) > hi);
-                        if (diag_tile)
-                            need = need || (keep && il <= 64 * wc + 32 * bj + (lane & 31));
-                        else
-                            need = need || keep;
-                    }
+            for (uint32_t kk = 0; kk < KSL; ++kk) {
+                // the k-step's fragments in one block, waited for inside it
+                // (outputs early-clobber: no read can overwrite an address);
+                // the copy of the next slab stays in flight
+                const uint32_t va = ldsA + buf * (NP * SLAB * 16) + kk * 1024;
+                const uint32_t vb = ldsB + buf * (NP * SLAB * 16) + kk * 1024;
+                uint4 a0, a1, a2, a3, b0, b1, l0, l1, l2, l3, m0, m1;
+                if (SPLIT) {
+                    asm volatile(
+                        "ds_read_b128 %0, %12\n"
+                        "ds_read_b128 %1, %12 offset:2048\n"
+                        "ds_read_b128 %2, %12 offset:4096\n"
+                        "ds_read_b128 %3, %12 offset:6144\n"
+                        "ds_read_b128 %4, %13\n"
+                        "ds_read_b128 %5, %13 offset:2048\n"
+                        "ds_read_b128 %6, %12 offset:16384\n"
+                        "ds_read_b128 %7, %12 offset:18432\n"
+                        "ds_read_b128 %8, %12 offset:20480\n"
+                        "ds_read_b128 %9, %12 offset:22528\n"
+                        "ds_read_b128 %10, %13 offset:16384\n"
+                        "ds_read_b128 %11, %13 offset:18432\n"
+                        "s_waitcnt lgkmcnt(0)"
+                        : "=&v"(a0), "=&v"(a1), "=&v"(a2), "=&v"(a3), "=&v"(b0), "=&v"(b1),
+                          "=&v"(l0), "=&v"(l1), "=&v"(l2), "=&v"(l3), "=&v"(m0), "=&v"(m1)
+                        : "v"(va), "v"(vb)
+                        : "memory");
+                } else {
+                    asm volatile(
+                        "ds_read_b128 %0, %6\n"
+                        "ds_read_b128 %1, %6 offset:2048\n"
+                        "ds_read_b128 %2, %6 offset:4096\n"
+                        "ds_read_b128 %3, %6 offset:6144\n"
+                        "ds_read_b128 %4, %7\n"
+                        "ds_read_b128 %5, %7 offset:2048\n"
+                        "s_waitcnt lgkmcnt(0)"
+                        : "=&v"(a0), "=&v"(a1), "=&v"(a2), "=&v"(a3), "=&v"(b0), "=&v"(b1)
+                        : "v"(va), "v"(vb)
+                        : "memory");
+                    l0 = l1 = l2 = l3 = m0 = m1 = make_uint4(0, 0, 0, 0);
                 }
-            if (!__builtin_amdgcn_ballot_w64(need))
-                continue;
+                const bf16x8 ah[4] = {as_bf16x8(a0), as_bf16x8(a1), as_bf16x8(a2), as_bf16x8(a3)};
+                const bf16x8 bh[2] = {as_bf16x8(b0), as_bf16x8(b1)};
+#pragma unroll
+                for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+                    for (int bj = 0; bj < 2; ++bj)
+                        acc[bi][bj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                            ah[bi], bh[bj], acc[bi][bj], 0, 0, 0);
+                if (SPLIT) {
+                    const bf16x8 al[4] = {as_bf16x8(l0), as_bf16x8(l1), as_bf16x8(l2),
+                                          as_bf16x8(l3)};
+                    const bf16x8 bl[2] = {as_bf16x8(m0), as_bf16x8(m1)};
+#pragma unroll
+                    for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+                        for (int bj = 0; bj < 2; ++bj)
+                            acc[bi][bj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                                ah[bi], bl[bj], acc[bi][bj], 0, 0, 0);
+#pragma unroll
+                    for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+                        for (int bj = 0; bj < 2; ++bj)
+                            acc[bi][bj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                                al[bi], bh[bj], acc[bi][bj], 0, 0, 0);
+                }
+            }
+        }
 
-            uint64_t amask = 0;  // certain accepts, bit 32 bi + 2 e + bj
+        // ---- epilogue.  C[i][j] of MFMA tile (bi, bj) is in lane
+        // j + 32h, element e, with i = 8 (e / 4) + 4 h + e % 4.
+        const bool diag_tile = self && cb == qbeg;  // inside: pairs p <= c only
+        float4 tb[2];
 #pragma unroll
-            for (int bi = 0; bi < 2; ++bi)
+        for (int bj = 0; bj < 2; ++bj)
+            tb[bj] = tB[64 * wc + 32 * bj + (lane & 31)];
+        bool need = sampled;
 #pragma unroll
-                for (int e = 0; e < 16; ++e) {
-                    const uint32_t il = 64 * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
-                    const float4 ta = tA[il];
-                    const uint32_t p = qbeg + il;
+        for (int bi = 0; bi < 4; ++bi)
 #pragma unroll
-                    for (int bj = 0; bj < 2; ++bj) {
-                        const uint32_t jl = 64 * wc + 32 * bj + (lane & 31);
-                        const uint32_t c = cb + jl;
-                        const bool valid = p < w.qend && c < w.cend && (!diag_tile || p <= c);
-                        const float G = acc[bi][bj][e];
-                        const bool a_ok = valid && fmaf(-2.f, G, ta.x + tb[bj].x) <= lo;
-                        const bool r_ok = fmaf(-2.f, G, ta.y + tb[bj].y) > hi;
-                        const bool inband = valid && !a_ok && !r_ok;
-                        amask |= (uint64_t)a_ok << (32 * bi + 2 * e + bj);
-                        uint32_t cls = 0;
-                        bool push = inband;
-                        if (DIAG) {
-                            cls = a_ok ? 1u : (inband ? 0u : 2u);
-                            push = inband || a_ok || (sampled && valid);
+            for (int e = 0; e < 16; ++e) {
+                const uint32_t il = 128 * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
+                const float ay = tA[il].y;
+#pragma unroll
+                for (int bj = 0; bj < 2; ++bj) {
+                    const bool keep = !(fmaf(-2.f, acc[bi][bj][e], ay + tb[bj].y) > hi);
+                    if (diag_tile)
+                        need = need || (keep && il <= 64 * wc + 32 * bj + (lane & 31));
+                    else
+                        need = need || keep;
+                }
+            }
+        if (!__builtin_amdgcn_ballot_w64(need))
+            continue;
+
+        // full pass, one 32-row block (bi) at a time
+        for (int bi = 0; bi < 4; ++bi) {
+            floatx16 cur[2];
+            switch (bi) {  // register arrays need constant indices
+            case 0:
+                cur[0] = acc[0][0];
+                cur[1] = acc[0][1];
+                break;
+            case 1:
+                cur[0] = acc[1][0];
+                cur[1] = acc[1][1];
+                break;
+            case 2:
+                cur[0] = acc[2][0];
+                cur[1] = acc[2][1];
+                break;
+            default:
+                cur[0] = acc[3][0];
+                cur[1] = acc[3][1];
+                break;
+            }
+            uint64_t amask = 0;  // certain accepts, bit 2 e + bj
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                const uint32_t il = 128 * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
+                const float4 ta = tA[il];
+                const uint32_t p = qbeg + il;
+#pragma unroll
+                for (int bj = 0; bj < 2; ++bj) {
+                    const uint32_t jl = 64 * wc + 32 * bj + (lane & 31);
+                    const uint32_t c = cb + jl;
+                    const bool valid = p < w.qend && c < w.cend && (!diag_tile || p <= c);
+                    const float G = cur[bj][e];
+                    const bool a_ok = valid && fmaf(-2.f, G, ta.x + tb[bj].x) <= lo;
+                    const bool r_ok = fmaf(-2.f, G, ta.y + tb[bj].y) > hi;
+                    const bool inband = valid && !a_ok && !r_ok;
+                    amask |= (uint64_t)a_ok << (2 * e + bj);
+                    uint32_t cls = 0;
+                    bool push = inband;
+                    if (DIAG) {
+                        cls = a_ok ? 1u : (inband ? 0u : 2u);
+                        push = inband || a_ok || (sampled && valid);
+                    }
+                    const uint64_t mb = __builtin_amdgcn_ballot_w64(push);
+                    if (mb) {
+                        const uint32_t k = (uint32_t)__popcll(mb);
+                        if (bfill + k > BHITS)
+                            bflush();
+                        if (push) {
+                            const uint32_t idx = bfill + (uint32_t)__popcll(mb & lt);
+                            bb[idx] = ((unsigned long long)jA[il] << 32) |
+                                      ((unsigned long long)cls << 30) | jB[jl];
+                            sb[idx] = fmaf(-2.f, G, ta.z + tb[bj].z);
                         }
-                        const uint64_t mb = __builtin_amdgcn_ballot_w64(push);
-                        if (mb) {
-                            const uint32_t k = (uint32_t)__popcll(mb);
-                            if (bfill + k > BHITS)
-                                bflush();
-                            if (push) {
-                                const uint32_t idx = bfill + (uint32_t)__popcll(mb & lt);
-                                bb[idx] = ((unsigned long long)jA[il] << 32) |
-                                          ((unsigned long long)cls << 30) | jB[jl];
-                                sb[idx] = fmaf(-2.f, G, ta.z + tb[bj].z);
-                            }
-                            bfill += k;
-                            b0fill += DIAG ? (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(inband))
-                                           : k;
-                        }
+                        bfill += k;
+                        b0fill += DIAG ? (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(inband))
+                                       : k;
                     }
                 }
+            }
             // certain accepts: both orders (the self pair once)
             while (true) {
                 const bool has = amask != 0;
@@ -599,8 +661,8 @@ __global__ __launch_bounds__(256, 2) void k_mfma_join(
                 if (has) {
                     const uint32_t k = (uint32_t)__builtin_ctzll(amask);
                     amask &= amask - 1;
-                    const uint32_t bi = k >> 5, e = (k >> 1) & 15u, bj = k & 1u;
-                    il = 64 * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
+                    const uint32_t e = k >> 1, bj = k & 1u;
+                    il = 128 * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
                     jl = 64 * wc + 32 * bj + (lane & 31);
                 }
                 const bool two = has && (qbeg + il) != (cb + jl);
@@ -911,7 +973,7 @@ uint64_t MfmaRunner::run(const Index &ix, JoinRunner &J, const double *xr, uint3
         MSJ_HIP(hipEventRecord(k0, s));
         if (nitems) {
 #define MSJ_MJ(SP, DG)                                                                          \
-    LAUNCH((k_mfma_join<SP, DG>), grid, 256, fh.p, SP ? fl.p : fh.p, KS, thr.p, porig.p, pjoin.p, \
+    LAUNCH((k_mfma_join<SP, DG>), grid, 512, fh.p, SP ? fl.p : fh.p, KS, thr.p, porig.p, pjoin.p, \
            items.p, smap, per_xcd, lo, hi, qbits, (unsigned long long *)J.pairs.p,              \
            (unsigned long long)J.pairs.cap, ctr.p, (unsigned long long *)band.p, bscore.p,      \
            (unsigned long long)band.cap, sample_every, (unsigned long long)budget)
