@@ -204,6 +204,13 @@ def main():
     ap.add_argument("--calibrate", action="store_true",
                     help="recompute eps from the seeded 10,000-query sample, print it, exit")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--single-pass", action="store_true",
+                    help="time the counting join itself (one join instead of 1 + warmup + "
+                         "steps; its kernel keeps work counters, a few percent slower)")
+    ap.add_argument("--rows", type=int, default=0,
+                    help="join only the first ROWS rows of the configuration's data")
+    ap.add_argument("--verbose", action="store_true",
+                    help="library progress lines (layers, join item ranges) on stderr")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-threads", type=int, default=16)
     args = ap.parse_args()
@@ -217,6 +224,10 @@ def main():
     cfg = CONFIGS[args.config]
     log("[bench] generating %s: n=%d d=%d" % (args.config, cfg["n"], cfg["d"]))
     x = make_data(cfg)
+    if args.rows:
+        x = np.ascontiguousarray(x[:args.rows])
+        cfg = dict(cfg, n=x.shape[0], workload=cfg["workload"] + " (first %d rows)" % x.shape[0])
+        log("[bench] joining the first %d rows" % x.shape[0])
     if args.eps is not None:
         eps = args.eps
     elif "eps" in cfg and not args.calibrate:
@@ -235,9 +246,13 @@ def main():
     # counting pass (untimed): algorithmic FP64 work of the distance kernel
     storage = msj.STORAGE_F32 if cfg.get("storage") == "f32" else msj.STORAGE_F64
     form = FORMS[args.form]
-    with msj.Context(output=msj.OUT_HOST, count_work=1, storage=storage, form=form) as cc:
+    verbose = 1 if args.verbose else 0
+    with msj.Context(output=msj.OUT_HOST, count_work=1, storage=storage, form=form,
+                     verbose=verbose) as cc:
         cc.load(x)
+        t_count = time.perf_counter()
         ref_run = cc.join(eps)
+        t_count = time.perf_counter() - t_count
     dims_eval = ref_run.stats["dims_evaluated"]
     gpu_rows = (ref_run.counts, ref_run.offsets, ref_run.nbrs, ref_run.node_order)
     log("[bench] counting pass: R=%d nodes=%d candidates=%d dims=%d" % (
@@ -251,19 +266,27 @@ def main():
 
     # msj_ctx_join returns after its stream has drained: the clock below
     # brackets exactly the K joins.
-    with msj.Context(output=msj.OUT_DEVICE, storage=storage, form=form) as ctx:
-        ctx.load(x)
-        for w in range(args.warmup):
-            r = ctx.join(eps, copy_arrays=False)
-            log("[bench] warmup %d: %.1f ms" % (w, r.stats["ms_total"]))
-        kms, stage = [], []
-        t0 = time.perf_counter()
-        for k in range(args.steps):
-            r = ctx.join(eps, copy_arrays=False)
-            kms.append(r.stats["ms_join_kernel"])
-            stage.append((r.stats["ms_index"], r.stats["ms_join"], r.stats["ms_output"]))
-        wall = time.perf_counter() - t0
-    ms_step = wall * 1e3 / args.steps
+    if args.single_pass:
+        r = ref_run
+        kms = [r.stats["ms_join_kernel"]]
+        stage = [(r.stats["ms_index"], r.stats["ms_join"], r.stats["ms_output"])]
+        ms_step = r.stats["ms_total"] - r.stats["ms_d2h"]
+        args.steps, args.warmup = 1, 0
+    else:
+        with msj.Context(output=msj.OUT_DEVICE, storage=storage, form=form,
+                         verbose=verbose) as ctx:
+            ctx.load(x)
+            for w in range(args.warmup):
+                r = ctx.join(eps, copy_arrays=False)
+                log("[bench] warmup %d: %.1f ms" % (w, r.stats["ms_total"]))
+            kms, stage = [], []
+            t0 = time.perf_counter()
+            for k in range(args.steps):
+                r = ctx.join(eps, copy_arrays=False)
+                kms.append(r.stats["ms_join_kernel"])
+                stage.append((r.stats["ms_index"], r.stats["ms_join"], r.stats["ms_output"]))
+            wall = time.perf_counter() - t0
+        ms_step = wall * 1e3 / args.steps
     st = r.stats
     C = st["num_calcs"]
     R = r.total_pairs
@@ -293,7 +316,9 @@ def main():
         "mean_neighbours": R / cfg["n"] - 1.0,
         "data": "synthetic (seeded, generated in-process)",
         "config": {"workload": cfg["workload"], "n": cfg["n"], "d": cfg["d"], "eps": eps,
-                   "eps_hex": eps_hex, "form": args.form, "parallelism": "none (1 GPU)"},
+                   "eps_hex": eps_hex, "form": args.form, "parallelism": "none (1 GPU)",
+                   "timing": "one counting join (work counters on)" if args.single_pass
+                   else "%d joins on resident data" % args.steps},
         "join_time_s": ms_step / 1e3, "pairs_per_s": R / (ms_step / 1e3), "total_pairs": R,
         "candidate_evals": C, "num_nodes": st["num_nodes"],
         "ms_index": float(np.mean([s[0] for s in stage])),

@@ -330,7 +330,7 @@ void do_join(msj_ctx *c, double eps, msj_result *out)
                         c->ev[2], c->ev[3], &launches, &kms, &st);
     } else {
         R = c->join.run_plain(c->ix, c->xr.p, n, d, eps, cap, s, c->ev[2], c->ev[3], &launches,
-                              &kms, c->prm.count_work ? &dims : nullptr);
+                              &kms, c->prm.count_work ? &dims : nullptr, c->prm.verbose != 0);
         st.form_used = MSJ_FORM_PLAIN;
     }
     st.dims_evaluated = dims;

@@ -215,9 +215,12 @@ struct JoinRunner {
     // copy xr; returns the number of pairs R.  Grows `pairs` and reruns when
     // R exceeds the capacity.  With dims_evaluated, a counting build of the
     // kernel also sums the FP64 dimensions the early-exit test consumed.
+    // progress != 0: the items run in 16 launches, each followed by a line
+    // on stderr (long joins stay observable).
     uint64_t run_plain(const Index &ix, const double *xr, uint32_t n, uint32_t d, double eps,
                        uint64_t cap_hint, hipStream_t s, hipEvent_t k0, hipEvent_t k1,
-                       uint64_t *launches, float *kernel_ms, uint64_t *dims_evaluated);
+                       uint64_t *launches, float *kernel_ms, uint64_t *dims_evaluated,
+                       bool progress = false);
     // Sorts the R pairs and writes counts[n], offsets[n+1], nbrs[R].
     void finish(uint32_t n, uint64_t R, uint32_t *counts, uint64_t *offsets, uint32_t *nbrs,
                 hipStream_t s);

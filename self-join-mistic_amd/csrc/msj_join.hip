@@ -30,6 +30,7 @@
 // sort then yields every row's neighbours ascending, the counts and the
 // CSR offsets.
 #include <algorithm>
+#include <cstdio>
 
 #include "msj_join_items.h"
 
@@ -480,7 +481,7 @@ void JoinRunner::plan(const Index &ix, uint32_t n, uint32_t d, hipStream_t s, bo
 uint64_t JoinRunner::run_plain(const Index &ix, const double *xr, uint32_t n, uint32_t d,
                                double eps, uint64_t cap_hint, hipStream_t s, hipEvent_t k0,
                                hipEvent_t k1, uint64_t *launches, float *kernel_ms,
-                               uint64_t *dims_evaluated)
+                               uint64_t *dims_evaluated, bool progress)
 {
     counter.ensure(1);
     if (pairs.cap < cap_hint)
@@ -496,9 +497,22 @@ uint64_t JoinRunner::run_plain(const Index &ix, const double *xr, uint32_t n, ui
         if (work)
             LAUNCH(k_zero<unsigned long long>, 1, 64, work, (size_t)1);
         MSJ_HIP(hipEventRecord(k0, s));
-        if (nitems) {
+        if (nitems && !progress) {
             dispatch_plain(view(ix, xr, n, d, eps), work, s);
             ++*launches;
+        } else if (nitems) {
+            const JobView V = view(ix, xr, n, d, eps);
+            const uint32_t parts = 16, step = (nitems + parts - 1) / parts;
+            for (uint32_t k = 0; k * step < nitems; ++k) {
+                JobView P = V;
+                P.items = V.items + (size_t)k * step;
+                P.nitems = std::min(step, nitems - k * step);
+                dispatch_plain(P, work, s);
+                ++*launches;
+                MSJ_HIP(hipStreamSynchronize(s));
+                std::fprintf(stderr, "[msj] join: %u/%u item ranges done\n", k + 1,
+                             (nitems + step - 1) / step);
+            }
         }
         MSJ_HIP(hipEventRecord(k1, s));
         unsigned long long R = 0;
