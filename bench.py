@@ -164,8 +164,12 @@ def cpu_baseline(x, eps, num_calcs, gpu_rows, seconds, threads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle  # noqa: WPS433 - the checker and CPU baseline
 
-    rate_guess = 1.5e8 * threads  # evaluations/s, refined by the measurement
+    # pilot: about 2e8 evaluations give this machine's rate at this shape
+    pilot = oracle.self_join(x, eps, sample_every=max(1, int(num_calcs // 2e8)),
+                             nthreads=threads)
+    rate_guess = pilot.evals_done / max(pilot.t_join, 1e-3)
     every = max(1, int(num_calcs // max(1.0, rate_guess * seconds)))
+    log("[bench] CPU pilot: %.3g evals/s; sampling every %d-th query" % (rate_guess, every))
     t0 = time.perf_counter()
     r = oracle.self_join(x, eps, sample_every=every, nthreads=threads)
     wall = time.perf_counter() - t0
