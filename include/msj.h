@@ -74,7 +74,9 @@ typedef struct msj_params {
     uint32_t storage;            /* msj_storage of the input                        */
     uint32_t output;             /* msj_output                                      */
     uint32_t verbose;            /* 1: per-layer progress on stdout                 */
-    uint32_t count_work;         /* 1: count FP64 dimensions evaluated (slower)     */
+    uint32_t count_work;         /* 1: plain form counts the FP64 dimensions
+                                    evaluated; MFMA forms verify their filter
+                                    classes in FP64 (both slower)                 */
     uint32_t reserved0;
     uint64_t page_pairs;         /* MSJ_OUT_HOST: neighbour ids per page of the
                                     pinned double buffer (0: 2^24)                 */

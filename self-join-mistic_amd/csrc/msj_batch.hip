@@ -5,25 +5,25 @@
 //   1. node nets: every job's index build, join order, FP64 rows and work
 //      items, spread over worker threads, each with its own stream (the
 //      builders' per-layer host round trips of different jobs overlap);
-//   2. join: one persistent kernel (grid = workgroups per CU x CUs) whose
-//      waves pop work items from a device queue (one atomic ticket per
-//      CHUNK items) holding the items of all jobs in job order; an item is
-//      run with the plain form's per-item code for its job's row width
-//      (msj_join_items.h), and its result keys go to that job's pair buffer.
-//      The reset of the counters and the persistent kernel are captured once
-//      as a hipGraph and replayed (the job table they read is refreshed by a
-//      copy before each replay);
+//   2. join: per row-width class one persistent kernel (grid = workgroups
+//      per CU x CUs) whose waves pop work items from the class's segment of a
+//      device queue (one atomic ticket per item) holding the items of all
+//      jobs; an item runs the plain form's per-item code for its job's row
+//      width (msj_join_items.h), and its result keys go to that job's pair
+//      buffer.  The reset of the counters and the class kernels (concurrent
+//      branches) are captured once as a hipGraph and replayed (the job table
+//      they read is refreshed by a copy before each replay);
 //   3. output: every job's pair sort and CSR arrays, on the worker streams.
 // Per-job results are those of msj_self_join (the same per-item code, the
 // same node nets).
 //
 // Autotuner.  The tile shape of the plain form is the work per item (query
 // rows x candidate columns, set by the lane evaluations per item) and the
-// number of resident workgroups per CU.  On the first run of a batch, for
-// each padded row width present, the jobs of that width are planned with
-// 2^16, 2^18 and 2^20 evaluations per item and run alone through the
-// persistent kernel; the fastest is kept.  Then the whole batch is timed
-// with 1, 2 and 4 workgroups per CU.  The choices are kept for later runs.
+// number of resident workgroups per CU.  For each class not tuned yet, the
+// class's jobs are planned with 2^16, 2^18 and 2^20 evaluations per item and
+// the class kernel is timed alone on a prefix of its queue; then, with the
+// best of these, 1, 2 and 4 workgroups per CU.  The choices are kept by the
+// batch for later runs.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -32,7 +32,6 @@
 #include <exception>
 #include <map>
 #include <memory>
-#include <mutex>
 #include <thread>
 #include <vector>
 

@@ -42,12 +42,14 @@
 // thus a linear copy into LDS and a linear, conflict-free read from it, and
 // the same array serves as operand A (query rows) and operand B (candidates).
 //
-// Work.  A workgroup (4 waves) owns 128 A rows of node X and walks up to
-// CBT tiles of 128 B rows of a node Y >= X of X's upper neighbour list
-// (inside X only tiles on or above the diagonal, pairs p <= c); each wave
-// computes a 64 x 64 block as 2 x 2 MFMA 32x32 tiles.  K slabs of 32 dims are
-// double-staged: the next slab's global loads are in flight while the
-// current slab runs on the matrix cores.
+// Work.  A tile is 256 A rows of node X against 256 B rows of a node Y >= X
+// of X's upper neighbour list (inside X only tiles on or above the diagonal,
+// pairs p <= c); a workgroup of 8 waves computes one tile at a time, wave
+// (wr, wc) a 128 x 64 block as 4 x 2 MFMA 32x32 tiles.  Tiles are grouped
+// into super-blocks of up to 8 x 8 tiles (the work items); consecutive tile
+// slots run on one XCD.  32-dimension slabs are copied global -> LDS
+// directly into a double buffer: the next slab's copy is in flight while the
+// current one runs on the matrix cores.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
