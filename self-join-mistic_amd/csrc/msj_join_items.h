@@ -164,7 +164,8 @@ __device__ inline void item_sym(const JobView &J, const JoinItem &w, double *qs,
             }
             c = ystart + (ci - ybase);
         }
-        const bool self_node = cvalid && Y == X;
+        const uint64_t vmask = wave_vote(cvalid);           // lanes with a candidate
+        const uint64_t smask = wave_vote(cvalid && Y == X);  // ... inside node X
         const double *crow = J.xr + (size_t)c * DS;  // c = 0 for idle lanes
         double cc[DS];
 #pragma unroll
@@ -177,16 +178,19 @@ __device__ inline void item_sym(const JobView &J, const JoinItem &w, double *qs,
             const uint32_t qi = (uint32_t)__builtin_ctzll(qmask);
             qmask &= qmask - 1ull;
             const uint32_t p = x0 + w.qbeg + qi;
-            bool alive = cvalid && (!self_node || p <= c);
-            if (!wave_vote(alive))
+            // undecided lanes as a wave mask; inside X only pairs p <= c
+            uint64_t am = vmask;
+            if (smask)
+                am &= ~smask | wave_vote(p <= c);
+            if (!am)
                 continue;
             const double *qrow = qs + qi * DS;  // wave-uniform LDS address
             double s = 0.0;
 #pragma unroll
             for (int l = 0; l < DS; l += 4) {
-                if (l > 0 && !wave_vote(alive))
+                if (l > 4 && !am)
                     break;
-                if (COUNT && alive && (uint32_t)l < J.d)
+                if (COUNT && ((am >> lane) & 1ull) && (uint32_t)l < J.d)
                     dims += min(4u, J.d - (uint32_t)l);
                 const double2 qa = *(const double2 *)(qrow + l);
                 const double2 qb = *(const double2 *)(qrow + l + 2);
@@ -201,9 +205,16 @@ __device__ inline void item_sym(const JobView &J, const JoinItem &w, double *qs,
                 double t3 = qb.y - cc[l + 3];
                 double q3 = t3 * t3;
                 s = s + q3;
-                alive = alive && !(s > J.eps2);
+                // partial sums only grow, so the test after the first chunk
+                // can wait for the second (the counting build tests every
+                // chunk: its dimension count follows the per-chunk exit)
+                if (COUNT || l > 0 || l + 4 >= DS)
+                    am &= wave_vote(!(s > J.eps2));
             }
-            H.emit(J, alive, p, c, alive ? (unsigned long long)J.perm[p] : 0ull, corig);
+            if (am) {
+                const bool acc = (am >> lane) & 1ull;
+                H.emit(J, acc, p, c, acc ? (unsigned long long)J.perm[p] : 0ull, corig);
+            }
         }
     }
 }
@@ -281,16 +292,18 @@ __device__ inline void item_hd(const JobView &J, const JoinItem &w, double *qs, 
             // continue out of it; a decided query or chunk is skipped by its vote
 #pragma unroll
             for (int q = 0; q < HQ; ++q) {
-                bool al = (alive >> q) & 1u;  // 0 for q >= qn
-                if (!wave_vote(al))
+                // undecided lanes of query q as a wave mask (0 for q >= qn)
+                uint64_t am = wave_vote((alive >> q) & 1u);
+                if (!am)
                     continue;
+                const uint64_t am0 = am;
                 const double *qrow = qs + q * HSL;  // wave-uniform LDS address
                 double s = sq[q];
 #pragma unroll
                 for (int k = 0; k < HSL; k += 4) {
-                    if (l0 + k >= ds || !wave_vote(al))
+                    if (l0 + k >= ds || !am)
                         continue;
-                    if (COUNT && al && l0 + k < J.d)
+                    if (COUNT && ((am >> lane) & 1ull) && l0 + k < J.d)
                         dims += min(4u, J.d - (l0 + k));
                     const double2 qa = *(const double2 *)(qrow + k);
                     const double2 qb = *(const double2 *)(qrow + k + 2);
@@ -305,10 +318,10 @@ __device__ inline void item_hd(const JobView &J, const JoinItem &w, double *qs, 
                     double t3 = qb.y - cc[k + 3];
                     double p3 = t3 * t3;
                     s = s + p3;
-                    al = al && !(s > J.eps2);
+                    am &= wave_vote(!(s > J.eps2));
                 }
                 sq[q] = s;
-                if (!al)
+                if (am != am0 && !((am >> lane) & 1ull))
                     alive &= ~(1u << q);
             }
         }
