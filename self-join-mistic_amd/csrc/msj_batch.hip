@@ -53,7 +53,7 @@ __device__ inline void batch_loop(const JobView *__restrict__ views,
                                   const uint64_t *__restrict__ job_off, uint32_t jb, uint32_t je,
                                   unsigned long long *__restrict__ queue, uint64_t limit)
 {
-    __shared__ double qsh[WPB][LDS_WAVE_BYTES / 8];
+    __shared__ __align__(16) double qsh[WPB][LDS_WAVE_BYTES / 8];
     __shared__ unsigned long long hsh[WPB][HITS];
     const uint32_t lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
     WaveHits H(hsh[wv], lane);
@@ -62,10 +62,13 @@ __device__ inline void batch_loop(const JobView *__restrict__ views,
     const uint64_t total = min(job_off[je] - base, limit);
     // every wave leaves the loop once the queue is drained
     while (true) {
-        unsigned long long t = 0;
+        unsigned long long tv = 0;
         if (lane == 0)
-            t = atomicAdd(queue, 1ull);
-        t = __shfl(t, 0, WAVE);
+            tv = atomicAdd(queue, 1ull);
+        // lane 0's ticket, in SGPRs: the job lookup and the job's view stay
+        // scalar
+        const uint64_t t = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(tv >> 32)) << 32) |
+                           __builtin_amdgcn_readfirstlane((uint32_t)tv);
         if (t >= total)
             break;
         const uint64_t i = base + t;
@@ -98,8 +101,8 @@ __global__ __launch_bounds__(256) void k_join_batch(const JobView *__restrict__ 
     batch_loop<DS>(views, job_off, jb, je, queue, limit);
 }
 
-// ds > 32: three waves per SIMD (the slab-major code fits 168 VGPRs).
-__global__ __launch_bounds__(256, 3) void k_join_batch_hd(const JobView *__restrict__ views,
+// ds > 32: four waves per SIMD, as the single-join kernel.
+__global__ __launch_bounds__(256, 4) void k_join_batch_hd(const JobView *__restrict__ views,
                                                           const uint64_t *__restrict__ job_off,
                                                           uint32_t jb, uint32_t je,
                                                           unsigned long long *__restrict__ queue,

@@ -76,7 +76,7 @@ __device__ inline void add_dims(uint64_t dims, unsigned long long *work)
 template <int DS, bool COUNT>
 __global__ __launch_bounds__(256) void k_join_sym(JobView J, unsigned long long *__restrict__ work)
 {
-    __shared__ double qsh[WPB][LDS_WAVE_BYTES / 8];
+    __shared__ __align__(16) double qsh[WPB][LDS_WAVE_BYTES / 8];
     __shared__ unsigned long long hsh[WPB][HITS];
     const uint32_t lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
     WaveHits H(hsh[wv], lane);
@@ -93,12 +93,12 @@ __global__ __launch_bounds__(256) void k_join_sym(JobView J, unsigned long long 
         add_dims(dims, work);
 }
 
-// The same for ds > 32 (slab-major variant).
+// The same for ds > 32 (slab-major variant); 128 VGPRs, four waves per SIMD.
 template <bool COUNT>
-__global__ __launch_bounds__(256, 3) void k_join_hd(JobView J,
+__global__ __launch_bounds__(256, 4) void k_join_hd(JobView J,
                                                     unsigned long long *__restrict__ work)
 {
-    __shared__ double qsh[WPB][HQ * HSL];
+    __shared__ __align__(16) double qsh[WPB][LDS_WAVE_BYTES / 8];  // query slab + survivor queue
     __shared__ unsigned long long hsh[WPB][HITS];
     const uint32_t lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
     WaveHits H(hsh[wv], lane);
