@@ -146,50 +146,89 @@ __global__ __launch_bounds__(256) void k_rp_distmat(const T *__restrict__ xs, ui
 }
 
 // utils.cu:29-41 per column: sequential mean, then sequential sum of squares.
-// The additions stay in row order; loads are issued eight rows ahead so the
-// chain is not serialised on memory latency.
-__global__ void k_col_sigma(const double *__restrict__ A, uint32_t rows, uint32_t cols,
-                            double *__restrict__ sigma)
+// One block per column.  The additions stay in row order, made by one
+// thread; the block's other threads stream the column through a
+// double-buffered LDS tile, so the next tile's global loads are in flight
+// while the current one is summed.
+constexpr uint32_t SIG_TILE = 2048;  // rows per LDS tile
+constexpr uint32_t SIG_BLOCK = 256;
+
+__global__ __launch_bounds__(SIG_BLOCK) void k_col_sigma(const double *__restrict__ A,
+                                                          uint32_t rows, uint32_t cols,
+                                                          double *__restrict__ sigma)
 {
-    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= cols)
-        return;
-    constexpr uint32_t U = 8;
+    constexpr uint32_t PER = SIG_TILE / SIG_BLOCK;
+    __shared__ double tile[2][SIG_TILE];
+    __shared__ double mean_sh;
+    const uint32_t c = blockIdx.x, tid = threadIdx.x;
+    const uint32_t ntiles = (rows + SIG_TILE - 1) / SIG_TILE;
     double mean = 0.0;
-    uint32_t r = 0;
-    for (; r + U <= rows; r += U) {
-        double v[U];
+    for (int pass = 0; pass < 2; ++pass) {
+        double acc = 0.0;  // thread 0
+        double v[PER];
+        auto fetch = [&](uint32_t t) {
 #pragma unroll
-        for (uint32_t k = 0; k < U; ++k)
-            v[k] = A[(size_t)(r + k) * cols + c];
+            for (uint32_t k = 0; k < PER; ++k) {
+                const uint32_t r = t * SIG_TILE + k * SIG_BLOCK + tid;
+                v[k] = r < rows ? A[(size_t)r * cols + c] : 0.0;
+            }
+        };
+        auto put = [&](uint32_t t) {
 #pragma unroll
-        for (uint32_t k = 0; k < U; ++k)
-            mean = mean + v[k];
-    }
-    for (; r < rows; ++r)
-        mean = mean + A[(size_t)r * cols + c];
-    mean = mean / (double)rows;
-    double dev = 0.0;
-    r = 0;
-    for (; r + U <= rows; r += U) {
-        double v[U];
-#pragma unroll
-        for (uint32_t k = 0; k < U; ++k)
-            v[k] = A[(size_t)(r + k) * cols + c];
-#pragma unroll
-        for (uint32_t k = 0; k < U; ++k) {
-            double t = v[k] - mean;
-            double sq = t * t;
-            dev = dev + sq;
+            for (uint32_t k = 0; k < PER; ++k)
+                tile[t & 1][k * SIG_BLOCK + tid] = v[k];
+        };
+        if (ntiles) {
+            fetch(0);
+            put(0);
         }
+        __syncthreads();
+        for (uint32_t t = 0; t < ntiles; ++t) {
+            if (t + 1 < ntiles)
+                fetch(t + 1);
+            if (tid == 0) {
+                const double *T = tile[t & 1];
+                const uint32_t m = min(SIG_TILE, rows - t * SIG_TILE);
+                uint32_t r = 0;
+                for (; r + 16 <= m; r += 16) {
+                    double w[16];
+#pragma unroll
+                    for (uint32_t k = 0; k < 16; ++k)
+                        w[k] = T[r + k];
+#pragma unroll
+                    for (uint32_t k = 0; k < 16; ++k) {
+                        if (pass == 0) {
+                            acc = acc + w[k];
+                        } else {
+                            double d = w[k] - mean;
+                            double sq = d * d;
+                            acc = acc + sq;
+                        }
+                    }
+                }
+                for (; r < m; ++r) {
+                    if (pass == 0) {
+                        acc = acc + T[r];
+                    } else {
+                        double d = T[r] - mean;
+                        double sq = d * d;
+                        acc = acc + sq;
+                    }
+                }
+            }
+            if (t + 1 < ntiles)
+                put(t + 1);
+            __syncthreads();
+        }
+        if (tid == 0) {
+            if (pass == 0)
+                mean_sh = acc / (double)rows;
+            else
+                sigma[c] = sqrt(acc / (double)rows);
+        }
+        __syncthreads();
+        mean = mean_sh;
     }
-    for (; r < rows; ++r) {
-        double t = A[(size_t)r * cols + c] - mean;
-        double sq = t * t;
-        dev = dev + sq;
-    }
-    dev = dev / (double)rows;
-    sigma[c] = sqrt(dev);
 }
 
 // utils.cu:105-118: max corner over the sample rows, starting from 0.  A
@@ -581,7 +620,7 @@ void build_impl(IndexBuilder &B, const T *xs, uint32_t n, uint32_t d, double eps
         if (S > 0)
             LAUNCH(k_rp_distmat<T>, grid_for((size_t)S * ntest, 256), 256, xs, n, d, S,
                    B.test_rp.p, ntest, B.distmat.p);
-        LAUNCH(k_col_sigma, 1, 64 * ((ntest + 63) / 64), B.distmat.p, S, ntest, B.sigma.p);
+        LAUNCH(k_col_sigma, ntest, SIG_BLOCK, B.distmat.p, S, ntest, B.sigma.p);
         LAUNCH(k_col_max<T>, d, 256, xs, n, d, S, B.maxc.p);
         MSJ_HIP(hipMemcpyAsync(h_sig.data(), B.sigma.p, ntest * sizeof(double),
                                hipMemcpyDeviceToHost, s));

@@ -53,6 +53,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "msj_internal.h"
@@ -356,6 +357,7 @@ __global__ __launch_bounds__(512, 1) void k_mfma_join(
     __shared__ uint4 sA[2][NP][SLAB];
     __shared__ uint4 sB[2][NP][SLAB];
     __shared__ float4 tA[TM], tB[TM];
+    __shared__ __align__(16) float tAy[TM];  // tA[.].y, four rows per 16-byte read
     __shared__ uint32_t oA[TM], oB[TM], jA[TM], jB[TM];
     __shared__ unsigned long long hsh[NW][MHITS];
     __shared__ unsigned long long bsh[NW][BHITS];
@@ -411,50 +413,128 @@ __global__ __launch_bounds__(512, 1) void k_mfma_join(
     // Tile slots in a grid-stride loop over virtual block ids; id v runs
     // slot (v % 8) * per_xcd + v / 8, so the slots of one XCD (the hardware
     // deals workgroups out round robin) are consecutive: one super-block.
+    // Slots outside their item (ragged node edges, below the diagonal) are
+    // skipped.
+    //
+    // Tiles are software-pipelined: during the last slab of a tile the
+    // workgroup already copies the next tile's first slab into the free LDS
+    // buffer and loads its row terms into registers, so neither waits on
+    // global memory when the next tile starts.
     const uint64_t vtotal = 8ull * per_xcd;
-    for (uint64_t v = blockIdx.x; v < vtotal; v += gridDim.x) {
+    struct Slot {
+        uint64_t it;
+        uint32_t ta, tbk;
+        bool in;
+    };
+    auto slot_of = [&](uint64_t v) {
+        Slot S;
         const uint64_t t = (v % 8u) * per_xcd + v / 8u;
-        bool have = t < sm.slot_base[SBS];
+        S.in = t < sm.slot_base[SBS];
         uint32_t cls = 0;
-        while (have && cls + 1 < SBS && t >= sm.slot_base[cls + 1])
+        while (S.in && cls + 1 < SBS && t >= sm.slot_base[cls + 1])
             ++cls;
         const uint32_t sz = SBS - cls;
-        const uint64_t local = have ? t - sm.slot_base[cls] : 0;
-        const uint64_t it = sm.item_base[cls] + local / (sz * sz);
-        const uint32_t r = (uint32_t)(local % (sz * sz)), ta = r / sz, tbk = r % sz;
-        MItem w = {0, 0, 0, 0, 0};
-        if (have)
-            w = items[it];
-        const uint32_t qbeg = w.qbeg + ta * TM, cb = w.cbeg + tbk * TM;
-        have = have && qbeg < w.qend && cb < w.cend && !((w.self & 0x200u) && tbk < ta);
-        __syncthreads();  // the previous tile is done with tA, tB and the slabs
+        const uint64_t local = S.in ? t - sm.slot_base[cls] : 0;
+        S.it = sm.item_base[cls] + local / (sz * sz);
+        const uint32_t r = (uint32_t)(local % (sz * sz));
+        S.ta = r / sz;
+        S.tbk = r % sz;
+        return S;
+    };
+    auto valid = [&](const Slot &S, const MItem &w) {
+        return S.in && w.qbeg + S.ta * TM < w.qend && w.cbeg + S.tbk * TM < w.cend &&
+               !((w.self & 0x200u) && S.tbk < S.ta);
+    };
+    // slab copy into buffer `buf`: unit u = ((operand * NP + plane) * RB +
+    // block) * KSL + kstep is one wave-wide 16-byte-per-lane load straight
+    // into LDS (the lane layout of a fragment block is its memory layout);
+    // wave wv issues units wv, wv + 8, ...
+    auto copy_slab = [&](uint32_t rbA, uint32_t rbB, uint32_t k0, uint32_t buf) {
+#pragma unroll
+        for (uint32_t m = 0; m < UNITS / NW; ++m) {
+            const uint32_t u = wv + NW * m;
+            const uint32_t kk = u % KSL, blk = (u / KSL) % RB, pl = (u / (KSL * RB)) % NP;
+            const uint32_t op = u / (KSL * RB * NP);
+            const uint4 *src =
+                (pl ? fl : fh) + ((size_t)((op ? rbB : rbA) + blk) * KS + k0 + kk) * WAVE + lane;
+            uint4 *dst = (op ? &sB[buf][pl][0] : &sA[buf][pl][0]) + (blk * KSL + kk) * WAVE;
+            __builtin_amdgcn_global_load_lds((const void *)src,
+                                             (void __attribute__((address_space(3))) *)dst, 16,
+                                             0, 0);
+        }
+    };
+    // row terms: threads 0..TM-1 the A rows, TM..2TM-1 the B rows; rows
+    // outside the item (padding, the next node) get +inf terms
+    float4 t4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    uint32_t o4 = 0, j4 = 0;
+    auto fetch_terms = [&](const Slot &S, const MItem &w) {
+        const bool isb = tid >= TM;
+        const uint32_t k = isb ? tid - TM : tid;
+        const uint32_t r_ = (isb ? w.cbeg + S.tbk * TM : w.qbeg + S.ta * TM) + k;
+        t4 = thr[r_];
+        if (r_ >= (isb ? w.cend : w.qend))
+            t4 = make_float4(INF, INF, INF, INF);
+        o4 = porig[r_];
+        j4 = pjoin[r_];
+    };
+
+    // the first tile
+    uint64_t v = blockIdx.x;
+    Slot cs = {0, 0, 0, false};
+    MItem w = {0, 0, 0, 0, 0};
+    bool have = false;
+    for (; v < vtotal; v += gridDim.x) {
+        cs = slot_of(v);
+        if (!cs.in)
+            continue;
+        w = items[cs.it];
+        if (valid(cs, w)) {
+            have = true;
+            break;
+        }
+    }
+    uint32_t buf = 0;  // LDS buffer of the current slab
+    if (have) {
+        fetch_terms(cs, w);
+        copy_slab((w.qbeg + cs.ta * TM) / 32, (w.cbeg + cs.tbk * TM) / 32, 0, 0);
+    }
+    while (have) {
+        __syncthreads();  // the previous tile is done with tA .. jB
+        {
+            const bool isb = tid >= TM;
+            const uint32_t k = isb ? tid - TM : tid;
+            (isb ? tB : tA)[k] = t4;
+            (isb ? oB : oA)[k] = o4;
+            (isb ? jB : jA)[k] = j4;
+            if (!isb)
+                tAy[k] = t4.y;
+        }
         // the band counter is read every 32nd slot of a workgroup: one global
         // address polled by every workgroup on every tile is a hot spot
         if (tid == 0 && band_budget != 0 && ((v / gridDim.x) & 31u) == 0)
             s_stop = __hip_atomic_load(&ctr[7], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >
                      band_budget;
-        if (have) {  // threads 0..TM-1 stage A rows, TM..2TM-1 B rows
-            const bool isb = tid >= TM;
-            const uint32_t k = isb ? tid - TM : tid;
-            const uint32_t r_ = (isb ? cb : qbeg) + k;
-            float4 t4 = thr[r_];
-            if (r_ >= (isb ? w.cend : w.qend))
-                t4 = make_float4(INF, INF, INF, INF);
-            (isb ? tB : tA)[k] = t4;
-            (isb ? oB : oA)[k] = porig[r_];
-            (isb ? jB : jA)[k] = pjoin[r_];
-        }
-        __syncthreads();
-        if (!have || s_stop)
-            continue;  // uniform over the workgroup
+        // a row of the tile with NaN terms (non-finite or huge values) sends
+        // every 32-row block to the full pass
+        const bool tile_nan = __syncthreads_or(t4.x != t4.x || t4.y != t4.y) != 0;
+        if (s_stop)
+            break;  // uniform over the workgroup
+        const uint64_t it = cs.it;
+        const uint32_t qbeg = w.qbeg + cs.ta * TM, cb = w.cbeg + cs.tbk * TM;
         const bool sampled = DIAG && (it % sample_every == 0);
         const bool self = (w.self & 0x100u) != 0;
         const uint32_t rbA = qbeg / 32, rbB = cb / 32;
+        // the next slot's item, loaded now, tested before the last slab
+        uint64_t vn = v + gridDim.x;
+        Slot ns = {0, 0, 0, false};
+        MItem nw = {0, 0, 0, 0, 0};
+        if (vn < vtotal) {
+            ns = slot_of(vn);
+            if (ns.in)
+                nw = items[ns.it];
+        }
+        bool nhave = false;
 
-        // slab copy: unit u = ((operand * NP + plane) * RB + block) * KSL + kstep
-        // is one wave-wide 16-byte-per-lane load straight into LDS (the lane
-        // layout of a fragment block is its memory layout); wave wv issues
-        // units wv, wv + 8, ...
         floatx16 acc[4][2];
 #pragma unroll
         for (int bi = 0; bi < 4; ++bi)
@@ -463,24 +543,6 @@ __global__ __launch_bounds__(512, 1) void k_mfma_join(
 #pragma unroll
                 for (int e = 0; e < 16; ++e)
                     acc[bi][bj][e] = 0.f;
-        // slab copy into buffer `buf`: unit u = ((operand * NP + plane) * RB +
-        // block) * KSL + kstep is one wave-wide 16-byte-per-lane load straight
-        // into LDS; wave wv issues units wv, wv + 8, ...
-        auto copy_slab = [&](uint32_t k0, uint32_t buf) {
-#pragma unroll
-            for (uint32_t m = 0; m < UNITS / NW; ++m) {
-                const uint32_t u = wv + NW * m;
-                const uint32_t kk = u % KSL, blk = (u / KSL) % RB, pl = (u / (KSL * RB)) % NP;
-                const uint32_t op = u / (KSL * RB * NP);
-                const uint4 *src = (pl ? fl : fh) +
-                                   ((size_t)((op ? rbB : rbA) + blk) * KS + k0 + kk) * WAVE +
-                                   lane;
-                uint4 *dst = (op ? &sB[buf][pl][0] : &sA[buf][pl][0]) + (blk * KSL + kk) * WAVE;
-                __builtin_amdgcn_global_load_lds((const void *)src,
-                                                 (void __attribute__((address_space(3))) *)dst,
-                                                 16, 0, 0);
-            }
-        };
         // LDS byte addresses of this lane's first A and B fragments (k-step 0,
         // buffer 0); fragment (block, k-step) lies (block * KSL + kstep) KiB
         // further, the low plane SLAB * 16 bytes, buffer 1 NP * SLAB * 16.
@@ -489,15 +551,34 @@ __global__ __launch_bounds__(512, 1) void k_mfma_join(
         const uint32_t ldsB = (uint32_t)(uintptr_t)(
             (__attribute__((address_space(3))) uint4 *)&sB[0][0][(2 * wc * KSL) * WAVE + lane]);
         static_assert(KSL == 2 && SLAB * 16 == 16384, "fragment read offsets below");
-        copy_slab(0, 0);
-        for (uint32_t k0 = 0, sl = 0; k0 < KS; k0 += KSL, ++sl) {
-            const uint32_t buf = sl & 1u;
+        for (uint32_t k0 = 0; k0 < KS; k0 += KSL, buf ^= 1u) {
             // this wave's copies of the slab have landed; after the barrier,
             // everyone's have, and nobody still reads the other buffer
             asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
             __syncthreads();
-            if (k0 + KSL < KS)
-                copy_slab(k0 + KSL, buf ^ 1u);  // in flight during this slab's MFMAs
+            if (k0 + KSL < KS) {
+                copy_slab(rbA, rbB, k0 + KSL, buf ^ 1u);  // in flight during this slab's MFMAs
+            } else {
+                // last slab: find the next tile (a skipped slot costs one more
+                // item load) and start its first slab and row terms
+                while (vn < vtotal) {
+                    if (valid(ns, nw)) {
+                        nhave = true;
+                        break;
+                    }
+                    vn += gridDim.x;
+                    if (vn >= vtotal)
+                        break;
+                    ns = slot_of(vn);
+                    if (ns.in)
+                        nw = items[ns.it];
+                }
+                if (nhave) {
+                    copy_slab((nw.qbeg + ns.ta * TM) / 32, (nw.cbeg + ns.tbk * TM) / 32, 0,
+                              buf ^ 1u);
+                    fetch_terms(ns, nw);
+                }
+            }
 #pragma unroll
             for (uint32_t kk = 0; kk < KSL; ++kk) {
                 // the k-step's fragments in one block, waited for inside it
@@ -574,95 +655,125 @@ __global__ __launch_bounds__(512, 1) void k_mfma_join(
 #pragma unroll
         for (int bj = 0; bj < 2; ++bj)
             tb[bj] = tB[64 * wc + 32 * bj + (lane & 31)];
-        bool need = sampled;
+        // fast pass: per 32-row block the smallest reject score of its pairs
+        // (in a diagonal tile of pairs p <= c); a block whose smallest score
+        // exceeds hi holds certain rejects only.  Scores of padding rows and
+        // of rows outside the item are +inf; NaN scores (NaN terms) are
+        // caught by tile_nan.
+        float mn[4];
+        auto fast = [&](auto dg) {
+            constexpr bool DT = decltype(dg)::value;
 #pragma unroll
-        for (int bi = 0; bi < 4; ++bi)
+            for (int bi = 0; bi < 4; ++bi) {
+                float m = INF;
 #pragma unroll
-            for (int e = 0; e < 16; ++e) {
-                const uint32_t il = 128 * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
-                const float ay = tA[il].y;
+                for (int eg = 0; eg < 4; ++eg) {
+                    const uint32_t i0 = 128 * wr + 32 * bi + 8 * eg + 4 * h;
+                    const float4 ay4 = *(const float4 *)&tAy[i0];
+                    const float ay[4] = {ay4.x, ay4.y, ay4.z, ay4.w};
 #pragma unroll
-                for (int bj = 0; bj < 2; ++bj) {
-                    const bool keep = !(fmaf(-2.f, acc[bi][bj][e], ay + tb[bj].y) > hi);
-                    if (diag_tile)
-                        need = need || (keep && il <= 64 * wc + 32 * bj + (lane & 31));
-                    else
-                        need = need || keep;
+                    for (int ei = 0; ei < 4; ++ei)
+#pragma unroll
+                        for (int bj = 0; bj < 2; ++bj) {
+                            float sc = fmaf(-2.f, acc[bi][bj][4 * eg + ei], ay[ei] + tb[bj].y);
+                            if (DT && i0 + ei > 64 * wc + 32 * bj + (lane & 31))
+                                sc = INF;
+                            m = fminf(m, sc);
+                        }
                 }
+                mn[bi] = m;
             }
-        if (!__builtin_amdgcn_ballot_w64(need))
-            continue;
+        };
+        if (diag_tile)
+            fast(std::true_type());
+        else
+            fast(std::false_type());
 
-        // full pass, one 32-row block (bi) at a time
+        // full pass, for the 32-row blocks (bi) that need it: the block's
+        // 32 pairs per lane (slot 2 e + bj) classified into lane bit masks of
+        // certain accepts and band entries (with DIAG also the verified
+        // classes)
+        bool cvalid[2];
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+            cvalid[bj] = cb + 64 * wc + 32 * bj + (lane & 31) < w.cend;
+        uint32_t amask[4] = {0u, 0u, 0u, 0u}, bmask[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
         for (int bi = 0; bi < 4; ++bi) {
-            floatx16 cur[2];
-            switch (bi) {  // register arrays need constant indices
-            case 0:
-                cur[0] = acc[0][0];
-                cur[1] = acc[0][1];
-                break;
-            case 1:
-                cur[0] = acc[1][0];
-                cur[1] = acc[1][1];
-                break;
-            case 2:
-                cur[0] = acc[2][0];
-                cur[1] = acc[2][1];
-                break;
-            default:
-                cur[0] = acc[3][0];
-                cur[1] = acc[3][1];
-                break;
-            }
-            uint64_t amask = 0;  // certain accepts, bit 2 e + bj
+            const bool need = sampled || tile_nan || !(mn[bi] > hi);
+            if (!__builtin_amdgcn_ballot_w64(need))
+                continue;
 #pragma unroll
             for (int e = 0; e < 16; ++e) {
                 const uint32_t il = 128 * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
                 const float4 ta = tA[il];
                 const uint32_t p = qbeg + il;
+                const bool pv = p < w.qend;
 #pragma unroll
                 for (int bj = 0; bj < 2; ++bj) {
-                    const uint32_t jl = 64 * wc + 32 * bj + (lane & 31);
-                    const uint32_t c = cb + jl;
-                    const bool valid = p < w.qend && c < w.cend && (!diag_tile || p <= c);
-                    const float G = cur[bj][e];
+                    const uint32_t c = cb + 64 * wc + 32 * bj + (lane & 31);
+                    const bool valid = pv && cvalid[bj] && (!diag_tile || p <= c);
+                    const float G = acc[bi][bj][e];
                     const bool a_ok = valid && fmaf(-2.f, G, ta.x + tb[bj].x) <= lo;
                     const bool r_ok = fmaf(-2.f, G, ta.y + tb[bj].y) > hi;
                     const bool inband = valid && !a_ok && !r_ok;
-                    amask |= (uint64_t)a_ok << (2 * e + bj);
-                    uint32_t cls = 0;
-                    bool push = inband;
-                    if (DIAG) {
-                        cls = a_ok ? 1u : (inband ? 0u : 2u);
-                        push = inband || a_ok || (sampled && valid);
-                    }
-                    const uint64_t mb = __builtin_amdgcn_ballot_w64(push);
-                    if (mb) {
-                        const uint32_t k = (uint32_t)__popcll(mb);
-                        if (bfill + k > BHITS)
-                            bflush();
-                        if (push) {
-                            const uint32_t idx = bfill + (uint32_t)__popcll(mb & lt);
-                            bb[idx] = ((unsigned long long)jA[il] << 32) |
-                                      ((unsigned long long)cls << 30) | jB[jl];
-                            sb[idx] = fmaf(-2.f, G, ta.z + tb[bj].z);
-                        }
-                        bfill += k;
-                        b0fill += DIAG ? (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(inband))
-                                       : k;
-                    }
+                    const bool push = DIAG ? (inband || a_ok || (sampled && valid)) : inband;
+                    amask[bi] |= (uint32_t)a_ok << (2 * e + bj);
+                    bmask[bi] |= (uint32_t)push << (2 * e + bj);
                 }
             }
-            // certain accepts: both orders (the self pair once)
+        }
+        // band entries (rare): staged slot by slot, with their scores
+#pragma unroll
+        for (int bi = 0; bi < 4; ++bi) {
+            if (!__builtin_amdgcn_ballot_w64(bmask[bi] != 0))
+                continue;
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                const uint32_t il = 128 * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
+#pragma unroll
+                for (int bj = 0; bj < 2; ++bj) {
+                    const bool push = (bmask[bi] >> (2 * e + bj)) & 1u;
+                    const uint64_t mb = __builtin_amdgcn_ballot_w64(push);
+                    if (!mb)
+                        continue;
+                    const uint32_t jl = 64 * wc + 32 * bj + (lane & 31);
+                    const uint32_t k = (uint32_t)__popcll(mb);
+                    if (bfill + k > BHITS)
+                        bflush();
+                    const bool a_ok = (amask[bi] >> (2 * e + bj)) & 1u;
+                    bool inband = push;
+                    if (push) {
+                        const float4 ta = tA[il];
+                        const float G = acc[bi][bj][e];
+                        const uint32_t p = qbeg + il, c = cb + jl;
+                        const bool valid = p < w.qend && c < w.cend && (!diag_tile || p <= c);
+                        const bool r_ok = fmaf(-2.f, G, ta.y + tb[bj].y) > hi;
+                        inband = valid && !a_ok && !r_ok;
+                        const uint32_t cls = DIAG ? (a_ok ? 1u : (inband ? 0u : 2u)) : 0u;
+                        const uint32_t idx = bfill + (uint32_t)__popcll(mb & lt);
+                        bb[idx] = ((unsigned long long)jA[il] << 32) |
+                                  ((unsigned long long)cls << 30) | jB[jl];
+                        sb[idx] = fmaf(-2.f, G, ta.z + tb[bj].z);
+                    }
+                    bfill += k;
+                    b0fill += DIAG ? (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(inband)) : k;
+                }
+            }
+        }
+        // certain accepts: both orders (the self pair once)
+#pragma unroll
+        for (int bi = 0; bi < 4; ++bi) {
+            uint32_t am = amask[bi];
             while (true) {
-                const bool has = amask != 0;
+                const bool has = am != 0;
                 const uint64_t m1 = __builtin_amdgcn_ballot_w64(has);
                 if (!m1)
                     break;
                 uint32_t il = 0, jl = 0;
                 if (has) {
-                    const uint32_t k = (uint32_t)__builtin_ctzll(amask);
-                    amask &= amask - 1;
+                    const uint32_t k = (uint32_t)__builtin_ctz(am);
+                    am &= am - 1;
                     const uint32_t e = k >> 1, bj = k & 1u;
                     il = 128 * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
                     jl = 64 * wc + 32 * bj + (lane & 31);
@@ -682,7 +793,13 @@ __global__ __launch_bounds__(512, 1) void k_mfma_join(
                 fill += tot;
             }
         }
+        v = vn;
+        cs = ns;
+        w = nw;
+        have = nhave;
     }
+    // no copy into this workgroup's LDS may still be in flight at its end
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     if (fill)
         flush();
     if (bfill)
