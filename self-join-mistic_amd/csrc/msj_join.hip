@@ -209,7 +209,9 @@ __global__ __launch_bounds__(256) void k_item_fill(const uint32_t *__restrict__ 
     uint32_t qper, nq, cper, nc;
     item_shape(sz, L, qmax, epi, &qper, &nq, &cper, &nc);
     const uint32_t local = k - itoff[x];
-    const uint32_t qi = local / nc, ck = local % nc;
+    // candidate-chunk-major: the query groups of one candidate chunk are
+    // consecutive items (run side by side, they share the chunk in cache)
+    const uint32_t ck = local / nq, qi = local % nq;
     JoinItem w;
     w.x = x;
     w.qbeg = qi * qper;
