@@ -6,9 +6,10 @@
 //      items, spread over worker threads, each with its own stream (the
 //      builders' per-layer host round trips of different jobs overlap);
 //   2. join: per row-width class one persistent kernel (grid = workgroups
-//      per CU x CUs) whose waves pop work items from the class's segment of a
-//      device queue (one atomic ticket per item) holding the items of all
-//      jobs; an item runs the plain form's per-item code for its job's row
+//      per CU x CUs) whose waves pop work items from the class's part of a
+//      device queue (one atomic ticket per item; the part is cut into one
+//      segment per XCD) holding the items of all jobs; an item runs the
+//      plain form's per-item code for its job's row
 //      width (msj_join_items.h), and its result keys go to that job's pair
 //      buffer.  The reset of the counters and the class kernels (concurrent
 //      branches) are captured once as a hipGraph and replayed (the job table
@@ -48,6 +49,17 @@ namespace {
 // queue; ticket t is item job_off[jb] + t of the table, i.e. an item of one
 // of the class's jobs [jb, je).  At most `limit` tickets are served (the
 // autotuner times prefixes of the queue).
+constexpr uint32_t NSEG = 8;  // queue segments per class: one per XCD
+
+// The XCD (accelerator complex die) this wave runs on; any value in
+// [0, NSEG) is correct for the queue logic, the right one keeps a segment
+// under one L2.
+__device__ inline uint32_t xcd_id()
+{
+    // s_getreg of HW_REG_XCC_ID (id 20), bits [3:0]
+    return (uint32_t)__builtin_amdgcn_s_getreg(20 | (0 << 6) | (3 << 11)) % NSEG;
+}
+
 template <int DS>
 __device__ inline void batch_loop(const JobView *__restrict__ views,
                                   const uint64_t *__restrict__ job_off, uint32_t jb, uint32_t je,
@@ -60,17 +72,27 @@ __device__ inline void batch_loop(const JobView *__restrict__ views,
     uint64_t dims = 0;
     const uint64_t base = job_off[jb];
     const uint64_t total = min(job_off[je] - base, limit);
-    // every wave leaves the loop once the queue is drained
-    while (true) {
+    // The queue is split into NSEG contiguous segments, one per XCD: a wave
+    // drains its own XCD's segment first, so consecutive items (the query
+    // groups of one candidate chunk) run at the same time under one L2; then
+    // it helps with the others.  Every wave leaves once all are drained.
+    const uint32_t home = xcd_id();
+    uint32_t k = 0;
+    while (k < NSEG) {
+        const uint32_t x = (home + k) % NSEG;
+        const uint64_t slo = total * x / NSEG, shi = total * (x + 1) / NSEG;
         unsigned long long tv = 0;
         if (lane == 0)
-            tv = atomicAdd(queue, 1ull);
+            tv = atomicAdd(&queue[x], 1ull);
         // lane 0's ticket, in SGPRs: the job lookup and the job's view stay
         // scalar
-        const uint64_t t = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(tv >> 32)) << 32) |
-                           __builtin_amdgcn_readfirstlane((uint32_t)tv);
-        if (t >= total)
-            break;
+        const uint64_t tk = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(tv >> 32)) << 32) |
+                            __builtin_amdgcn_readfirstlane((uint32_t)tv);
+        if (tk >= shi - slo) {
+            ++k;  // segment drained
+            continue;
+        }
+        const uint64_t t = slo + tk;
         const uint64_t i = base + t;
         uint32_t lo = jb, hi = je;  // job with job_off[j] <= i < job_off[j+1]
         while (hi - lo > 1) {
@@ -294,15 +316,15 @@ struct BatchEngine {
     {
         hipStream_t s = stream;
         const uint32_t nj = (uint32_t)jobs.size();
-        LAUNCH(k_batch_reset, grid_for(std::max<uint32_t>(nj, MAX_CLASSES), 64), 64, views.p, nj,
-               queues.p, MAX_CLASSES);
+        LAUNCH(k_batch_reset, grid_for(std::max<uint32_t>(nj, MAX_CLASSES * NSEG), 64), 64, views.p,
+               nj, queues.p, MAX_CLASSES * NSEG);
         MSJ_HIP(hipEventRecord(fork, s));
         for (uint32_t c = 0; c < cls.size(); ++c) {
             if (only >= 0 && (int)c != only)
                 continue;
             MSJ_HIP(hipStreamWaitEvent(cstream[c], fork, 0));
-            launch_class(cls[c], grid_for_wg(cls[c].wg), views.p, job_off.p, queues.p + c, limit,
-                         cstream[c]);
+            launch_class(cls[c], grid_for_wg(cls[c].wg), views.p, job_off.p,
+                         queues.p + c * NSEG, limit, cstream[c]);
             MSJ_HIP(hipEventRecord(join_ev[c], cstream[c]));
             MSJ_HIP(hipStreamWaitEvent(s, join_ev[c], 0));
         }
@@ -458,7 +480,7 @@ BatchEngine *batch_create(const msj_params &p)
         b->workers.emplace_back(new Worker());
         MSJ_HIP(hipStreamCreate(&b->workers.back()->s));
     }
-    b->queues.ensure(MAX_CLASSES);
+    b->queues.ensure(MAX_CLASSES * NSEG);
     return b.release();
 }
 
