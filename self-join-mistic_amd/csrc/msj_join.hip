@@ -98,7 +98,7 @@ template <bool COUNT>
 __global__ __launch_bounds__(256, 4) void k_join_hd(JobView J,
                                                     unsigned long long *__restrict__ work)
 {
-    __shared__ __align__(16) double qsh[WPB][LDS_WAVE_BYTES / 8];  // query slab + survivor queue
+    __shared__ __align__(16) double qsh[WPB][HQ * HSL];
     __shared__ unsigned long long hsh[WPB][HITS];
     const uint32_t lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
     WaveHits H(hsh[wv], lane);

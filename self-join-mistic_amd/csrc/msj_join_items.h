@@ -33,8 +33,6 @@ constexpr uint32_t LDS_WAVE_BYTES = 8192;  // staged query rows per wave
 constexpr uint32_t HITS = 256;             // per-wave LDS buffer of result keys
 constexpr int HSL = 16;                    // high-d kernel: dimensions per slab
 constexpr int HQ = 16;                     // high-d kernel: queries per item
-constexpr uint32_t HQCAP = 256;            // high-d kernel: survivor queue entries per wave
-constexpr uint32_t HPUSH = 20;             // ... queue a query's pairs once <= HPUSH lanes remain
 
 // Queries per work item for a row width DS (in-register dimensions): as many
 // FP64 rows as fit LDS_WAVE_BYTES, at most 64.
@@ -221,96 +219,6 @@ __device__ inline void item_sym(const JobView &J, const JoinItem &w, double *qs,
     }
 }
 
-// A pair of the high-d kernel's survivor queue: query q of the item, the
-// dimension k it resumes at, candidate row c, partial sum s after k dims.
-struct HdPair {
-    uint32_t qk;  // q | k << 8
-    uint32_t c;
-    double s;
-};
-static_assert(HQ * HSL * 8 + HQCAP * sizeof(HdPair) <= LDS_WAVE_BYTES, "high-d LDS slice");
-
-// Runs the n queued pairs to their decisions.  Every lane advances its own
-// pair by one slab (HSL dims, fewer at the row's end) per step: the slab's
-// coordinates of both rows are loaded at once, its chunks run in the slab
-// loop's order from the queued partial sum, and the test is made once at
-// the slab's end (partial sums only grow, so a test after every chunk
-// decides the same; the counting build tests every chunk to count the
-// dimensions the per-chunk exit consumes).  A lane whose pair is decided
-// takes the next queued pair.
-template <bool COUNT>
-__device__ inline void hd_drain(const JobView &J, uint32_t q0, const HdPair *dq, uint32_t n,
-                                WaveHits &H, uint64_t &dims)
-{
-    const uint32_t ds = J.ds;
-    wave_sync();  // the queue writes of every lane are visible
-    uint32_t head = 0;
-    bool act = false;
-    uint32_t p = 0, c = 0, k = 0;
-    double s = 0.0;
-    while (true) {
-        const uint64_t idle = wave_vote(!act);
-        if (idle && head < n) {
-            const uint32_t e = head + (uint32_t)__popcll(idle & H.lt);
-            if (!act && e < n) {
-                const HdPair E = dq[e];
-                p = q0 + (E.qk & 0xffu);
-                k = E.qk >> 8;
-                c = E.c;
-                s = E.s;
-                act = true;
-            }
-            head = min(n, head + (uint32_t)__popcll(idle));
-        }
-        if (!wave_vote(act))
-            break;
-        bool acc = false;
-        if (act) {
-            const double *qr = J.xr + (size_t)p * ds + k;
-            const double *cr = J.xr + (size_t)c * ds + k;
-            double2 qv[HSL / 2], cv[HSL / 2];
-#pragma unroll
-            for (int j = 0; j < HSL / 2; ++j) {
-                if (k + 2 * j < ds) {
-                    qv[j] = *(const double2 *)(qr + 2 * j);
-                    cv[j] = *(const double2 *)(cr + 2 * j);
-                }
-            }
-            bool live = true;  // counting build only: no chunk test failed yet
-#pragma unroll
-            for (int j = 0; j < HSL / 2; j += 2) {
-                if (k + 2 * j < ds) {
-                    if (COUNT && live && k + 2 * j < J.d)
-                        dims += min(4u, J.d - (k + 2 * j));
-                    double t0 = qv[j].x - cv[j].x;
-                    s = fma(t0, t0, s);
-                    double t1 = qv[j].y - cv[j].y;
-                    double u1 = t1 * t1;
-                    s = s + u1;
-                    double t2 = qv[j + 1].x - cv[j + 1].x;
-                    double u2 = t2 * t2;
-                    s = s + u2;
-                    double t3 = qv[j + 1].y - cv[j + 1].y;
-                    double u3 = t3 * t3;
-                    s = s + u3;
-                    if (COUNT)
-                        live = live && !(s > J.eps2);
-                }
-            }
-            k += HSL;
-            const bool rej = s > J.eps2;
-            if (rej || k >= ds) {
-                act = false;
-                acc = !rej;
-            }
-        }
-        if (wave_vote(acc))
-            H.emit(J, acc, p, c, acc ? (unsigned long long)J.perm[p] : 0ull,
-                   acc ? (unsigned long long)J.perm[c] : 0ull);
-    }
-    wave_sync();  // every lane has read its entries before the queue is reused
-}
-
 // High-dimensional rows (ds > 32).  Slab-major order: the wave holds the
 // 64 candidates' next HSL dimensions in VGPRs (each loaded once) and
 // advances the partial sums of all of the item's queries (up to HQ, one FP64
@@ -319,19 +227,10 @@ __device__ inline void hd_drain(const JobView &J, uint32_t q0, const HdPair *dq,
 // staged in the wave's LDS slice and read as a broadcast.  Decisions are
 // those of the per-pair sequence: the chunks of every pair run in order,
 // each followed by the s > eps^2 test.
-//
-// Survivor queue.  Most pairs are decided within the first slabs; a few
-// need many more.  Once a query has at most HPUSH undecided lanes after a
-// slab, its pairs leave the slab loop for the wave's LDS queue (behind the
-// query slab in the LDS slice) with their partial sums; hd_drain finishes
-// them a full wave at a time, so the slab loop stops paying for the
-// stragglers' idle lanes.
 template <bool COUNT>
 __device__ inline void item_hd(const JobView &J, const JoinItem &w, double *qs, WaveHits &H,
                                uint64_t &dims)
 {
-    HdPair *dq = (HdPair *)(qs + HQ * HSL);
-    uint32_t qcnt = 0;  // wave-uniform
     const uint32_t lane = H.lane;
     const uint32_t ds = J.ds;
     const uint32_t X = w.x;
@@ -430,18 +329,6 @@ __device__ inline void item_hd(const JobView &J, const JoinItem &w, double *qs, 
                 sq[q] = s;
                 if (am != am0 && !((am >> lane) & 1ull))
                     alive &= ~(1u << q);
-                // few undecided lanes left: queue them
-                if (am && l0 + HSL < ds) {
-                    const uint32_t na = (uint32_t)__popcll(am);
-                    if (na <= HPUSH && qcnt + na <= HQCAP) {
-                        if ((am >> lane) & 1ull) {
-                            const uint32_t e = qcnt + (uint32_t)__popcll(am & H.lt);
-                            dq[e] = HdPair{(uint32_t)q | ((l0 + HSL) << 8), c, s};
-                            alive &= ~(1u << q);
-                        }
-                        qcnt += na;
-                    }
-                }
             }
         }
         // the pairs still undecided after every slab are results
@@ -451,13 +338,7 @@ __device__ inline void item_hd(const JobView &J, const JoinItem &w, double *qs, 
             const uint32_t p = q0 + q;
             H.emit(J, acc, p, c, acc ? (unsigned long long)J.perm[p] : 0ull, corig);
         }
-        if (qcnt >= WAVE) {
-            hd_drain<COUNT>(J, q0, dq, qcnt, H, dims);
-            qcnt = 0;
-        }
     }
-    if (qcnt)
-        hd_drain<COUNT>(J, q0, dq, qcnt, H, dims);
 }
 
 // Runs item w of join J with the variant for its row width; the LDS slice
