@@ -723,17 +723,36 @@ __global__ __launch_bounds__(512, 1) void k_mfma_join(
                 }
             }
         }
-        // band entries (rare): staged slot by slot, with their scores
-#pragma unroll
+        // band entries (rare): staged slot by slot, with their scores; one
+        // 32-row block at a time (not unrolled: a copy of its accumulators)
+#pragma unroll 1
         for (int bi = 0; bi < 4; ++bi) {
-            if (!__builtin_amdgcn_ballot_w64(bmask[bi] != 0))
+            // register arrays need constant indices
+            const uint32_t bm = bi == 0 ? bmask[0] : bi == 1 ? bmask[1] : bi == 2 ? bmask[2] : bmask[3];
+            if (!__builtin_amdgcn_ballot_w64(bm != 0))
                 continue;
+            const uint32_t amk = bi == 0 ? amask[0] : bi == 1 ? amask[1] : bi == 2 ? amask[2] : amask[3];
+            floatx16 cur[2];
+            switch (bi) {
+            case 0:
+                cur[0] = acc[0][0], cur[1] = acc[0][1];
+                break;
+            case 1:
+                cur[0] = acc[1][0], cur[1] = acc[1][1];
+                break;
+            case 2:
+                cur[0] = acc[2][0], cur[1] = acc[2][1];
+                break;
+            default:
+                cur[0] = acc[3][0], cur[1] = acc[3][1];
+                break;
+            }
 #pragma unroll
             for (int e = 0; e < 16; ++e) {
                 const uint32_t il = 128 * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
 #pragma unroll
                 for (int bj = 0; bj < 2; ++bj) {
-                    const bool push = (bmask[bi] >> (2 * e + bj)) & 1u;
+                    const bool push = (bm >> (2 * e + bj)) & 1u;
                     const uint64_t mb = __builtin_amdgcn_ballot_w64(push);
                     if (!mb)
                         continue;
@@ -741,11 +760,11 @@ __global__ __launch_bounds__(512, 1) void k_mfma_join(
                     const uint32_t k = (uint32_t)__popcll(mb);
                     if (bfill + k > BHITS)
                         bflush();
-                    const bool a_ok = (amask[bi] >> (2 * e + bj)) & 1u;
+                    const bool a_ok = (amk >> (2 * e + bj)) & 1u;
                     bool inband = push;
                     if (push) {
                         const float4 ta = tA[il];
-                        const float G = acc[bi][bj][e];
+                        const float G = cur[bj][e];
                         const uint32_t p = qbeg + il, c = cb + jl;
                         const bool valid = p < w.qend && c < w.cend && (!diag_tile || p <= c);
                         const bool r_ok = fmaf(-2.f, G, ta.y + tb[bj].y) > hi;
