@@ -64,7 +64,8 @@ namespace msj {
 
 namespace {
 
-constexpr uint32_t TM = 256;     // rows per workgroup tile, A and B (== threads per block)
+constexpr uint32_t TM = 256;     // rows per tile of the work-item grid, A and B
+constexpr uint32_t KS_HALF = 8;  // k-steps (16 dims) up to which half tiles are used
 constexpr uint32_t KSL = 2;      // 16-dim k-steps per LDS slab
 constexpr uint32_t SBS = 8;      // tiles per side of a super-block
 constexpr uint32_t MHITS = 128;  // per-wave LDS buffer of result keys
@@ -335,8 +336,8 @@ __global__ __launch_bounds__(256) void k_sblk_fill(const uint32_t *__restrict__ 
 // reject (or NaN-scored) takes the full pass, one 32-row block at a time:
 // it tests validity explicitly, collects certain accepts into a lane mask
 // and stages band entries in LDS.
-template <bool SPLIT, bool DIAG>
-__global__ __launch_bounds__(512, 1) void k_mfma_join(
+template <bool SPLIT, bool DIAG, bool HALF>
+__global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
     const uint4 *__restrict__ fh, const uint4 *__restrict__ fl, uint32_t KS,
     const float4 *__restrict__ thr, const uint32_t *__restrict__ porig,
     const uint32_t *__restrict__ pjoin, const MItem *__restrict__ items, SlotMap sm,
@@ -346,19 +347,23 @@ __global__ __launch_bounds__(512, 1) void k_mfma_join(
     uint32_t sample_every, unsigned long long band_budget)
 {
     constexpr uint32_t NP = SPLIT ? 2 : 1;
-    constexpr uint32_t RB = TM / 32;              // row blocks per tile side
-    constexpr uint32_t SLAB = RB * KSL * WAVE;    // uint4 per plane per operand
-    constexpr uint32_t UNITS = 2 * NP * RB * KSL;  // 1 KB copies per slab
-    constexpr uint32_t NW = 8;                    // waves per workgroup
+    constexpr uint32_t TN = HALF ? TM / 2 : TM;   // B rows of a workgroup tile
+    constexpr uint32_t KL = SPLIT && HALF ? 1 : KSL;  // k-steps per LDS slab
+    constexpr uint32_t RBA = TM / 32;             // A row blocks per workgroup tile
+    constexpr uint32_t RBB = TN / 32;             // B row blocks per workgroup tile
+    constexpr uint32_t SLA = RBA * KL * WAVE;     // uint4 per plane, A
+    constexpr uint32_t SLB = RBB * KL * WAVE;     // uint4 per plane, B
+    constexpr uint32_t UNITS = NP * (RBA + RBB) * KL;  // 1 KB copies per slab
+    constexpr uint32_t NW = HALF ? 4 : 8;         // waves per workgroup
     constexpr uint32_t BHITS = 64;                // per-wave band staging
     static_assert(UNITS % NW == 0, "every wave issues the same number of copies");
     // two slab buffers: the next slab is copied (global -> LDS, no VGPRs)
     // while the matrix cores work on the current one
-    __shared__ uint4 sA[2][NP][SLAB];
-    __shared__ uint4 sB[2][NP][SLAB];
-    __shared__ float4 tA[TM], tB[TM];
+    __shared__ uint4 sA[2][NP][SLA];
+    __shared__ uint4 sB[2][NP][SLB];
+    __shared__ float4 tA[TM], tB[TN];
     __shared__ __align__(16) float tAy[TM];  // tA[.].y, four rows per 16-byte read
-    __shared__ uint32_t oA[TM], oB[TM], jA[TM], jB[TM];
+    __shared__ uint32_t oA[TM], oB[TN], jA[TM], jB[TN];
     __shared__ unsigned long long hsh[NW][MHITS];
     __shared__ unsigned long long bsh[NW][BHITS];
     __shared__ float ssh[NW][BHITS];
@@ -367,7 +372,7 @@ __global__ __launch_bounds__(512, 1) void k_mfma_join(
         s_stop = 0;
 
     const uint32_t tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid / WAVE;
-    const uint32_t wr = wv >> 2, wc = wv & 3, h = lane / 32;
+    const uint32_t wr = HALF ? wv >> 1 : wv >> 2, wc = HALF ? wv & 1 : wv & 3, h = lane / 32;
     const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (WAVE - lane));
     const float INF = __builtin_inff();
     unsigned long long *hb = hsh[wv];
@@ -410,10 +415,11 @@ __global__ __launch_bounds__(512, 1) void k_mfma_join(
         b0fill = 0;
     };
 
-    // Tile slots in a grid-stride loop over virtual block ids; id v runs
-    // slot (v % 8) * per_xcd + v / 8, so the slots of one XCD (the hardware
-    // deals workgroups out round robin) are consecutive: one super-block.
-    // Slots outside their item (ragged node edges, below the diagonal) are
+    // Half-tile slots in a grid-stride loop over virtual block ids; id v
+    // runs half slot (v % 8) * per_xcd + v / 8 (tile slot = half slot / 2,
+    // B half = half slot % 2), so the slots of one XCD (the hardware deals
+    // workgroups out round robin) are consecutive: one super-block.  Slots
+    // outside their item (ragged node edges, below the diagonal) are
     // skipped.
     //
     // Tiles are software-pipelined: during the last slab of a tile the
@@ -423,12 +429,14 @@ __global__ __launch_bounds__(512, 1) void k_mfma_join(
     const uint64_t vtotal = 8ull * per_xcd;
     struct Slot {
         uint64_t it;
-        uint32_t ta, tbk;
+        uint32_t ta, tbk, half;
         bool in;
     };
     auto slot_of = [&](uint64_t v) {
         Slot S;
-        const uint64_t t = (v % 8u) * per_xcd + v / 8u;
+        const uint64_t u = (v % 8u) * per_xcd + v / 8u;
+        const uint64_t t = HALF ? u >> 1 : u;
+        S.half = HALF ? (uint32_t)(u & 1u) : 0u;
         S.in = t < sm.slot_base[SBS];
         uint32_t cls = 0;
         while (S.in && cls + 1 < SBS && t >= sm.slot_base[cls + 1])
@@ -441,46 +449,59 @@ __global__ __launch_bounds__(512, 1) void k_mfma_join(
         S.tbk = r % sz;
         return S;
     };
+    // first A row and first B row of a half tile
+    auto a0_of = [&](const Slot &S, const MItem &w) { return w.qbeg + S.ta * TM; };
+    auto b0_of = [&](const Slot &S, const MItem &w) { return w.cbeg + S.tbk * TM + S.half * TN; };
     auto valid = [&](const Slot &S, const MItem &w) {
-        return S.in && w.qbeg + S.ta * TM < w.qend && w.cbeg + S.tbk * TM < w.cend &&
+        return S.in && a0_of(S, w) < w.qend && b0_of(S, w) < w.cend &&
                !((w.self & 0x200u) && S.tbk < S.ta);
     };
-    // slab copy into buffer `buf`: unit u = ((operand * NP + plane) * RB +
-    // block) * KSL + kstep is one wave-wide 16-byte-per-lane load straight
-    // into LDS (the lane layout of a fragment block is its memory layout);
-    // wave wv issues units wv, wv + 8, ...
+    // slab copy into buffer `buf`: unit u = ((plane * (RBA + RBB)) + block)
+    // * KL + kstep, blocks 0..RBA-1 of A then RBB of B, is one wave-wide
+    // 16-byte-per-lane load straight into LDS (the lane layout of a fragment
+    // block is its memory layout); wave wv issues units wv, wv + NW, ...
     auto copy_slab = [&](uint32_t rbA, uint32_t rbB, uint32_t k0, uint32_t buf) {
 #pragma unroll
         for (uint32_t m = 0; m < UNITS / NW; ++m) {
             const uint32_t u = wv + NW * m;
-            const uint32_t kk = u % KSL, blk = (u / KSL) % RB, pl = (u / (KSL * RB)) % NP;
-            const uint32_t op = u / (KSL * RB * NP);
+            const uint32_t kk = u % KL, blk = (u / KL) % (RBA + RBB), pl = u / (KL * (RBA + RBB));
+            const bool op = blk >= RBA;
+            const uint32_t b = op ? blk - RBA : blk;
             const uint4 *src =
-                (pl ? fl : fh) + ((size_t)((op ? rbB : rbA) + blk) * KS + k0 + kk) * WAVE + lane;
-            uint4 *dst = (op ? &sB[buf][pl][0] : &sA[buf][pl][0]) + (blk * KSL + kk) * WAVE;
+                (pl ? fl : fh) + ((size_t)((op ? rbB : rbA) + b) * KS + k0 + kk) * WAVE + lane;
+            uint4 *dst = (op ? &sB[buf][pl][0] : &sA[buf][pl][0]) + (b * KL + kk) * WAVE;
             __builtin_amdgcn_global_load_lds((const void *)src,
                                              (void __attribute__((address_space(3))) *)dst, 16,
                                              0, 0);
         }
     };
-    // row terms: threads 0..TM-1 the A rows, TM..2TM-1 the B rows; rows
-    // outside the item (padding, the next node) get +inf terms
-    float4 t4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    uint32_t o4 = 0, j4 = 0;
+    // row terms: thread k < TM loads A row k; B row k is loaded by thread k
+    // (half tiles: k < TN) or TM + k (full tiles); rows outside the item
+    // (padding, the next node) get +inf terms
+    float4 t4 = make_float4(0.f, 0.f, 0.f, 0.f), u4 = t4;
+    uint32_t o4 = 0, j4 = 0, ou = 0, ju = 0;
     auto fetch_terms = [&](const Slot &S, const MItem &w) {
-        const bool isb = tid >= TM;
-        const uint32_t k = isb ? tid - TM : tid;
-        const uint32_t r_ = (isb ? w.cbeg + S.tbk * TM : w.qbeg + S.ta * TM) + k;
-        t4 = thr[r_];
-        if (r_ >= (isb ? w.cend : w.qend))
-            t4 = make_float4(INF, INF, INF, INF);
-        o4 = porig[r_];
-        j4 = pjoin[r_];
+        if (tid < TM) {
+            const uint32_t ra = a0_of(S, w) + tid;
+            t4 = thr[ra];
+            if (ra >= w.qend)
+                t4 = make_float4(INF, INF, INF, INF);
+            o4 = porig[ra];
+            j4 = pjoin[ra];
+        }
+        if (HALF ? tid < TN : tid >= TM) {
+            const uint32_t rb = b0_of(S, w) + (HALF ? tid : tid - TM);
+            u4 = thr[rb];
+            if (rb >= w.cend)
+                u4 = make_float4(INF, INF, INF, INF);
+            ou = porig[rb];
+            ju = pjoin[rb];
+        }
     };
 
     // the first tile
     uint64_t v = blockIdx.x;
-    Slot cs = {0, 0, 0, false};
+    Slot cs = {0, 0, 0, 0, false};
     MItem w = {0, 0, 0, 0, 0};
     bool have = false;
     for (; v < vtotal; v += gridDim.x) {
@@ -496,18 +517,21 @@ __global__ __launch_bounds__(512, 1) void k_mfma_join(
     uint32_t buf = 0;  // LDS buffer of the current slab
     if (have) {
         fetch_terms(cs, w);
-        copy_slab((w.qbeg + cs.ta * TM) / 32, (w.cbeg + cs.tbk * TM) / 32, 0, 0);
+        copy_slab(a0_of(cs, w) / 32, b0_of(cs, w) / 32, 0, 0);
     }
     while (have) {
         __syncthreads();  // the previous tile is done with tA .. jB
-        {
-            const bool isb = tid >= TM;
-            const uint32_t k = isb ? tid - TM : tid;
-            (isb ? tB : tA)[k] = t4;
-            (isb ? oB : oA)[k] = o4;
-            (isb ? jB : jA)[k] = j4;
-            if (!isb)
-                tAy[k] = t4.y;
+        if (tid < TM) {
+            tA[tid] = t4;
+            oA[tid] = o4;
+            jA[tid] = j4;
+            tAy[tid] = t4.y;
+        }
+        if (HALF ? tid < TN : tid >= TM) {
+            const uint32_t k = HALF ? tid : tid - TM;
+            tB[k] = u4;
+            oB[k] = ou;
+            jB[k] = ju;
         }
         // the band counter is read every 32nd slot of a workgroup: one global
         // address polled by every workgroup on every tile is a hot spot
@@ -516,17 +540,19 @@ __global__ __launch_bounds__(512, 1) void k_mfma_join(
                      band_budget;
         // a row of the tile with NaN terms (non-finite or huge values) sends
         // every 32-row block to the full pass
-        const bool tile_nan = __syncthreads_or(t4.x != t4.x || t4.y != t4.y) != 0;
+        const bool tile_nan = __syncthreads_or((tid < TM && (t4.x != t4.x || t4.y != t4.y)) ||
+                                               ((HALF ? tid < TN : tid >= TM) &&
+                                                (u4.x != u4.x || u4.y != u4.y))) != 0;
         if (s_stop)
             break;  // uniform over the workgroup
         const uint64_t it = cs.it;
-        const uint32_t qbeg = w.qbeg + cs.ta * TM, cb = w.cbeg + cs.tbk * TM;
+        const uint32_t qbeg = a0_of(cs, w), cb = b0_of(cs, w);
         const bool sampled = DIAG && (it % sample_every == 0);
         const bool self = (w.self & 0x100u) != 0;
         const uint32_t rbA = qbeg / 32, rbB = cb / 32;
         // the next slot's item, loaded now, tested before the last slab
         uint64_t vn = v + gridDim.x;
-        Slot ns = {0, 0, 0, false};
+        Slot ns = {0, 0, 0, 0, false};
         MItem nw = {0, 0, 0, 0, 0};
         if (vn < vtotal) {
             ns = slot_of(vn);
@@ -544,20 +570,23 @@ __global__ __launch_bounds__(512, 1) void k_mfma_join(
                 for (int e = 0; e < 16; ++e)
                     acc[bi][bj][e] = 0.f;
         // LDS byte addresses of this lane's first A and B fragments (k-step 0,
-        // buffer 0); fragment (block, k-step) lies (block * KSL + kstep) KiB
-        // further, the low plane SLAB * 16 bytes, buffer 1 NP * SLAB * 16.
+        // buffer 0); fragment (block, k-step) lies (block * KL + kstep) KiB
+        // further, the low plane SLA * 16 (A) or SLB * 16 (B) bytes, buffer 1
+        // NP times that.
         const uint32_t ldsA = (uint32_t)(uintptr_t)(
-            (__attribute__((address_space(3))) uint4 *)&sA[0][0][(4 * wr * KSL) * WAVE + lane]);
+            (__attribute__((address_space(3))) uint4 *)&sA[0][0][(4 * wr * KL) * WAVE + lane]);
         const uint32_t ldsB = (uint32_t)(uintptr_t)(
-            (__attribute__((address_space(3))) uint4 *)&sB[0][0][(2 * wc * KSL) * WAVE + lane]);
-        static_assert(KSL == 2 && SLAB * 16 == 16384, "fragment read offsets below");
-        for (uint32_t k0 = 0; k0 < KS; k0 += KSL, buf ^= 1u) {
+            (__attribute__((address_space(3))) uint4 *)&sB[0][0][(2 * wc * KL) * WAVE + lane]);
+        static_assert(SPLIT && HALF ? (KL == 1 && SLA * 16 == 8192 && SLB * 16 == 4096)
+                                    : (KL == 2 && SLA * 16 == 16384 && SLB * 16 == TN * 64),
+                      "fragment read offsets below");
+        for (uint32_t k0 = 0; k0 < KS; k0 += KL, buf ^= 1u) {
             // this wave's copies of the slab have landed; after the barrier,
             // everyone's have, and nobody still reads the other buffer
             asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
             __syncthreads();
-            if (k0 + KSL < KS) {
-                copy_slab(rbA, rbB, k0 + KSL, buf ^ 1u);  // in flight during this slab's MFMAs
+            if (k0 + KL < KS) {
+                copy_slab(rbA, rbB, k0 + KL, buf ^ 1u);  // in flight during this slab's MFMAs
             } else {
                 // last slab: find the next tile (a skipped slot costs one more
                 // item load) and start its first slab and row terms
@@ -574,20 +603,38 @@ __global__ __launch_bounds__(512, 1) void k_mfma_join(
                         nw = items[ns.it];
                 }
                 if (nhave) {
-                    copy_slab((nw.qbeg + ns.ta * TM) / 32, (nw.cbeg + ns.tbk * TM) / 32, 0,
-                              buf ^ 1u);
+                    copy_slab(a0_of(ns, nw) / 32, b0_of(ns, nw) / 32, 0, buf ^ 1u);
                     fetch_terms(ns, nw);
                 }
             }
 #pragma unroll
-            for (uint32_t kk = 0; kk < KSL; ++kk) {
+            for (uint32_t kk = 0; kk < KL; ++kk) {
                 // the k-step's fragments in one block, waited for inside it
                 // (outputs early-clobber: no read can overwrite an address);
                 // the copy of the next slab stays in flight
-                const uint32_t va = ldsA + buf * (NP * SLAB * 16) + kk * 1024;
-                const uint32_t vb = ldsB + buf * (NP * SLAB * 16) + kk * 1024;
+                const uint32_t va = ldsA + buf * (NP * SLA * 16) + kk * 1024;
+                const uint32_t vb = ldsB + buf * (NP * SLB * 16) + kk * 1024;
                 uint4 a0, a1, a2, a3, b0, b1, l0, l1, l2, l3, m0, m1;
-                if (SPLIT) {
+                if (SPLIT && HALF) {  // one k-step per slab: blocks 1 KiB apart
+                    asm volatile(
+                        "ds_read_b128 %0, %12\n"
+                        "ds_read_b128 %1, %12 offset:1024\n"
+                        "ds_read_b128 %2, %12 offset:2048\n"
+                        "ds_read_b128 %3, %12 offset:3072\n"
+                        "ds_read_b128 %4, %13\n"
+                        "ds_read_b128 %5, %13 offset:1024\n"
+                        "ds_read_b128 %6, %12 offset:8192\n"
+                        "ds_read_b128 %7, %12 offset:9216\n"
+                        "ds_read_b128 %8, %12 offset:10240\n"
+                        "ds_read_b128 %9, %12 offset:11264\n"
+                        "ds_read_b128 %10, %13 offset:4096\n"
+                        "ds_read_b128 %11, %13 offset:5120\n"
+                        "s_waitcnt lgkmcnt(0)"
+                        : "=&v"(a0), "=&v"(a1), "=&v"(a2), "=&v"(a3), "=&v"(b0), "=&v"(b1),
+                          "=&v"(l0), "=&v"(l1), "=&v"(l2), "=&v"(l3), "=&v"(m0), "=&v"(m1)
+                        : "v"(va), "v"(vb)
+                        : "memory");
+                } else if (SPLIT) {  // two k-steps per slab: blocks 2 KiB apart
                     asm volatile(
                         "ds_read_b128 %0, %12\n"
                         "ds_read_b128 %1, %12 offset:2048\n"
@@ -650,7 +697,10 @@ __global__ __launch_bounds__(512, 1) void k_mfma_join(
 
         // ---- epilogue.  C[i][j] of MFMA tile (bi, bj) is in lane
         // j + 32h, element e, with i = 8 (e / 4) + 4 h + e % 4.
-        const bool diag_tile = self && cb == qbeg;  // inside: pairs p <= c only
+        // a half tile crossing the diagonal of a node with itself: pairs
+        // p <= c only (there cb >= qbeg: doff = cb - qbeg is 0 or TN)
+        const bool diag_tile = self && cb < qbeg + TM && qbeg < cb + TN;
+        const uint32_t doff = cb - qbeg;
         float4 tb[2];
 #pragma unroll
         for (int bj = 0; bj < 2; ++bj)
@@ -676,7 +726,7 @@ __global__ __launch_bounds__(512, 1) void k_mfma_join(
 #pragma unroll
                         for (int bj = 0; bj < 2; ++bj) {
                             float sc = fmaf(-2.f, acc[bi][bj][4 * eg + ei], ay[ei] + tb[bj].y);
-                            if (DT && i0 + ei > 64 * wc + 32 * bj + (lane & 31))
+                            if (DT && i0 + ei > 64 * wc + 32 * bj + (lane & 31) + doff)
                                 sc = INF;
                             m = fminf(m, sc);
                         }
@@ -1103,7 +1153,11 @@ uint64_t MfmaRunner::run(const Index &ix, JoinRunner &J, const double *xr, uint3
             band.ensure(bwant);
             bscore.ensure(bwant);
         }
-        const uint64_t slots = smap.slot_base[SBS];
+        // d <= 16 KS_HALF: half tiles (256 x 128, two workgroups per CU, the
+        // epilogue of one overlaps the other's MFMAs); above: 256 x 256 tiles
+        // (twice the operand reuse per slab copy)
+        const bool half = KS <= KS_HALF;
+        const uint64_t slots = (half ? 2 : 1) * smap.slot_base[SBS];
         const uint64_t per_xcd = (slots + 7) / 8;
         const uint32_t grid =
             (uint32_t)std::max<uint64_t>(8, std::min<uint64_t>(per_xcd * 8, MAX_GRID));
@@ -1111,7 +1165,14 @@ uint64_t MfmaRunner::run(const Index &ix, JoinRunner &J, const double *xr, uint3
         MSJ_HIP(hipEventRecord(k0, s));
         if (nitems) {
 #define MSJ_MJ(SP, DG)                                                                          \
-    LAUNCH((k_mfma_join<SP, DG>), grid, 512, fh.p, SP ? fl.p : fh.p, KS, thr.p, porig.p, pjoin.p, \
+    do {                                                                                        \
+        if (half)                                                                               \
+            MSJ_MJH(SP, DG, true, 256);                                                         \
+        else                                                                                    \
+            MSJ_MJH(SP, DG, false, 512);                                                        \
+    } while (0)
+#define MSJ_MJH(SP, DG, HF, NT)                                                                 \
+    LAUNCH((k_mfma_join<SP, DG, HF>), grid, NT, fh.p, SP ? fl.p : fh.p, KS, thr.p, porig.p, pjoin.p, \
            items.p, smap, per_xcd, lo, hi, qbits, (unsigned long long *)J.pairs.p,              \
            (unsigned long long)J.pairs.cap, ctr.p, (unsigned long long *)band.p, bscore.p,      \
            (unsigned long long)band.cap, sample_every, (unsigned long long)budget)
@@ -1127,6 +1188,7 @@ uint64_t MfmaRunner::run(const Index &ix, JoinRunner &J, const double *xr, uint3
                     MSJ_MJ(false, false);
             }
 #undef MSJ_MJ
+#undef MSJ_MJH
             if (diag)
                 LAUNCH(k_band_check<true>, 2048, 256, xr, ds, eps2, J.jperm.p,
                        (const unsigned long long *)band.p, bscore.p,
