@@ -44,12 +44,15 @@
 //
 // Work.  A tile is 256 A rows of node X against 256 B rows of a node Y >= X
 // of X's upper neighbour list (inside X only tiles on or above the diagonal,
-// pairs p <= c); a workgroup of 8 waves computes one tile at a time, wave
-// (wr, wc) a 128 x 64 block as 4 x 2 MFMA 32x32 tiles.  Tiles are grouped
-// into super-blocks of up to 8 x 8 tiles (the work items); consecutive tile
-// slots run on one XCD.  32-dimension slabs are copied global -> LDS
-// directly into a double buffer: the next slab's copy is in flight while the
-// current one runs on the matrix cores.
+// pairs p <= c).  Above d = 128 a workgroup of 8 waves computes one tile at a
+// time, wave (wr, wc) a 128 x 64 block as 4 x 2 MFMA 32x32 tiles; up to
+// d = 128 two workgroups of 4 waves share a CU, each computing a half tile
+// (256 x 128), so one's epilogue overlaps the other's MFMAs.  Tiles are
+// grouped into super-blocks of up to 8 x 8 tiles (the work items);
+// consecutive tile slots run on one XCD.  Slabs of 32 dimensions (16 for
+// split half tiles) are copied global -> LDS directly into a double buffer:
+// the next slab's copy is in flight while the current one runs on the
+// matrix cores.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -321,13 +324,14 @@ __global__ __launch_bounds__(256) void k_sblk_fill(const uint32_t *__restrict__ 
 // (count_work): class 1, a certain accept (every one is verified); class 2,
 // a certain reject of a sampled work item (verified).
 //
-// Tile.  A workgroup (8 waves) computes a 256 x 256 tile; wave (wr, wc) the
-// 128 x 64 block of rows 128 wr.. and columns 64 wc.., as 4 x 2 MFMA 32x32
-// tiles (128 FP32 accumulators per lane).  Per 32-dimension slab the
-// workgroup copies 16 row blocks of fragments (8 of A, 8 of B; both planes
-// when split) from global memory straight into LDS, and each wave reads its
-// 4 + 2 fragments per k-step from there; the next slab's copy runs into the
-// other LDS buffer while the matrix cores work on the current one.
+// Tile.  A workgroup (NW waves) computes a TM x TN tile (256 x 256, or a
+// 256 x 128 half tile with HALF); wave (wr, wc) the 128 x 64 block of rows
+// 128 wr.. and columns 64 wc.., as 4 x 2 MFMA 32x32 tiles (128 FP32
+// accumulators per lane).  Per slab of KL k-steps the workgroup copies the
+// row blocks of fragments (8 of A, TN / 32 of B; both planes when split)
+// from global memory straight into LDS, and each wave reads its 4 + 2
+// fragments per k-step from there; the next slab's copy runs into the other
+// LDS buffer while the matrix cores work on the current one.
 //
 // Epilogue.  The A and B rows' terms are staged per tile; rows outside the
 // item (padding, the next node) get +inf terms, so the fast pass -- one add,
