@@ -265,15 +265,20 @@ __global__ __launch_bounds__(256) void k_rows_nbrs(const uint64_t *__restrict__ 
         nbrs[i] = (uint32_t)(pairs[i] & qmask);
 }
 
-// Per-column minimum and maximum of the first columns (one block per column).
+// Per-column minimum and maximum of the first columns: blockIdx.y is the
+// column, the blocks along x reduce row ranges into part[j][b] = (lo, hi);
+// k_col_range_final reduces those (a minimum or maximum does not depend on
+// the order it is taken in).
+constexpr uint32_t CR_BLOCKS = 64;
+
 template <class T>
 __global__ __launch_bounds__(256) void k_col_range(const T *__restrict__ xs, uint32_t n,
-                                                   double *__restrict__ mm)
+                                                   double *__restrict__ part)
 {
     __shared__ double lo_s[256 / WAVE], hi_s[256 / WAVE];
-    const uint32_t j = blockIdx.x;
+    const uint32_t j = blockIdx.y, b = blockIdx.x;
     double lo = __builtin_inf(), hi = -__builtin_inf();
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+    for (uint32_t i = b * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         double v = (double)xs[(size_t)j * n + i];
         lo = v < lo ? v : lo;
         hi = v > hi ? v : hi;
@@ -290,9 +295,25 @@ __global__ __launch_bounds__(256) void k_col_range(const T *__restrict__ xs, uin
             lo = lo_s[w] < lo ? lo_s[w] : lo;
             hi = hi_s[w] > hi ? hi_s[w] : hi;
         }
-        mm[2 * j] = lo;
-        mm[2 * j + 1] = hi;
+        part[2 * ((size_t)j * gridDim.x + b)] = lo;
+        part[2 * ((size_t)j * gridDim.x + b) + 1] = hi;
     }
+}
+
+__global__ void k_col_range_final(const double *__restrict__ part, uint32_t nb,
+                                  double *__restrict__ mm)
+{
+    const uint32_t j = blockIdx.x;
+    if (threadIdx.x != 0)
+        return;
+    double lo = __builtin_inf(), hi = -__builtin_inf();
+    for (uint32_t b = 0; b < nb; ++b) {
+        const double l = part[2 * ((size_t)j * nb + b)], h = part[2 * ((size_t)j * nb + b) + 1];
+        lo = l < lo ? l : lo;
+        hi = h > hi ? h : hi;
+    }
+    mm[2 * j] = lo;
+    mm[2 * j + 1] = hi;
 }
 
 __device__ inline uint32_t spread3(uint32_t v)  // 10 bits -> every third bit
@@ -402,15 +423,22 @@ void JoinRunner::order(const Index &ix, const void *xs, bool f32, uint32_t n, ui
                        hipStream_t s)
 {
     const uint32_t km = d < 3 ? d : 3u;
-    mm.ensure(2 * 3);
+    mm.ensure(2 * 3 + 2 * 3 * CR_BLOCKS);  // the result, then the partial ranges
+    const uint32_t crb = std::max<uint32_t>(1u, std::min<uint32_t>(CR_BLOCKS, (n + 255) / 256));
     jperm.ensure(n);
     jkeys.ensure(n);
     if (f32) {
-        LAUNCH(k_col_range<float>, km, 256, (const float *)xs, n, mm.p);
+        hipLaunchKernelGGL(k_col_range<float>, dim3(crb, km), dim3(256), 0, s, (const float *)xs,
+                           n, mm.p + 6);
+        MSJ_HIP(hipGetLastError());
+        LAUNCH(k_col_range_final, km, 64, mm.p + 6, crb, mm.p);
         LAUNCH(k_morton_keys<float>, grid_for(n, 256), 256, (const float *)xs, n, km, mm.p,
                ix.nstart.p, ix.M, ix.perm.p, jkeys.p, jperm.p);
     } else {
-        LAUNCH(k_col_range<double>, km, 256, (const double *)xs, n, mm.p);
+        hipLaunchKernelGGL(k_col_range<double>, dim3(crb, km), dim3(256), 0, s,
+                           (const double *)xs, n, mm.p + 6);
+        MSJ_HIP(hipGetLastError());
+        LAUNCH(k_col_range_final, km, 64, mm.p + 6, crb, mm.p);
         LAUNCH(k_morton_keys<double>, grid_for(n, 256), 256, (const double *)xs, n, km, mm.p,
                ix.nstart.p, ix.M, ix.perm.p, jkeys.p, jperm.p);
     }
