@@ -52,6 +52,10 @@ CASES = [
     ("mixture 20000x18", lambda: datagen.mixture(20000, 18, 13), 0.204),
     ("uniform 4096x2", lambda: datagen.uniform(4096, 2, 11), 0.02),
     ("mixture 3000x33", lambda: datagen.mixture(3000, 33, 24), 0.45),
+    # either side of the tile-geometry switch: d = 128 (8 k-steps) runs half
+    # tiles, d = 129 (10 k-steps) full 256 x 256 tiles
+    ("correlated 4000x128 half tiles", lambda: datagen.correlated(4000, 128, 41), 1.497),
+    ("correlated 4000x129 full tiles", lambda: datagen.correlated(4000, 129, 42), 1.495),
 ]
 
 
