@@ -170,11 +170,18 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_hist(const uint64_t *__restrict
     const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (WAVE - lane));
     cnt[threadIdx.x] = 0;
     __syncthreads();
-    size_t base = (size_t)blockIdx.x * RS_TILE;
+    const size_t base = (size_t)blockIdx.x * RS_TILE + threadIdx.x;
+    // the tile's keys are loaded up front: one memory round trip per tile
+    uint64_t kr[RS_ROUNDS];
+#pragma unroll
     for (int r = 0; r < RS_ROUNDS; ++r) {
-        size_t i = base + (size_t)r * RS_BLOCK + threadIdx.x;
-        bool valid = i < n;
-        uint32_t dg = valid ? (uint32_t)(keys[i] >> shift) & 0xFFu : 0u;
+        const size_t i = base + (size_t)r * RS_BLOCK;
+        kr[r] = i < n ? keys[i] : 0ull;
+    }
+#pragma unroll
+    for (int r = 0; r < RS_ROUNDS; ++r) {
+        bool valid = base + (size_t)r * RS_BLOCK < n;
+        uint32_t dg = (uint32_t)(kr[r] >> shift) & 0xFFu;
         uint64_t m = digit_match(valid, dg);
         if (valid && (m & lt) == 0)  // one LDS atomic per digit per wave
             atomicAdd(&cnt[dg], (uint32_t)__popcll(m));
@@ -204,12 +211,23 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(const uint64_t *__restr
         wcnt[w][t] = 0;
     __syncthreads();
     const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (WAVE - lane));
-    size_t tile0 = (size_t)blockIdx.x * RS_TILE;
+    const size_t tile0 = (size_t)blockIdx.x * RS_TILE + t;
+    // the tile's keys and values are loaded up front (one memory round trip
+    // per tile); the rounds below then rank and store them in input order
+    uint64_t kr[RS_ROUNDS];
+    uint32_t vr[RS_ROUNDS];
+#pragma unroll
     for (int r = 0; r < RS_ROUNDS; ++r) {
-        size_t i = tile0 + (size_t)r * RS_BLOCK + t;
+        const size_t i = tile0 + (size_t)r * RS_BLOCK;
+        kr[r] = i < n ? kin[i] : 0ull;
+        vr[r] = (i < n && vin) ? vin[i] : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < RS_ROUNDS; ++r) {
+        const size_t i = tile0 + (size_t)r * RS_BLOCK;
         bool valid = i < n;
-        uint64_t key = valid ? kin[i] : 0;
-        uint32_t val = (valid && vin) ? vin[i] : 0;
+        uint64_t key = kr[r];
+        uint32_t val = vr[r];
         uint32_t dg = (uint32_t)(key >> shift) & 0xFFu;
         uint64_t m = digit_match(valid, dg);
         uint32_t rank = (uint32_t)__popcll(m & lt);
