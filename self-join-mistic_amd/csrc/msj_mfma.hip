@@ -336,10 +336,12 @@ __global__ __launch_bounds__(256) void k_sblk_fill(const uint32_t *__restrict__ 
 // Epilogue.  The A and B rows' terms are staged per tile; rows outside the
 // item (padding, the next node) get +inf terms, so the fast pass -- one add,
 // one fma, one compare per pair -- rejects them without a validity test.
-// Only a wave whose 128 x 64 block holds a pair that is not a certain
-// reject (or NaN-scored) takes the full pass, one 32-row block at a time:
-// it tests validity explicitly, collects certain accepts into a lane mask
-// and stages band entries in LDS.
+// Only a 32-row block that holds a pair that is not a certain reject (or
+// NaN-scored) takes the full pass: it tests validity explicitly, collects
+// certain accepts into a lane mask and stages band entries in LDS.  Half
+// tiles take the blocks one at a time through all passes (a band block's
+// accumulators go through a per-lane LDS column); full tiles run the fast
+// pass over all four blocks first.
 template <bool SPLIT, bool DIAG, bool HALF>
 __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
     const uint4 *__restrict__ fh, const uint4 *__restrict__ fl, uint32_t KS,
@@ -375,7 +377,9 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
     if (threadIdx.x == 0)
         s_stop = 0;
 
-    const uint32_t tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid / WAVE;
+    const uint32_t tid = threadIdx.x, lane = tid & (WAVE - 1);
+    // half tiles: the wave index in a scalar register
+    const uint32_t wv = HALF ? __builtin_amdgcn_readfirstlane(tid / WAVE) : tid / WAVE;
     const uint32_t wr = HALF ? wv >> 1 : wv >> 2, wc = HALF ? wv & 1 : wv & 3, h = lane / 32;
     const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (WAVE - lane));
     const float INF = __builtin_inff();
@@ -709,161 +713,321 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
 #pragma unroll
         for (int bj = 0; bj < 2; ++bj)
             tb[bj] = tB[64 * wc + 32 * bj + (lane & 31)];
-        // fast pass: per 32-row block the smallest reject score of its pairs
-        // (in a diagonal tile of pairs p <= c); a block whose smallest score
-        // exceeds hi holds certain rejects only.  Scores of padding rows and
-        // of rows outside the item are +inf; NaN scores (NaN terms) are
-        // caught by tile_nan.
-        float mn[4];
-        auto fast = [&](auto dg) {
-            constexpr bool DT = decltype(dg)::value;
+        if constexpr (HALF) {
+            // half tiles: one 32-row block at a time through all passes
+            constexpr uint32_t NB = 2;    // 32-column MFMA blocks per wave
+            using Mask = uint32_t;        // 16 NB pair slots
+            const uint32_t c0 = 64 * wc;  // first tile column of this wave
+            __shared__ float gst[NW][8][WAVE];  // per-lane scores of band slots
+            bool cvalid[NB];
 #pragma unroll
-            for (int bi = 0; bi < 4; ++bi) {
-                float m = INF;
-#pragma unroll
-                for (int eg = 0; eg < 4; ++eg) {
-                    const uint32_t i0 = 128 * wr + 32 * bi + 8 * eg + 4 * h;
-                    const float4 ay4 = *(const float4 *)&tAy[i0];
-                    const float ay[4] = {ay4.x, ay4.y, ay4.z, ay4.w};
-#pragma unroll
-                    for (int ei = 0; ei < 4; ++ei)
-#pragma unroll
-                        for (int bj = 0; bj < 2; ++bj) {
-                            float sc = fmaf(-2.f, acc[bi][bj][4 * eg + ei], ay[ei] + tb[bj].y);
-                            if (DT && i0 + ei > 64 * wc + 32 * bj + (lane & 31) + doff)
-                                sc = INF;
-                            m = fminf(m, sc);
-                        }
-                }
-                mn[bi] = m;
-            }
-        };
-        if (diag_tile)
-            fast(std::true_type());
-        else
-            fast(std::false_type());
-
-        // full pass, for the 32-row blocks (bi) that need it: the block's
-        // 32 pairs per lane (slot 2 e + bj) classified into lane bit masks of
-        // certain accepts and band entries (with DIAG also the verified
-        // classes)
-        bool cvalid[2];
-#pragma unroll
-        for (int bj = 0; bj < 2; ++bj)
-            cvalid[bj] = cb + 64 * wc + 32 * bj + (lane & 31) < w.cend;
-        uint32_t amask[4] = {0u, 0u, 0u, 0u}, bmask[4] = {0u, 0u, 0u, 0u};
-#pragma unroll
-        for (int bi = 0; bi < 4; ++bi) {
-            const bool need = sampled || tile_nan || !(mn[bi] > hi);
-            if (!__builtin_amdgcn_ballot_w64(need))
-                continue;
-#pragma unroll
-            for (int e = 0; e < 16; ++e) {
-                const uint32_t il = 128 * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
-                const float4 ta = tA[il];
-                const uint32_t p = qbeg + il;
-                const bool pv = p < w.qend;
-#pragma unroll
-                for (int bj = 0; bj < 2; ++bj) {
-                    const uint32_t c = cb + 64 * wc + 32 * bj + (lane & 31);
-                    const bool valid = pv && cvalid[bj] && (!diag_tile || p <= c);
-                    const float G = acc[bi][bj][e];
-                    const bool a_ok = valid && fmaf(-2.f, G, ta.x + tb[bj].x) <= lo;
-                    const bool r_ok = fmaf(-2.f, G, ta.y + tb[bj].y) > hi;
-                    const bool inband = valid && !a_ok && !r_ok;
-                    const bool push = DIAG ? (inband || a_ok || (sampled && valid)) : inband;
-                    amask[bi] |= (uint32_t)a_ok << (2 * e + bj);
-                    bmask[bi] |= (uint32_t)push << (2 * e + bj);
-                }
-            }
-        }
-        // band entries (rare): staged slot by slot, with their scores; one
-        // 32-row block at a time (not unrolled: a copy of its accumulators)
+            for (int bj = 0; bj < (int)NB; ++bj)
+                cvalid[bj] = cb + c0 + 32 * bj + (lane & 31) < w.cend;
+            // band entries (rare): staged slot by slot, with their scores.
+            // Per 32 x 32 MFMA block and half (8 slots per lane) the lane's
+            // accumulators go to its own column of the wave's LDS stage, read
+            // back by slot index in a loop that is not unrolled.
+            auto band_run = [&](uint32_t bi, uint32_t bj, uint32_t hf, Mask bm, Mask amk) {
+                const uint32_t jl = c0 + 32 * bj + (lane & 31);
+                const float4 tbq = tB[jl];
 #pragma unroll 1
-        for (int bi = 0; bi < 4; ++bi) {
-            // register arrays need constant indices
-            const uint32_t bm = bi == 0 ? bmask[0] : bi == 1 ? bmask[1] : bi == 2 ? bmask[2] : bmask[3];
-            if (!__builtin_amdgcn_ballot_w64(bm != 0))
-                continue;
-            const uint32_t amk = bi == 0 ? amask[0] : bi == 1 ? amask[1] : bi == 2 ? amask[2] : amask[3];
-            floatx16 cur[2];
-            switch (bi) {
-            case 0:
-                cur[0] = acc[0][0], cur[1] = acc[0][1];
-                break;
-            case 1:
-                cur[0] = acc[1][0], cur[1] = acc[1][1];
-                break;
-            case 2:
-                cur[0] = acc[2][0], cur[1] = acc[2][1];
-                break;
-            default:
-                cur[0] = acc[3][0], cur[1] = acc[3][1];
-                break;
-            }
-#pragma unroll
-            for (int e = 0; e < 16; ++e) {
-                const uint32_t il = 128 * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
-#pragma unroll
-                for (int bj = 0; bj < 2; ++bj) {
-                    const bool push = (bm >> (2 * e + bj)) & 1u;
+                for (uint32_t j = 0; j < 8; ++j) {
+                    const uint32_t e = 8 * hf + j;
+                    const uint32_t il = 128 * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
+                    const bool push = (bm >> (NB * e + bj)) & 1u;
                     const uint64_t mb = __builtin_amdgcn_ballot_w64(push);
                     if (!mb)
                         continue;
-                    const uint32_t jl = 64 * wc + 32 * bj + (lane & 31);
                     const uint32_t k = (uint32_t)__popcll(mb);
                     if (bfill + k > BHITS)
                         bflush();
-                    const bool a_ok = (amk >> (2 * e + bj)) & 1u;
+                    const bool a_ok = (amk >> (NB * e + bj)) & 1u;
                     bool inband = push;
                     if (push) {
                         const float4 ta = tA[il];
-                        const float G = cur[bj][e];
+                        const float G = gst[wv][j][lane];
                         const uint32_t p = qbeg + il, c = cb + jl;
                         const bool valid = p < w.qend && c < w.cend && (!diag_tile || p <= c);
-                        const bool r_ok = fmaf(-2.f, G, ta.y + tb[bj].y) > hi;
+                        const bool r_ok = fmaf(-2.f, G, ta.y + tbq.y) > hi;
                         inband = valid && !a_ok && !r_ok;
                         const uint32_t cls = DIAG ? (a_ok ? 1u : (inband ? 0u : 2u)) : 0u;
                         const uint32_t idx = bfill + (uint32_t)__popcll(mb & lt);
                         bb[idx] = ((unsigned long long)jA[il] << 32) |
                                   ((unsigned long long)cls << 30) | jB[jl];
-                        sb[idx] = fmaf(-2.f, G, ta.z + tb[bj].z);
+                        sb[idx] = fmaf(-2.f, G, ta.z + tbq.z);
                     }
                     bfill += k;
                     b0fill += DIAG ? (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(inband)) : k;
                 }
-            }
-        }
-        // certain accepts: both orders (the self pair once)
+            };
+            // slots 8 hf .. 8 hf + 7 of column block bj: bits NB e + bj
+            constexpr Mask HALF_COL = 0x5555u;
+
+            // One 32-row block (bi) at a time, so that only its accumulators are
+            // read out of the matrix registers together.
 #pragma unroll
-        for (int bi = 0; bi < 4; ++bi) {
-            uint32_t am = amask[bi];
-            while (true) {
-                const bool has = am != 0;
-                const uint64_t m1 = __builtin_amdgcn_ballot_w64(has);
-                if (!m1)
+            for (int bi = 0; bi < 4; ++bi) {
+                // fast pass: the block's smallest reject score (in a diagonal tile,
+                // of pairs p <= c); a block whose smallest score exceeds hi holds
+                // certain rejects only.  Scores of padding rows and of rows
+                // outside the item are +inf; NaN scores (NaN terms) are caught by
+                // tile_nan.
+                auto fast = [&](auto dg) __attribute__((always_inline)) {
+                    constexpr bool DT = decltype(dg)::value;
+                    float m = INF;
+#pragma unroll
+                    for (int eg = 0; eg < 4; ++eg) {
+                        const uint32_t i0 = 128 * wr + 32 * bi + 8 * eg + 4 * h;
+                        const float4 ay4 = *(const float4 *)&tAy[i0];
+                        const float ay[4] = {ay4.x, ay4.y, ay4.z, ay4.w};
+#pragma unroll
+                        for (int ei = 0; ei < 4; ++ei)
+#pragma unroll
+                            for (int bj = 0; bj < (int)NB; ++bj) {
+                                const float g = acc[bi][bj][4 * eg + ei];
+                                float sc = fmaf(-2.f, g, ay[ei] + tb[bj].y);
+                                if (DT && i0 + ei > c0 + 32 * bj + (lane & 31) + doff)
+                                    sc = INF;
+                                m = fminf(m, sc);
+                            }
+                    }
+                    return m;
+                };
+                const float mn = diag_tile ? fast(std::true_type()) : fast(std::false_type());
+                const bool need = sampled || tile_nan || !(mn > hi);
+                if (!__builtin_amdgcn_ballot_w64(need))
+                    continue;
+                // full pass: the block's 16 NB pairs per lane (slot NB e + bj)
+                // classified into lane bit masks of certain accepts and band
+                // entries (with DIAG also the verified classes); bitwise logic,
+                // no branches
+                Mask amask = 0, bmask = 0;
+#pragma unroll
+                for (int e = 0; e < 16; ++e) {
+                    const uint32_t il = 128 * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
+                    const float4 ta = tA[il];
+                    const uint32_t p = qbeg + il;
+                    const bool pv = p < w.qend;
+#pragma unroll
+                    for (int bj = 0; bj < (int)NB; ++bj) {
+                        const uint32_t c = cb + c0 + 32 * bj + (lane & 31);
+                        const bool valid = pv & cvalid[bj] & (!diag_tile | (p <= c));
+                        const float G = acc[bi][bj][e];
+                        const bool a_ok = valid & (fmaf(-2.f, G, ta.x + tb[bj].x) <= lo);
+                        const bool r_ok = fmaf(-2.f, G, ta.y + tb[bj].y) > hi;
+                        const bool inband = valid & !a_ok & !r_ok;
+                        const bool push = DIAG ? (inband | a_ok | (sampled & valid)) : inband;
+                        amask |= (Mask)a_ok << (NB * e + bj);
+                        bmask |= (Mask)push << (NB * e + bj);
+                    }
+                }
+                // band entries
+                if (__builtin_amdgcn_ballot_w64(bmask != 0)) {
+#pragma unroll
+                    for (int bj = 0; bj < (int)NB; ++bj)
+#pragma unroll
+                        for (int hf = 0; hf < 2; ++hf) {
+                            const Mask col = HALF_COL << (NB * 8 * hf + bj);
+                            if (__builtin_amdgcn_ballot_w64((bmask & col) != 0)) {
+#pragma unroll
+                                for (int j = 0; j < 8; ++j)
+                                    gst[wv][j][lane] = acc[bi][bj][8 * hf + j];
+                                band_run(bi, bj, hf, bmask, amask);
+                            }
+                        }
+                }
+                // certain accepts: both orders (the self pair once)
+                Mask am = amask;
+                while (true) {
+                    const bool has = am != 0;
+                    const uint64_t m1 = __builtin_amdgcn_ballot_w64(has);
+                    if (!m1)
+                        break;
+                    uint32_t il = 0, jl = 0;
+                    if (has) {
+                        const uint32_t k = (uint32_t)__builtin_ctzll((uint64_t)am);
+                        am &= am - 1;
+                        const uint32_t e = k / NB, bj = k % NB;
+                        il = 128 * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
+                        jl = c0 + 32 * bj + (lane & 31);
+                    }
+                    const bool two = has && (qbeg + il) != (cb + jl);
+                    const uint64_t m2 = __builtin_amdgcn_ballot_w64(two);
+                    const uint32_t tot = (uint32_t)(__popcll(m1) + __popcll(m2));
+                    if (fill + tot > MHITS)
+                        flush();
+                    if (has) {
+                        const unsigned long long po = oA[il], co = oB[jl];
+                        const uint32_t idx =
+                            fill + (uint32_t)(__popcll(m1 & lt) + __popcll(m2 & lt));
+                        hb[idx] = (po << qbits) | co;
+                        if (two)
+                            hb[idx + 1] = (co << qbits) | po;
+                    }
+                    fill += tot;
+                }
+            }
+        } else {
+            // fast pass: per 32-row block the smallest reject score of its pairs
+            // (in a diagonal tile of pairs p <= c); a block whose smallest score
+            // exceeds hi holds certain rejects only.  Scores of padding rows and
+            // of rows outside the item are +inf; NaN scores (NaN terms) are
+            // caught by tile_nan.
+            float mn[4];
+            auto fast = [&](auto dg) {
+                constexpr bool DT = decltype(dg)::value;
+#pragma unroll
+                for (int bi = 0; bi < 4; ++bi) {
+                    float m = INF;
+#pragma unroll
+                    for (int eg = 0; eg < 4; ++eg) {
+                        const uint32_t i0 = 128 * wr + 32 * bi + 8 * eg + 4 * h;
+                        const float4 ay4 = *(const float4 *)&tAy[i0];
+                        const float ay[4] = {ay4.x, ay4.y, ay4.z, ay4.w};
+#pragma unroll
+                        for (int ei = 0; ei < 4; ++ei)
+#pragma unroll
+                            for (int bj = 0; bj < 2; ++bj) {
+                                float sc = fmaf(-2.f, acc[bi][bj][4 * eg + ei], ay[ei] + tb[bj].y);
+                                if (DT && i0 + ei > 64 * wc + 32 * bj + (lane & 31) + doff)
+                                    sc = INF;
+                                m = fminf(m, sc);
+                            }
+                    }
+                    mn[bi] = m;
+                }
+            };
+            if (diag_tile)
+                fast(std::true_type());
+            else
+                fast(std::false_type());
+
+            // full pass, for the 32-row blocks (bi) that need it: the block's
+            // 32 pairs per lane (slot 2 e + bj) classified into lane bit masks of
+            // certain accepts and band entries (with DIAG also the verified
+            // classes)
+            bool cvalid[2];
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj)
+                cvalid[bj] = cb + 64 * wc + 32 * bj + (lane & 31) < w.cend;
+            uint32_t amask[4] = {0u, 0u, 0u, 0u}, bmask[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+            for (int bi = 0; bi < 4; ++bi) {
+                const bool need = sampled || tile_nan || !(mn[bi] > hi);
+                if (!__builtin_amdgcn_ballot_w64(need))
+                    continue;
+#pragma unroll
+                for (int e = 0; e < 16; ++e) {
+                    const uint32_t il = 128 * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
+                    const float4 ta = tA[il];
+                    const uint32_t p = qbeg + il;
+                    const bool pv = p < w.qend;
+#pragma unroll
+                    for (int bj = 0; bj < 2; ++bj) {
+                        const uint32_t c = cb + 64 * wc + 32 * bj + (lane & 31);
+                        const bool valid = pv && cvalid[bj] && (!diag_tile || p <= c);
+                        const float G = acc[bi][bj][e];
+                        const bool a_ok = valid && fmaf(-2.f, G, ta.x + tb[bj].x) <= lo;
+                        const bool r_ok = fmaf(-2.f, G, ta.y + tb[bj].y) > hi;
+                        const bool inband = valid && !a_ok && !r_ok;
+                        const bool push = DIAG ? (inband || a_ok || (sampled && valid)) : inband;
+                        amask[bi] |= (uint32_t)a_ok << (2 * e + bj);
+                        bmask[bi] |= (uint32_t)push << (2 * e + bj);
+                    }
+                }
+            }
+            // band entries (rare): staged slot by slot, with their scores; one
+            // 32-row block at a time (not unrolled: a copy of its accumulators)
+#pragma unroll 1
+            for (int bi = 0; bi < 4; ++bi) {
+                // register arrays need constant indices
+                const uint32_t bm =
+                    bi == 0 ? bmask[0] : bi == 1 ? bmask[1] : bi == 2 ? bmask[2] : bmask[3];
+                if (!__builtin_amdgcn_ballot_w64(bm != 0))
+                    continue;
+                const uint32_t amk =
+                    bi == 0 ? amask[0] : bi == 1 ? amask[1] : bi == 2 ? amask[2] : amask[3];
+                floatx16 cur[2];
+                switch (bi) {
+                case 0:
+                    cur[0] = acc[0][0], cur[1] = acc[0][1];
                     break;
-                uint32_t il = 0, jl = 0;
-                if (has) {
-                    const uint32_t k = (uint32_t)__builtin_ctz(am);
-                    am &= am - 1;
-                    const uint32_t e = k >> 1, bj = k & 1u;
-                    il = 128 * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
-                    jl = 64 * wc + 32 * bj + (lane & 31);
+                case 1:
+                    cur[0] = acc[1][0], cur[1] = acc[1][1];
+                    break;
+                case 2:
+                    cur[0] = acc[2][0], cur[1] = acc[2][1];
+                    break;
+                default:
+                    cur[0] = acc[3][0], cur[1] = acc[3][1];
+                    break;
                 }
-                const bool two = has && (qbeg + il) != (cb + jl);
-                const uint64_t m2 = __builtin_amdgcn_ballot_w64(two);
-                const uint32_t tot = (uint32_t)(__popcll(m1) + __popcll(m2));
-                if (fill + tot > MHITS)
-                    flush();
-                if (has) {
-                    const unsigned long long po = oA[il], co = oB[jl];
-                    const uint32_t idx = fill + (uint32_t)(__popcll(m1 & lt) + __popcll(m2 & lt));
-                    hb[idx] = (po << qbits) | co;
-                    if (two)
-                        hb[idx + 1] = (co << qbits) | po;
+#pragma unroll
+                for (int e = 0; e < 16; ++e) {
+                    const uint32_t il = 128 * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
+#pragma unroll
+                    for (int bj = 0; bj < 2; ++bj) {
+                        const bool push = (bm >> (2 * e + bj)) & 1u;
+                        const uint64_t mb = __builtin_amdgcn_ballot_w64(push);
+                        if (!mb)
+                            continue;
+                        const uint32_t jl = 64 * wc + 32 * bj + (lane & 31);
+                        const uint32_t k = (uint32_t)__popcll(mb);
+                        if (bfill + k > BHITS)
+                            bflush();
+                        const bool a_ok = (amk >> (2 * e + bj)) & 1u;
+                        bool inband = push;
+                        if (push) {
+                            const float4 ta = tA[il];
+                            const float G = cur[bj][e];
+                            const uint32_t p = qbeg + il, c = cb + jl;
+                            const bool valid = p < w.qend && c < w.cend && (!diag_tile || p <= c);
+                            const bool r_ok = fmaf(-2.f, G, ta.y + tb[bj].y) > hi;
+                            inband = valid && !a_ok && !r_ok;
+                            const uint32_t cls = DIAG ? (a_ok ? 1u : (inband ? 0u : 2u)) : 0u;
+                            const uint32_t idx = bfill + (uint32_t)__popcll(mb & lt);
+                            bb[idx] = ((unsigned long long)jA[il] << 32) |
+                                      ((unsigned long long)cls << 30) | jB[jl];
+                            sb[idx] = fmaf(-2.f, G, ta.z + tb[bj].z);
+                        }
+                        bfill += k;
+                        b0fill += DIAG ? (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(inband))
+                                       : k;
+                    }
                 }
-                fill += tot;
+            }
+            // certain accepts: both orders (the self pair once)
+#pragma unroll
+            for (int bi = 0; bi < 4; ++bi) {
+                uint32_t am = amask[bi];
+                while (true) {
+                    const bool has = am != 0;
+                    const uint64_t m1 = __builtin_amdgcn_ballot_w64(has);
+                    if (!m1)
+                        break;
+                    uint32_t il = 0, jl = 0;
+                    if (has) {
+                        const uint32_t k = (uint32_t)__builtin_ctz(am);
+                        am &= am - 1;
+                        const uint32_t e = k >> 1, bj = k & 1u;
+                        il = 128 * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
+                        jl = 64 * wc + 32 * bj + (lane & 31);
+                    }
+                    const bool two = has && (qbeg + il) != (cb + jl);
+                    const uint64_t m2 = __builtin_amdgcn_ballot_w64(two);
+                    const uint32_t tot = (uint32_t)(__popcll(m1) + __popcll(m2));
+                    if (fill + tot > MHITS)
+                        flush();
+                    if (has) {
+                        const unsigned long long po = oA[il], co = oB[jl];
+                        const uint32_t idx =
+                            fill + (uint32_t)(__popcll(m1 & lt) + __popcll(m2 & lt));
+                        hb[idx] = (po << qbits) | co;
+                        if (two)
+                            hb[idx + 1] = (co << qbits) | po;
+                    }
+                    fill += tot;
+                }
             }
         }
         v = vn;
