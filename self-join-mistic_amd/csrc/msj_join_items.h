@@ -141,6 +141,9 @@ __device__ inline void item_sym(const JobView &J, const JoinItem &w, double *qs,
     }
     wave_sync();
     const uint64_t qall = qn >= WAVE ? ~0ull : ((1ull << qn) - 1ull);
+    // row ids of the item's queries, query r in lane r (qn <= 64): an
+    // emitting query reads its id with a lane read, not a memory load
+    const uint32_t qperm = lane < qn ? J.perm[x0 + w.qbeg + lane] : 0u;
 
     // per-lane cursor into X's upper neighbour list, from the entry holding
     // the item's first candidate
@@ -213,7 +216,8 @@ __device__ inline void item_sym(const JobView &J, const JoinItem &w, double *qs,
             }
             if (am) {
                 const bool acc = (am >> lane) & 1ull;
-                H.emit(J, acc, p, c, acc ? (unsigned long long)J.perm[p] : 0ull, corig);
+                H.emit(J, acc, p, c, (unsigned long long)__builtin_amdgcn_readlane(qperm, qi),
+                       corig);
             }
         }
     }
@@ -237,6 +241,7 @@ __device__ inline void item_hd(const JobView &J, const JoinItem &w, double *qs, 
     const uint32_t x0 = J.nstart[X];
     const uint32_t qn = w.qend - w.qbeg;  // <= HQ
     const uint32_t q0 = x0 + w.qbeg;      // first query (node order)
+    const uint32_t qperm = lane < qn ? J.perm[q0 + lane] : 0u;  // query q's row id in lane q
     uint32_t a = first_entry(J, X, w.cbeg);
     uint32_t ybase = J.ucum[a];
     uint32_t Y = J.uadj[a];
@@ -336,7 +341,7 @@ __device__ inline void item_hd(const JobView &J, const JoinItem &w, double *qs, 
         for (uint32_t q = 0; q < qn; ++q) {
             const bool acc = (alive >> q) & 1u;
             const uint32_t p = q0 + q;
-            H.emit(J, acc, p, c, acc ? (unsigned long long)J.perm[p] : 0ull, corig);
+            H.emit(J, acc, p, c, (unsigned long long)__builtin_amdgcn_readlane(qperm, q), corig);
         }
     }
 }
