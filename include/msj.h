@@ -114,7 +114,9 @@ typedef struct msj_stats {
     uint64_t filter_violations;  /* of those, certain-accept/reject classes that
                                     FP64 contradicts (0 unless the bound is wrong) */
     uint32_t form_used;          /* msj_form the join actually ran                  */
-    uint32_t pad2;
+    uint32_t split_tiles_ppm;    /* split-BF16 form: tiles that ran the split
+                                    products, per million tiles (the others were
+                                    certain rejects under the BF16 bound)       */
     double   mfma_flops;         /* MFMA forms: matrix-core flops issued (tiles)    */
 } msj_stats;
 

@@ -53,6 +53,15 @@
 // split half tiles) are copied global -> LDS directly into a double buffer:
 // the next slab's copy is in flight while the current one runs on the
 // matrix cores.
+//
+// Split-BF16 full tiles are scored in BF16 first (high planes only, one
+// third of the products): a tile whose pairs are all certain rejects under
+// the BF16 bound (C of a BF16 accumulation of d products) is done; any
+// other tile (a pair that is not, a sampled item, NaN terms) adds the two
+// cross products over its slabs and is classified under the split-BF16
+// bound, as without this step.  The split bound covers the reordered
+// accumulation (any order of the FP32 additions).  On clustered data most
+// tiles pair rows of different clusters and stop after the BF16 pass.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -172,7 +181,8 @@ __global__ __launch_bounds__(256) void k_frag_prep(const double *__restrict__ xr
                                                    const uint32_t *__restrict__ pjoin,
                                                    uint32_t Ptot, double C,
                                                    uint4 *__restrict__ fh, uint4 *__restrict__ fl,
-                                                   float4 *__restrict__ thr)
+                                                   float4 *__restrict__ thr, double Cb,
+                                                   float2 *__restrict__ thrb)
 {
     const uint32_t r = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
     const uint32_t lane = threadIdx.x & (WAVE - 1);
@@ -209,16 +219,22 @@ __global__ __launch_bounds__(256) void k_frag_prep(const double *__restrict__ xr
         na += __shfl_xor(na, o, WAVE);
     if (lane == 0) {
         float4 t = make_float4(0.f, 0.f, 0.f, 0.f);
+        float2 tb = make_float2(0.f, 0.f);  // the same terms under the BF16 bound
         if (valid) {
             if (na <= 0x1p100) {
                 t.x = __double2float_ru((1.0 + C) * na * (1.0 + 0x1p-40));
                 t.y = __double2float_rd((1.0 - C) * na * (1.0 - 0x1p-40));
                 t.z = (float)na;
+                tb.x = __double2float_ru((1.0 + Cb) * na * (1.0 + 0x1p-40));
+                tb.y = __double2float_rd((1.0 - Cb) * na * (1.0 - 0x1p-40));
             } else {
                 t.x = t.y = t.z = __builtin_nanf("");
+                tb.x = tb.y = __builtin_nanf("");
             }
         }
         thr[r] = t;
+        if (thrb)
+            thrb[r] = tb;
     }
 }
 
@@ -325,7 +341,7 @@ __global__ __launch_bounds__(256) void k_sblk_fill(const uint32_t *__restrict__ 
 // a certain reject of a sampled work item (verified).
 //
 // Tile.  A workgroup (NW waves) computes a TM x TN tile (256 x 256, or a
-// 256 x 128 half tile with HALF); wave (wr, wc) the 128 x 64 block of rows
+// 256 x 128 half tile with HALF; REF: split full tiles, BF16 pass first); wave (wr, wc) the 128 x 64 block of rows
 // 128 wr.. and columns 64 wc.., as 4 x 2 MFMA 32x32 tiles (128 FP32
 // accumulators per lane).  Per slab of KL k-steps the workgroup copies the
 // row blocks of fragments (8 of A, TN / 32 of B; both planes when split)
@@ -342,10 +358,11 @@ __global__ __launch_bounds__(256) void k_sblk_fill(const uint32_t *__restrict__ 
 // tiles take the blocks one at a time through all passes (a band block's
 // accumulators go through a per-lane LDS column); full tiles run the fast
 // pass over all four blocks first.
-template <bool SPLIT, bool DIAG, bool HALF>
+template <bool SPLIT, bool DIAG, bool HALF, bool REF>
 __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
     const uint4 *__restrict__ fh, const uint4 *__restrict__ fl, uint32_t KS,
-    const float4 *__restrict__ thr, const uint32_t *__restrict__ porig,
+    const float4 *__restrict__ thr, const float2 *__restrict__ thrb,
+    const uint32_t *__restrict__ porig,
     const uint32_t *__restrict__ pjoin, const MItem *__restrict__ items, SlotMap sm,
     uint64_t per_xcd, float lo, float hi, int qbits, unsigned long long *__restrict__ out,
     unsigned long long cap, unsigned long long *__restrict__ ctr,
@@ -363,12 +380,15 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
     constexpr uint32_t NW = HALF ? 4 : 8;         // waves per workgroup
     constexpr uint32_t BHITS = 64;                // per-wave band staging
     static_assert(UNITS % NW == 0, "every wave issues the same number of copies");
+    static_assert(!REF || (SPLIT && !HALF), "BF16-first refinement: split full tiles only");
     // two slab buffers: the next slab is copied (global -> LDS, no VGPRs)
     // while the matrix cores work on the current one
     __shared__ uint4 sA[2][NP][SLA];
     __shared__ uint4 sB[2][NP][SLB];
     __shared__ float4 tA[TM], tB[TN];
     __shared__ __align__(16) float tAy[TM];  // tA[.].y, four rows per 16-byte read
+    // REF: the rows' reject terms under the BF16 bound (first pass)
+    __shared__ __align__(16) float tAyb[REF ? TM : 4], tByb[REF ? TN : 4];
     __shared__ uint32_t oA[TM], oB[TN], jA[TM], jB[TN];
     __shared__ unsigned long long hsh[NW][MHITS];
     __shared__ unsigned long long bsh[NW][BHITS];
@@ -468,9 +488,12 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
     // * KL + kstep, blocks 0..RBA-1 of A then RBB of B, is one wave-wide
     // 16-byte-per-lane load straight into LDS (the lane layout of a fragment
     // block is its memory layout); wave wv issues units wv, wv + NW, ...
-    auto copy_slab = [&](uint32_t rbA, uint32_t rbB, uint32_t k0, uint32_t buf) {
+    // (np = 1: the high plane only, the first pass of REF)
+    auto copy_slab = [&](uint32_t rbA, uint32_t rbB, uint32_t k0, uint32_t buf, uint32_t np) {
 #pragma unroll
         for (uint32_t m = 0; m < UNITS / NW; ++m) {
+            if (m * NP >= np * (UNITS / NW))
+                break;  // units of the low plane (plane is the slowest index)
             const uint32_t u = wv + NW * m;
             const uint32_t kk = u % KL, blk = (u / KL) % (RBA + RBB), pl = u / (KL * (RBA + RBB));
             const bool op = blk >= RBA;
@@ -487,25 +510,36 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
     // (half tiles: k < TN) or TM + k (full tiles); rows outside the item
     // (padding, the next node) get +inf terms
     float4 t4 = make_float4(0.f, 0.f, 0.f, 0.f), u4 = t4;
+    float tyb = 0.f, uyb = 0.f;  // REF: reject terms under the BF16 bound
     uint32_t o4 = 0, j4 = 0, ou = 0, ju = 0;
     auto fetch_terms = [&](const Slot &S, const MItem &w) {
         if (tid < TM) {
             const uint32_t ra = a0_of(S, w) + tid;
             t4 = thr[ra];
-            if (ra >= w.qend)
+            if (REF)
+                tyb = thrb[ra].y;
+            if (ra >= w.qend) {
                 t4 = make_float4(INF, INF, INF, INF);
+                tyb = INF;
+            }
             o4 = porig[ra];
             j4 = pjoin[ra];
         }
         if (HALF ? tid < TN : tid >= TM) {
             const uint32_t rb = b0_of(S, w) + (HALF ? tid : tid - TM);
             u4 = thr[rb];
-            if (rb >= w.cend)
+            if (REF)
+                uyb = thrb[rb].y;
+            if (rb >= w.cend) {
                 u4 = make_float4(INF, INF, INF, INF);
+                uyb = INF;
+            }
             ou = porig[rb];
             ju = pjoin[rb];
         }
     };
+    // planes copied for a tile's first pass (REF: the high plane only)
+    constexpr uint32_t NP0 = REF ? 1 : NP;
 
     // the first tile
     uint64_t v = blockIdx.x;
@@ -525,7 +559,7 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
     uint32_t buf = 0;  // LDS buffer of the current slab
     if (have) {
         fetch_terms(cs, w);
-        copy_slab(a0_of(cs, w) / 32, b0_of(cs, w) / 32, 0, 0);
+        copy_slab(a0_of(cs, w) / 32, b0_of(cs, w) / 32, 0, 0, NP0);
     }
     while (have) {
         __syncthreads();  // the previous tile is done with tA .. jB
@@ -534,12 +568,16 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
             oA[tid] = o4;
             jA[tid] = j4;
             tAy[tid] = t4.y;
+            if (REF)
+                tAyb[tid] = tyb;
         }
         if (HALF ? tid < TN : tid >= TM) {
             const uint32_t k = HALF ? tid : tid - TM;
             tB[k] = u4;
             oB[k] = ou;
             jB[k] = ju;
+            if (REF)
+                tByb[k] = uyb;
         }
         // the band counter is read every 32nd slot of a workgroup: one global
         // address polled by every workgroup on every tile is a hot spot
@@ -588,446 +626,521 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
         static_assert(SPLIT && HALF ? (KL == 1 && SLA * 16 == 8192 && SLB * 16 == 4096)
                                     : (KL == 2 && SLA * 16 == 16384 && SLB * 16 == TN * 64),
                       "fragment read offsets below");
-        for (uint32_t k0 = 0; k0 < KS; k0 += KL, buf ^= 1u) {
-            // this wave's copies of the slab have landed; after the barrier,
-            // everyone's have, and nobody still reads the other buffer
-            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-            __syncthreads();
-            if (k0 + KL < KS) {
-                copy_slab(rbA, rbB, k0 + KL, buf ^ 1u);  // in flight during this slab's MFMAs
-            } else {
-                // last slab: find the next tile (a skipped slot costs one more
-                // item load) and start its first slab and row terms
-                while (vn < vtotal) {
-                    if (valid(ns, nw)) {
-                        nhave = true;
-                        break;
+        // One pass over the k-slabs.  PH 0: the only pass; REF: PH 1 the high
+        // planes' products (BF16), PH 2 the cross products of a tile that
+        // needs them.
+        auto kpass = [&](auto phase) __attribute__((always_inline)) {
+            constexpr int PH = decltype(phase)::value;
+            for (uint32_t k0 = 0; k0 < KS; k0 += KL, buf ^= 1u) {
+                // this wave's copies of the slab have landed; after the barrier,
+                // everyone's have, and nobody still reads the other buffer
+                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+                __syncthreads();
+                if (k0 + KL < KS) {
+                    // in flight during this slab's MFMAs
+                    copy_slab(rbA, rbB, k0 + KL, buf ^ 1u, PH == 1 ? 1u : NP);
+                } else if (PH == 0) {
+                    // last slab: find the next tile (a skipped slot costs one more
+                    // item load) and start its first slab and row terms
+                    while (vn < vtotal) {
+                        if (valid(ns, nw)) {
+                            nhave = true;
+                            break;
+                        }
+                        vn += gridDim.x;
+                        if (vn >= vtotal)
+                            break;
+                        ns = slot_of(vn);
+                        if (ns.in)
+                            nw = items[ns.it];
                     }
-                    vn += gridDim.x;
-                    if (vn >= vtotal)
-                        break;
-                    ns = slot_of(vn);
-                    if (ns.in)
-                        nw = items[ns.it];
+                    if (nhave) {
+                        copy_slab(a0_of(ns, nw) / 32, b0_of(ns, nw) / 32, 0, buf ^ 1u, NP0);
+                        fetch_terms(ns, nw);
+                    }
                 }
-                if (nhave) {
-                    copy_slab(a0_of(ns, nw) / 32, b0_of(ns, nw) / 32, 0, buf ^ 1u);
-                    fetch_terms(ns, nw);
+#pragma unroll
+                for (uint32_t kk = 0; kk < KL; ++kk) {
+                    // the k-step's fragments in one block, waited for inside it
+                    // (outputs early-clobber: no read can overwrite an address);
+                    // the copy of the next slab stays in flight
+                    const uint32_t va = ldsA + buf * (NP * SLA * 16) + kk * 1024;
+                    const uint32_t vb = ldsB + buf * (NP * SLB * 16) + kk * 1024;
+                    uint4 a0, a1, a2, a3, b0, b1, l0, l1, l2, l3, m0, m1;
+                    if (SPLIT && HALF) {  // one k-step per slab: blocks 1 KiB apart
+                        asm volatile(
+                            "ds_read_b128 %0, %12\n"
+                            "ds_read_b128 %1, %12 offset:1024\n"
+                            "ds_read_b128 %2, %12 offset:2048\n"
+                            "ds_read_b128 %3, %12 offset:3072\n"
+                            "ds_read_b128 %4, %13\n"
+                            "ds_read_b128 %5, %13 offset:1024\n"
+                            "ds_read_b128 %6, %12 offset:8192\n"
+                            "ds_read_b128 %7, %12 offset:9216\n"
+                            "ds_read_b128 %8, %12 offset:10240\n"
+                            "ds_read_b128 %9, %12 offset:11264\n"
+                            "ds_read_b128 %10, %13 offset:4096\n"
+                            "ds_read_b128 %11, %13 offset:5120\n"
+                            "s_waitcnt lgkmcnt(0)"
+                            : "=&v"(a0), "=&v"(a1), "=&v"(a2), "=&v"(a3), "=&v"(b0), "=&v"(b1),
+                              "=&v"(l0), "=&v"(l1), "=&v"(l2), "=&v"(l3), "=&v"(m0), "=&v"(m1)
+                            : "v"(va), "v"(vb)
+                            : "memory");
+                    } else if (SPLIT && PH != 1) {  // two k-steps per slab: blocks 2 KiB apart
+                        asm volatile(
+                            "ds_read_b128 %0, %12\n"
+                            "ds_read_b128 %1, %12 offset:2048\n"
+                            "ds_read_b128 %2, %12 offset:4096\n"
+                            "ds_read_b128 %3, %12 offset:6144\n"
+                            "ds_read_b128 %4, %13\n"
+                            "ds_read_b128 %5, %13 offset:2048\n"
+                            "ds_read_b128 %6, %12 offset:16384\n"
+                            "ds_read_b128 %7, %12 offset:18432\n"
+                            "ds_read_b128 %8, %12 offset:20480\n"
+                            "ds_read_b128 %9, %12 offset:22528\n"
+                            "ds_read_b128 %10, %13 offset:16384\n"
+                            "ds_read_b128 %11, %13 offset:18432\n"
+                            "s_waitcnt lgkmcnt(0)"
+                            : "=&v"(a0), "=&v"(a1), "=&v"(a2), "=&v"(a3), "=&v"(b0), "=&v"(b1),
+                              "=&v"(l0), "=&v"(l1), "=&v"(l2), "=&v"(l3), "=&v"(m0), "=&v"(m1)
+                            : "v"(va), "v"(vb)
+                            : "memory");
+                    } else {
+                        asm volatile(
+                            "ds_read_b128 %0, %6\n"
+                            "ds_read_b128 %1, %6 offset:2048\n"
+                            "ds_read_b128 %2, %6 offset:4096\n"
+                            "ds_read_b128 %3, %6 offset:6144\n"
+                            "ds_read_b128 %4, %7\n"
+                            "ds_read_b128 %5, %7 offset:2048\n"
+                            "s_waitcnt lgkmcnt(0)"
+                            : "=&v"(a0), "=&v"(a1), "=&v"(a2), "=&v"(a3), "=&v"(b0), "=&v"(b1)
+                            : "v"(va), "v"(vb)
+                            : "memory");
+                        l0 = l1 = l2 = l3 = m0 = m1 = make_uint4(0, 0, 0, 0);
+                    }
+                    const bf16x8 ah[4] = {as_bf16x8(a0), as_bf16x8(a1), as_bf16x8(a2),
+                                          as_bf16x8(a3)};
+                    const bf16x8 bh[2] = {as_bf16x8(b0), as_bf16x8(b1)};
+                    if (PH != 2) {  // the second REF pass adds the cross products only
+#pragma unroll
+                        for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+                            for (int bj = 0; bj < 2; ++bj)
+                                acc[bi][bj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                                    ah[bi], bh[bj], acc[bi][bj], 0, 0, 0);
+                    }
+                    if (SPLIT && PH != 1) {
+                        const bf16x8 al[4] = {as_bf16x8(l0), as_bf16x8(l1), as_bf16x8(l2),
+                                              as_bf16x8(l3)};
+                        const bf16x8 bl[2] = {as_bf16x8(m0), as_bf16x8(m1)};
+#pragma unroll
+                        for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+                            for (int bj = 0; bj < 2; ++bj)
+                                acc[bi][bj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                                    ah[bi], bl[bj], acc[bi][bj], 0, 0, 0);
+#pragma unroll
+                        for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+                            for (int bj = 0; bj < 2; ++bj)
+                                acc[bi][bj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                                    al[bi], bh[bj], acc[bi][bj], 0, 0, 0);
+                    }
                 }
             }
+
+        };
+        bool refined = !REF;
+        if constexpr (!REF) {
+            kpass(std::integral_constant<int, 0>());
+        } else {
+            kpass(std::integral_constant<int, 1>());
+            // BF16 scores of the tile: if every pair is a certain reject
+            // under the BF16 bound, the tile is done; otherwise (or when
+            // sampled or NaN-scored) the cross products follow and the tile
+            // is classified under the split-BF16 bound
+            float tbb[2];
 #pragma unroll
-            for (uint32_t kk = 0; kk < KL; ++kk) {
-                // the k-step's fragments in one block, waited for inside it
-                // (outputs early-clobber: no read can overwrite an address);
-                // the copy of the next slab stays in flight
-                const uint32_t va = ldsA + buf * (NP * SLA * 16) + kk * 1024;
-                const uint32_t vb = ldsB + buf * (NP * SLB * 16) + kk * 1024;
-                uint4 a0, a1, a2, a3, b0, b1, l0, l1, l2, l3, m0, m1;
-                if (SPLIT && HALF) {  // one k-step per slab: blocks 1 KiB apart
-                    asm volatile(
-                        "ds_read_b128 %0, %12\n"
-                        "ds_read_b128 %1, %12 offset:1024\n"
-                        "ds_read_b128 %2, %12 offset:2048\n"
-                        "ds_read_b128 %3, %12 offset:3072\n"
-                        "ds_read_b128 %4, %13\n"
-                        "ds_read_b128 %5, %13 offset:1024\n"
-                        "ds_read_b128 %6, %12 offset:8192\n"
-                        "ds_read_b128 %7, %12 offset:9216\n"
-                        "ds_read_b128 %8, %12 offset:10240\n"
-                        "ds_read_b128 %9, %12 offset:11264\n"
-                        "ds_read_b128 %10, %13 offset:4096\n"
-                        "ds_read_b128 %11, %13 offset:5120\n"
-                        "s_waitcnt lgkmcnt(0)"
-                        : "=&v"(a0), "=&v"(a1), "=&v"(a2), "=&v"(a3), "=&v"(b0), "=&v"(b1),
-                          "=&v"(l0), "=&v"(l1), "=&v"(l2), "=&v"(l3), "=&v"(m0), "=&v"(m1)
-                        : "v"(va), "v"(vb)
-                        : "memory");
-                } else if (SPLIT) {  // two k-steps per slab: blocks 2 KiB apart
-                    asm volatile(
-                        "ds_read_b128 %0, %12\n"
-                        "ds_read_b128 %1, %12 offset:2048\n"
-                        "ds_read_b128 %2, %12 offset:4096\n"
-                        "ds_read_b128 %3, %12 offset:6144\n"
-                        "ds_read_b128 %4, %13\n"
-                        "ds_read_b128 %5, %13 offset:2048\n"
-                        "ds_read_b128 %6, %12 offset:16384\n"
-                        "ds_read_b128 %7, %12 offset:18432\n"
-                        "ds_read_b128 %8, %12 offset:20480\n"
-                        "ds_read_b128 %9, %12 offset:22528\n"
-                        "ds_read_b128 %10, %13 offset:16384\n"
-                        "ds_read_b128 %11, %13 offset:18432\n"
-                        "s_waitcnt lgkmcnt(0)"
-                        : "=&v"(a0), "=&v"(a1), "=&v"(a2), "=&v"(a3), "=&v"(b0), "=&v"(b1),
-                          "=&v"(l0), "=&v"(l1), "=&v"(l2), "=&v"(l3), "=&v"(m0), "=&v"(m1)
-                        : "v"(va), "v"(vb)
-                        : "memory");
-                } else {
-                    asm volatile(
-                        "ds_read_b128 %0, %6\n"
-                        "ds_read_b128 %1, %6 offset:2048\n"
-                        "ds_read_b128 %2, %6 offset:4096\n"
-                        "ds_read_b128 %3, %6 offset:6144\n"
-                        "ds_read_b128 %4, %7\n"
-                        "ds_read_b128 %5, %7 offset:2048\n"
-                        "s_waitcnt lgkmcnt(0)"
-                        : "=&v"(a0), "=&v"(a1), "=&v"(a2), "=&v"(a3), "=&v"(b0), "=&v"(b1)
-                        : "v"(va), "v"(vb)
-                        : "memory");
-                    l0 = l1 = l2 = l3 = m0 = m1 = make_uint4(0, 0, 0, 0);
-                }
-                const bf16x8 ah[4] = {as_bf16x8(a0), as_bf16x8(a1), as_bf16x8(a2), as_bf16x8(a3)};
-                const bf16x8 bh[2] = {as_bf16x8(b0), as_bf16x8(b1)};
+            for (int bj = 0; bj < 2; ++bj)
+                tbb[bj] = tByb[64 * wc + 32 * bj + (lane & 31)];
+            float m = INF;
 #pragma unroll
-                for (int bi = 0; bi < 4; ++bi)
+            for (int bi = 0; bi < 4; ++bi)
 #pragma unroll
-                    for (int bj = 0; bj < 2; ++bj)
-                        acc[bi][bj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-                            ah[bi], bh[bj], acc[bi][bj], 0, 0, 0);
-                if (SPLIT) {
-                    const bf16x8 al[4] = {as_bf16x8(l0), as_bf16x8(l1), as_bf16x8(l2),
-                                          as_bf16x8(l3)};
-                    const bf16x8 bl[2] = {as_bf16x8(m0), as_bf16x8(m1)};
+                for (int eg = 0; eg < 4; ++eg) {
+                    const uint32_t i0 = 128 * wr + 32 * bi + 8 * eg + 4 * h;
+                    const float4 ay4 = *(const float4 *)&tAyb[i0];
+                    const float ay[4] = {ay4.x, ay4.y, ay4.z, ay4.w};
 #pragma unroll
-                    for (int bi = 0; bi < 4; ++bi)
+                    for (int ei = 0; ei < 4; ++ei)
 #pragma unroll
                         for (int bj = 0; bj < 2; ++bj)
-                            acc[bi][bj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-                                ah[bi], bl[bj], acc[bi][bj], 0, 0, 0);
-#pragma unroll
-                    for (int bi = 0; bi < 4; ++bi)
-#pragma unroll
-                        for (int bj = 0; bj < 2; ++bj)
-                            acc[bi][bj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-                                al[bi], bh[bj], acc[bi][bj], 0, 0, 0);
+                            m = fminf(m, fmaf(-2.f, acc[bi][bj][4 * eg + ei], ay[ei] + tbb[bj]));
                 }
+            const bool need = sampled || tile_nan || !(m > hi);
+            if (__syncthreads_or(need) != 0) {
+                if (tid == 0)
+                    atomicAdd(&ctr[8], 1ull);
+                // both planes again from the first slab (buffer buf is free:
+                // the last slab used the other one)
+                copy_slab(rbA, rbB, 0, buf, NP);
+                kpass(std::integral_constant<int, 2>());
+                refined = true;
             }
         }
 
-        // ---- epilogue.  C[i][j] of MFMA tile (bi, bj) is in lane
-        // j + 32h, element e, with i = 8 (e / 4) + 4 h + e % 4.
-        // a half tile crossing the diagonal of a node with itself: pairs
-        // p <= c only (there cb >= qbeg: doff = cb - qbeg is 0 or TN)
-        const bool diag_tile = self && cb < qbeg + TM && qbeg < cb + TN;
-        const uint32_t doff = cb - qbeg;
-        float4 tb[2];
-#pragma unroll
-        for (int bj = 0; bj < 2; ++bj)
-            tb[bj] = tB[64 * wc + 32 * bj + (lane & 31)];
-        if constexpr (HALF) {
-            // half tiles: one 32-row block at a time through all passes
-            constexpr uint32_t NB = 2;    // 32-column MFMA blocks per wave
-            using Mask = uint32_t;        // 16 NB pair slots
-            const uint32_t c0 = 64 * wc;  // first tile column of this wave
-            __shared__ float gst[NW][8][WAVE];  // per-lane scores of band slots
-            bool cvalid[NB];
-#pragma unroll
-            for (int bj = 0; bj < (int)NB; ++bj)
-                cvalid[bj] = cb + c0 + 32 * bj + (lane & 31) < w.cend;
-            // band entries (rare): staged slot by slot, with their scores.
-            // Per 32 x 32 MFMA block and half (8 slots per lane) the lane's
-            // accumulators go to its own column of the wave's LDS stage, read
-            // back by slot index in a loop that is not unrolled.
-            auto band_run = [&](uint32_t bi, uint32_t bj, uint32_t hf, Mask bm, Mask amk) {
-                const uint32_t jl = c0 + 32 * bj + (lane & 31);
-                const float4 tbq = tB[jl];
-#pragma unroll 1
-                for (uint32_t j = 0; j < 8; ++j) {
-                    const uint32_t e = 8 * hf + j;
-                    const uint32_t il = 128 * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
-                    const bool push = (bm >> (NB * e + bj)) & 1u;
-                    const uint64_t mb = __builtin_amdgcn_ballot_w64(push);
-                    if (!mb)
-                        continue;
-                    const uint32_t k = (uint32_t)__popcll(mb);
-                    if (bfill + k > BHITS)
-                        bflush();
-                    const bool a_ok = (amk >> (NB * e + bj)) & 1u;
-                    bool inband = push;
-                    if (push) {
-                        const float4 ta = tA[il];
-                        const float G = gst[wv][j][lane];
-                        const uint32_t p = qbeg + il, c = cb + jl;
-                        const bool valid = p < w.qend && c < w.cend && (!diag_tile || p <= c);
-                        const bool r_ok = fmaf(-2.f, G, ta.y + tbq.y) > hi;
-                        inband = valid && !a_ok && !r_ok;
-                        const uint32_t cls = DIAG ? (a_ok ? 1u : (inband ? 0u : 2u)) : 0u;
-                        const uint32_t idx = bfill + (uint32_t)__popcll(mb & lt);
-                        bb[idx] = ((unsigned long long)jA[il] << 32) |
-                                  ((unsigned long long)cls << 30) | jB[jl];
-                        sb[idx] = fmaf(-2.f, G, ta.z + tbq.z);
-                    }
-                    bfill += k;
-                    b0fill += DIAG ? (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(inband)) : k;
-                }
-            };
-            // slots 8 hf .. 8 hf + 7 of column block bj: bits NB e + bj
-            constexpr Mask HALF_COL = 0x5555u;
-
-            // One 32-row block (bi) at a time, so that only its accumulators are
-            // read out of the matrix registers together.
-#pragma unroll
-            for (int bi = 0; bi < 4; ++bi) {
-                // fast pass: the block's smallest reject score (in a diagonal tile,
-                // of pairs p <= c); a block whose smallest score exceeds hi holds
-                // certain rejects only.  Scores of padding rows and of rows
-                // outside the item are +inf; NaN scores (NaN terms) are caught by
-                // tile_nan.
-                auto fast = [&](auto dg) __attribute__((always_inline)) {
-                    constexpr bool DT = decltype(dg)::value;
-                    float m = INF;
-#pragma unroll
-                    for (int eg = 0; eg < 4; ++eg) {
-                        const uint32_t i0 = 128 * wr + 32 * bi + 8 * eg + 4 * h;
-                        const float4 ay4 = *(const float4 *)&tAy[i0];
-                        const float ay[4] = {ay4.x, ay4.y, ay4.z, ay4.w};
-#pragma unroll
-                        for (int ei = 0; ei < 4; ++ei)
-#pragma unroll
-                            for (int bj = 0; bj < (int)NB; ++bj) {
-                                const float g = acc[bi][bj][4 * eg + ei];
-                                float sc = fmaf(-2.f, g, ay[ei] + tb[bj].y);
-                                if (DT && i0 + ei > c0 + 32 * bj + (lane & 31) + doff)
-                                    sc = INF;
-                                m = fminf(m, sc);
-                            }
-                    }
-                    return m;
-                };
-                const float mn = diag_tile ? fast(std::true_type()) : fast(std::false_type());
-                const bool need = sampled || tile_nan || !(mn > hi);
-                if (!__builtin_amdgcn_ballot_w64(need))
-                    continue;
-                // full pass: the block's 16 NB pairs per lane (slot NB e + bj)
-                // classified into lane bit masks of certain accepts and band
-                // entries (with DIAG also the verified classes); bitwise logic,
-                // no branches
-                Mask amask = 0, bmask = 0;
-#pragma unroll
-                for (int e = 0; e < 16; ++e) {
-                    const uint32_t il = 128 * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
-                    const float4 ta = tA[il];
-                    const uint32_t p = qbeg + il;
-                    const bool pv = p < w.qend;
-#pragma unroll
-                    for (int bj = 0; bj < (int)NB; ++bj) {
-                        const uint32_t c = cb + c0 + 32 * bj + (lane & 31);
-                        const bool valid = pv & cvalid[bj] & (!diag_tile | (p <= c));
-                        const float G = acc[bi][bj][e];
-                        const bool a_ok = valid & (fmaf(-2.f, G, ta.x + tb[bj].x) <= lo);
-                        const bool r_ok = fmaf(-2.f, G, ta.y + tb[bj].y) > hi;
-                        const bool inband = valid & !a_ok & !r_ok;
-                        const bool push = DIAG ? (inband | a_ok | (sampled & valid)) : inband;
-                        amask |= (Mask)a_ok << (NB * e + bj);
-                        bmask |= (Mask)push << (NB * e + bj);
-                    }
-                }
-                // band entries
-                if (__builtin_amdgcn_ballot_w64(bmask != 0)) {
-#pragma unroll
-                    for (int bj = 0; bj < (int)NB; ++bj)
-#pragma unroll
-                        for (int hf = 0; hf < 2; ++hf) {
-                            const Mask col = HALF_COL << (NB * 8 * hf + bj);
-                            if (__builtin_amdgcn_ballot_w64((bmask & col) != 0)) {
-#pragma unroll
-                                for (int j = 0; j < 8; ++j)
-                                    gst[wv][j][lane] = acc[bi][bj][8 * hf + j];
-                                band_run(bi, bj, hf, bmask, amask);
-                            }
-                        }
-                }
-                // certain accepts: both orders (the self pair once)
-                Mask am = amask;
-                while (true) {
-                    const bool has = am != 0;
-                    const uint64_t m1 = __builtin_amdgcn_ballot_w64(has);
-                    if (!m1)
-                        break;
-                    uint32_t il = 0, jl = 0;
-                    if (has) {
-                        const uint32_t k = (uint32_t)__builtin_ctzll((uint64_t)am);
-                        am &= am - 1;
-                        const uint32_t e = k / NB, bj = k % NB;
-                        il = 128 * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
-                        jl = c0 + 32 * bj + (lane & 31);
-                    }
-                    const bool two = has && (qbeg + il) != (cb + jl);
-                    const uint64_t m2 = __builtin_amdgcn_ballot_w64(two);
-                    const uint32_t tot = (uint32_t)(__popcll(m1) + __popcll(m2));
-                    if (fill + tot > MHITS)
-                        flush();
-                    if (has) {
-                        const unsigned long long po = oA[il], co = oB[jl];
-                        const uint32_t idx =
-                            fill + (uint32_t)(__popcll(m1 & lt) + __popcll(m2 & lt));
-                        hb[idx] = (po << qbits) | co;
-                        if (two)
-                            hb[idx + 1] = (co << qbits) | po;
-                    }
-                    fill += tot;
-                }
-            }
-        } else {
-            // fast pass: per 32-row block the smallest reject score of its pairs
-            // (in a diagonal tile of pairs p <= c); a block whose smallest score
-            // exceeds hi holds certain rejects only.  Scores of padding rows and
-            // of rows outside the item are +inf; NaN scores (NaN terms) are
-            // caught by tile_nan.
-            float mn[4];
-            auto fast = [&](auto dg) {
-                constexpr bool DT = decltype(dg)::value;
-#pragma unroll
-                for (int bi = 0; bi < 4; ++bi) {
-                    float m = INF;
-#pragma unroll
-                    for (int eg = 0; eg < 4; ++eg) {
-                        const uint32_t i0 = 128 * wr + 32 * bi + 8 * eg + 4 * h;
-                        const float4 ay4 = *(const float4 *)&tAy[i0];
-                        const float ay[4] = {ay4.x, ay4.y, ay4.z, ay4.w};
-#pragma unroll
-                        for (int ei = 0; ei < 4; ++ei)
-#pragma unroll
-                            for (int bj = 0; bj < 2; ++bj) {
-                                float sc = fmaf(-2.f, acc[bi][bj][4 * eg + ei], ay[ei] + tb[bj].y);
-                                if (DT && i0 + ei > 64 * wc + 32 * bj + (lane & 31) + doff)
-                                    sc = INF;
-                                m = fminf(m, sc);
-                            }
-                    }
-                    mn[bi] = m;
-                }
-            };
-            if (diag_tile)
-                fast(std::true_type());
-            else
-                fast(std::false_type());
-
-            // full pass, for the 32-row blocks (bi) that need it: the block's
-            // 32 pairs per lane (slot 2 e + bj) classified into lane bit masks of
-            // certain accepts and band entries (with DIAG also the verified
-            // classes)
-            bool cvalid[2];
+        if (refined) {
+            // ---- epilogue.  C[i][j] of MFMA tile (bi, bj) is in lane
+            // j + 32h, element e, with i = 8 (e / 4) + 4 h + e % 4.
+            // a half tile crossing the diagonal of a node with itself: pairs
+            // p <= c only (there cb >= qbeg: doff = cb - qbeg is 0 or TN)
+            const bool diag_tile = self && cb < qbeg + TM && qbeg < cb + TN;
+            const uint32_t doff = cb - qbeg;
+            float4 tb[2];
 #pragma unroll
             for (int bj = 0; bj < 2; ++bj)
-                cvalid[bj] = cb + 64 * wc + 32 * bj + (lane & 31) < w.cend;
-            uint32_t amask[4] = {0u, 0u, 0u, 0u}, bmask[4] = {0u, 0u, 0u, 0u};
+                tb[bj] = tB[64 * wc + 32 * bj + (lane & 31)];
+            if constexpr (HALF) {
+                // half tiles: one 32-row block at a time through all passes
+                constexpr uint32_t NB = 2;    // 32-column MFMA blocks per wave
+                using Mask = uint32_t;        // 16 NB pair slots
+                const uint32_t c0 = 64 * wc;  // first tile column of this wave
+                __shared__ float gst[NW][8][WAVE];  // per-lane scores of band slots
+                bool cvalid[NB];
 #pragma unroll
-            for (int bi = 0; bi < 4; ++bi) {
-                const bool need = sampled || tile_nan || !(mn[bi] > hi);
-                if (!__builtin_amdgcn_ballot_w64(need))
-                    continue;
-#pragma unroll
-                for (int e = 0; e < 16; ++e) {
-                    const uint32_t il = 128 * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
-                    const float4 ta = tA[il];
-                    const uint32_t p = qbeg + il;
-                    const bool pv = p < w.qend;
-#pragma unroll
-                    for (int bj = 0; bj < 2; ++bj) {
-                        const uint32_t c = cb + 64 * wc + 32 * bj + (lane & 31);
-                        const bool valid = pv && cvalid[bj] && (!diag_tile || p <= c);
-                        const float G = acc[bi][bj][e];
-                        const bool a_ok = valid && fmaf(-2.f, G, ta.x + tb[bj].x) <= lo;
-                        const bool r_ok = fmaf(-2.f, G, ta.y + tb[bj].y) > hi;
-                        const bool inband = valid && !a_ok && !r_ok;
-                        const bool push = DIAG ? (inband || a_ok || (sampled && valid)) : inband;
-                        amask[bi] |= (uint32_t)a_ok << (2 * e + bj);
-                        bmask[bi] |= (uint32_t)push << (2 * e + bj);
-                    }
-                }
-            }
-            // band entries (rare): staged slot by slot, with their scores; one
-            // 32-row block at a time (not unrolled: a copy of its accumulators)
+                for (int bj = 0; bj < (int)NB; ++bj)
+                    cvalid[bj] = cb + c0 + 32 * bj + (lane & 31) < w.cend;
+                // band entries (rare): staged slot by slot, with their scores.
+                // Per 32 x 32 MFMA block and half (8 slots per lane) the lane's
+                // accumulators go to its own column of the wave's LDS stage, read
+                // back by slot index in a loop that is not unrolled.
+                auto band_run = [&](uint32_t bi, uint32_t bj, uint32_t hf, Mask bm, Mask amk) {
+                    const uint32_t jl = c0 + 32 * bj + (lane & 31);
+                    const float4 tbq = tB[jl];
 #pragma unroll 1
-            for (int bi = 0; bi < 4; ++bi) {
-                // register arrays need constant indices
-                const uint32_t bm =
-                    bi == 0 ? bmask[0] : bi == 1 ? bmask[1] : bi == 2 ? bmask[2] : bmask[3];
-                if (!__builtin_amdgcn_ballot_w64(bm != 0))
-                    continue;
-                const uint32_t amk =
-                    bi == 0 ? amask[0] : bi == 1 ? amask[1] : bi == 2 ? amask[2] : amask[3];
-                floatx16 cur[2];
-                switch (bi) {
-                case 0:
-                    cur[0] = acc[0][0], cur[1] = acc[0][1];
-                    break;
-                case 1:
-                    cur[0] = acc[1][0], cur[1] = acc[1][1];
-                    break;
-                case 2:
-                    cur[0] = acc[2][0], cur[1] = acc[2][1];
-                    break;
-                default:
-                    cur[0] = acc[3][0], cur[1] = acc[3][1];
-                    break;
-                }
-#pragma unroll
-                for (int e = 0; e < 16; ++e) {
-                    const uint32_t il = 128 * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
-#pragma unroll
-                    for (int bj = 0; bj < 2; ++bj) {
-                        const bool push = (bm >> (2 * e + bj)) & 1u;
+                    for (uint32_t j = 0; j < 8; ++j) {
+                        const uint32_t e = 8 * hf + j;
+                        const uint32_t il = 128 * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
+                        const bool push = (bm >> (NB * e + bj)) & 1u;
                         const uint64_t mb = __builtin_amdgcn_ballot_w64(push);
                         if (!mb)
                             continue;
-                        const uint32_t jl = 64 * wc + 32 * bj + (lane & 31);
                         const uint32_t k = (uint32_t)__popcll(mb);
                         if (bfill + k > BHITS)
                             bflush();
-                        const bool a_ok = (amk >> (2 * e + bj)) & 1u;
+                        const bool a_ok = (amk >> (NB * e + bj)) & 1u;
                         bool inband = push;
                         if (push) {
                             const float4 ta = tA[il];
-                            const float G = cur[bj][e];
+                            const float G = gst[wv][j][lane];
                             const uint32_t p = qbeg + il, c = cb + jl;
                             const bool valid = p < w.qend && c < w.cend && (!diag_tile || p <= c);
-                            const bool r_ok = fmaf(-2.f, G, ta.y + tb[bj].y) > hi;
+                            const bool r_ok = fmaf(-2.f, G, ta.y + tbq.y) > hi;
                             inband = valid && !a_ok && !r_ok;
                             const uint32_t cls = DIAG ? (a_ok ? 1u : (inband ? 0u : 2u)) : 0u;
                             const uint32_t idx = bfill + (uint32_t)__popcll(mb & lt);
                             bb[idx] = ((unsigned long long)jA[il] << 32) |
                                       ((unsigned long long)cls << 30) | jB[jl];
-                            sb[idx] = fmaf(-2.f, G, ta.z + tb[bj].z);
+                            sb[idx] = fmaf(-2.f, G, ta.z + tbq.z);
                         }
                         bfill += k;
-                        b0fill += DIAG ? (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(inband))
-                                       : k;
+                        b0fill +=
+                            DIAG ? (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(inband)) : k;
+                    }
+                };
+                // slots 8 hf .. 8 hf + 7 of column block bj: bits NB e + bj
+                constexpr Mask HALF_COL = 0x5555u;
+
+                // One 32-row block (bi) at a time, so that only its accumulators are
+                // read out of the matrix registers together.
+#pragma unroll
+                for (int bi = 0; bi < 4; ++bi) {
+                    // fast pass: the block's smallest reject score (in a diagonal tile,
+                    // of pairs p <= c); a block whose smallest score exceeds hi holds
+                    // certain rejects only.  Scores of padding rows and of rows
+                    // outside the item are +inf; NaN scores (NaN terms) are caught by
+                    // tile_nan.
+                    auto fast = [&](auto dg) __attribute__((always_inline)) {
+                        constexpr bool DT = decltype(dg)::value;
+                        float m = INF;
+#pragma unroll
+                        for (int eg = 0; eg < 4; ++eg) {
+                            const uint32_t i0 = 128 * wr + 32 * bi + 8 * eg + 4 * h;
+                            const float4 ay4 = *(const float4 *)&tAy[i0];
+                            const float ay[4] = {ay4.x, ay4.y, ay4.z, ay4.w};
+#pragma unroll
+                            for (int ei = 0; ei < 4; ++ei)
+#pragma unroll
+                                for (int bj = 0; bj < (int)NB; ++bj) {
+                                    const float g = acc[bi][bj][4 * eg + ei];
+                                    float sc = fmaf(-2.f, g, ay[ei] + tb[bj].y);
+                                    if (DT && i0 + ei > c0 + 32 * bj + (lane & 31) + doff)
+                                        sc = INF;
+                                    m = fminf(m, sc);
+                                }
+                        }
+                        return m;
+                    };
+                    const float mn = diag_tile ? fast(std::true_type()) : fast(std::false_type());
+                    const bool need = sampled || tile_nan || !(mn > hi);
+                    if (!__builtin_amdgcn_ballot_w64(need))
+                        continue;
+                    // full pass: the block's 16 NB pairs per lane (slot NB e + bj)
+                    // classified into lane bit masks of certain accepts and band
+                    // entries (with DIAG also the verified classes); bitwise logic,
+                    // no branches
+                    Mask amask = 0, bmask = 0;
+#pragma unroll
+                    for (int e = 0; e < 16; ++e) {
+                        const uint32_t il = 128 * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
+                        const float4 ta = tA[il];
+                        const uint32_t p = qbeg + il;
+                        const bool pv = p < w.qend;
+#pragma unroll
+                        for (int bj = 0; bj < (int)NB; ++bj) {
+                            const uint32_t c = cb + c0 + 32 * bj + (lane & 31);
+                            const bool valid = pv & cvalid[bj] & (!diag_tile | (p <= c));
+                            const float G = acc[bi][bj][e];
+                            const bool a_ok = valid & (fmaf(-2.f, G, ta.x + tb[bj].x) <= lo);
+                            const bool r_ok = fmaf(-2.f, G, ta.y + tb[bj].y) > hi;
+                            const bool inband = valid & !a_ok & !r_ok;
+                            const bool push = DIAG ? (inband | a_ok | (sampled & valid)) : inband;
+                            amask |= (Mask)a_ok << (NB * e + bj);
+                            bmask |= (Mask)push << (NB * e + bj);
+                        }
+                    }
+                    // band entries
+                    if (__builtin_amdgcn_ballot_w64(bmask != 0)) {
+#pragma unroll
+                        for (int bj = 0; bj < (int)NB; ++bj)
+#pragma unroll
+                            for (int hf = 0; hf < 2; ++hf) {
+                                const Mask col = HALF_COL << (NB * 8 * hf + bj);
+                                if (__builtin_amdgcn_ballot_w64((bmask & col) != 0)) {
+#pragma unroll
+                                    for (int j = 0; j < 8; ++j)
+                                        gst[wv][j][lane] = acc[bi][bj][8 * hf + j];
+                                    band_run(bi, bj, hf, bmask, amask);
+                                }
+                            }
+                    }
+                    // certain accepts: both orders (the self pair once)
+                    Mask am = amask;
+                    while (true) {
+                        const bool has = am != 0;
+                        const uint64_t m1 = __builtin_amdgcn_ballot_w64(has);
+                        if (!m1)
+                            break;
+                        uint32_t il = 0, jl = 0;
+                        if (has) {
+                            const uint32_t k = (uint32_t)__builtin_ctzll((uint64_t)am);
+                            am &= am - 1;
+                            const uint32_t e = k / NB, bj = k % NB;
+                            il = 128 * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
+                            jl = c0 + 32 * bj + (lane & 31);
+                        }
+                        const bool two = has && (qbeg + il) != (cb + jl);
+                        const uint64_t m2 = __builtin_amdgcn_ballot_w64(two);
+                        const uint32_t tot = (uint32_t)(__popcll(m1) + __popcll(m2));
+                        if (fill + tot > MHITS)
+                            flush();
+                        if (has) {
+                            const unsigned long long po = oA[il], co = oB[jl];
+                            const uint32_t idx =
+                                fill + (uint32_t)(__popcll(m1 & lt) + __popcll(m2 & lt));
+                            hb[idx] = (po << qbits) | co;
+                            if (two)
+                                hb[idx + 1] = (co << qbits) | po;
+                        }
+                        fill += tot;
+                    }
+                }
+            } else {
+                // fast pass: per 32-row block the smallest reject score of its pairs
+                // (in a diagonal tile of pairs p <= c); a block whose smallest score
+                // exceeds hi holds certain rejects only.  Scores of padding rows and
+                // of rows outside the item are +inf; NaN scores (NaN terms) are
+                // caught by tile_nan.
+                float mn[4];
+                auto fast = [&](auto dg) {
+                    constexpr bool DT = decltype(dg)::value;
+#pragma unroll
+                    for (int bi = 0; bi < 4; ++bi) {
+                        float m = INF;
+#pragma unroll
+                        for (int eg = 0; eg < 4; ++eg) {
+                            const uint32_t i0 = 128 * wr + 32 * bi + 8 * eg + 4 * h;
+                            const float4 ay4 = *(const float4 *)&tAy[i0];
+                            const float ay[4] = {ay4.x, ay4.y, ay4.z, ay4.w};
+#pragma unroll
+                            for (int ei = 0; ei < 4; ++ei)
+#pragma unroll
+                                for (int bj = 0; bj < 2; ++bj) {
+                                    float sc =
+                                        fmaf(-2.f, acc[bi][bj][4 * eg + ei], ay[ei] + tb[bj].y);
+                                    if (DT && i0 + ei > 64 * wc + 32 * bj + (lane & 31) + doff)
+                                        sc = INF;
+                                    m = fminf(m, sc);
+                                }
+                        }
+                        mn[bi] = m;
+                    }
+                };
+                if (diag_tile)
+                    fast(std::true_type());
+                else
+                    fast(std::false_type());
+
+                // full pass, for the 32-row blocks (bi) that need it: the block's
+                // 32 pairs per lane (slot 2 e + bj) classified into lane bit masks of
+                // certain accepts and band entries (with DIAG also the verified
+                // classes)
+                bool cvalid[2];
+#pragma unroll
+                for (int bj = 0; bj < 2; ++bj)
+                    cvalid[bj] = cb + 64 * wc + 32 * bj + (lane & 31) < w.cend;
+                uint32_t amask[4] = {0u, 0u, 0u, 0u}, bmask[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+                for (int bi = 0; bi < 4; ++bi) {
+                    const bool need = sampled || tile_nan || !(mn[bi] > hi);
+                    if (!__builtin_amdgcn_ballot_w64(need))
+                        continue;
+#pragma unroll
+                    for (int e = 0; e < 16; ++e) {
+                        const uint32_t il = 128 * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
+                        const float4 ta = tA[il];
+                        const uint32_t p = qbeg + il;
+                        const bool pv = p < w.qend;
+#pragma unroll
+                        for (int bj = 0; bj < 2; ++bj) {
+                            const uint32_t c = cb + 64 * wc + 32 * bj + (lane & 31);
+                            const bool valid = pv && cvalid[bj] && (!diag_tile || p <= c);
+                            const float G = acc[bi][bj][e];
+                            const bool a_ok = valid && fmaf(-2.f, G, ta.x + tb[bj].x) <= lo;
+                            const bool r_ok = fmaf(-2.f, G, ta.y + tb[bj].y) > hi;
+                            const bool inband = valid && !a_ok && !r_ok;
+                            const bool push =
+                                DIAG ? (inband || a_ok || (sampled && valid)) : inband;
+                            amask[bi] |= (uint32_t)a_ok << (2 * e + bj);
+                            bmask[bi] |= (uint32_t)push << (2 * e + bj);
+                        }
+                    }
+                }
+                // band entries (rare): staged slot by slot, with their scores; one
+                // 32-row block at a time (not unrolled: a copy of its accumulators)
+#pragma unroll 1
+                for (int bi = 0; bi < 4; ++bi) {
+                    // register arrays need constant indices
+                    const uint32_t bm =
+                        bi == 0 ? bmask[0] : bi == 1 ? bmask[1] : bi == 2 ? bmask[2] : bmask[3];
+                    if (!__builtin_amdgcn_ballot_w64(bm != 0))
+                        continue;
+                    const uint32_t amk =
+                        bi == 0 ? amask[0] : bi == 1 ? amask[1] : bi == 2 ? amask[2] : amask[3];
+                    floatx16 cur[2];
+                    switch (bi) {
+                    case 0:
+                        cur[0] = acc[0][0], cur[1] = acc[0][1];
+                        break;
+                    case 1:
+                        cur[0] = acc[1][0], cur[1] = acc[1][1];
+                        break;
+                    case 2:
+                        cur[0] = acc[2][0], cur[1] = acc[2][1];
+                        break;
+                    default:
+                        cur[0] = acc[3][0], cur[1] = acc[3][1];
+                        break;
+                    }
+#pragma unroll
+                    for (int e = 0; e < 16; ++e) {
+                        const uint32_t il = 128 * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
+#pragma unroll
+                        for (int bj = 0; bj < 2; ++bj) {
+                            const bool push = (bm >> (2 * e + bj)) & 1u;
+                            const uint64_t mb = __builtin_amdgcn_ballot_w64(push);
+                            if (!mb)
+                                continue;
+                            const uint32_t jl = 64 * wc + 32 * bj + (lane & 31);
+                            const uint32_t k = (uint32_t)__popcll(mb);
+                            if (bfill + k > BHITS)
+                                bflush();
+                            const bool a_ok = (amk >> (2 * e + bj)) & 1u;
+                            bool inband = push;
+                            if (push) {
+                                const float4 ta = tA[il];
+                                const float G = cur[bj][e];
+                                const uint32_t p = qbeg + il, c = cb + jl;
+                                const bool valid =
+                                    p < w.qend && c < w.cend && (!diag_tile || p <= c);
+                                const bool r_ok = fmaf(-2.f, G, ta.y + tb[bj].y) > hi;
+                                inband = valid && !a_ok && !r_ok;
+                                const uint32_t cls = DIAG ? (a_ok ? 1u : (inband ? 0u : 2u)) : 0u;
+                                const uint32_t idx = bfill + (uint32_t)__popcll(mb & lt);
+                                bb[idx] = ((unsigned long long)jA[il] << 32) |
+                                          ((unsigned long long)cls << 30) | jB[jl];
+                                sb[idx] = fmaf(-2.f, G, ta.z + tb[bj].z);
+                            }
+                            bfill += k;
+                            b0fill += DIAG ? (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(inband))
+                                           : k;
+                        }
+                    }
+                }
+                // certain accepts: both orders (the self pair once)
+#pragma unroll
+                for (int bi = 0; bi < 4; ++bi) {
+                    uint32_t am = amask[bi];
+                    while (true) {
+                        const bool has = am != 0;
+                        const uint64_t m1 = __builtin_amdgcn_ballot_w64(has);
+                        if (!m1)
+                            break;
+                        uint32_t il = 0, jl = 0;
+                        if (has) {
+                            const uint32_t k = (uint32_t)__builtin_ctz(am);
+                            am &= am - 1;
+                            const uint32_t e = k >> 1, bj = k & 1u;
+                            il = 128 * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
+                            jl = 64 * wc + 32 * bj + (lane & 31);
+                        }
+                        const bool two = has && (qbeg + il) != (cb + jl);
+                        const uint64_t m2 = __builtin_amdgcn_ballot_w64(two);
+                        const uint32_t tot = (uint32_t)(__popcll(m1) + __popcll(m2));
+                        if (fill + tot > MHITS)
+                            flush();
+                        if (has) {
+                            const unsigned long long po = oA[il], co = oB[jl];
+                            const uint32_t idx =
+                                fill + (uint32_t)(__popcll(m1 & lt) + __popcll(m2 & lt));
+                            hb[idx] = (po << qbits) | co;
+                            if (two)
+                                hb[idx + 1] = (co << qbits) | po;
+                        }
+                        fill += tot;
                     }
                 }
             }
-            // certain accepts: both orders (the self pair once)
-#pragma unroll
-            for (int bi = 0; bi < 4; ++bi) {
-                uint32_t am = amask[bi];
-                while (true) {
-                    const bool has = am != 0;
-                    const uint64_t m1 = __builtin_amdgcn_ballot_w64(has);
-                    if (!m1)
-                        break;
-                    uint32_t il = 0, jl = 0;
-                    if (has) {
-                        const uint32_t k = (uint32_t)__builtin_ctz(am);
-                        am &= am - 1;
-                        const uint32_t e = k >> 1, bj = k & 1u;
-                        il = 128 * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
-                        jl = 64 * wc + 32 * bj + (lane & 31);
-                    }
-                    const bool two = has && (qbeg + il) != (cb + jl);
-                    const uint64_t m2 = __builtin_amdgcn_ballot_w64(two);
-                    const uint32_t tot = (uint32_t)(__popcll(m1) + __popcll(m2));
-                    if (fill + tot > MHITS)
-                        flush();
-                    if (has) {
-                        const unsigned long long po = oA[il], co = oB[jl];
-                        const uint32_t idx =
-                            fill + (uint32_t)(__popcll(m1 & lt) + __popcll(m2 & lt));
-                        hb[idx] = (po << qbits) | co;
-                        if (two)
-                            hb[idx + 1] = (co << qbits) | po;
-                    }
-                    fill += tot;
+        }
+        if constexpr (REF) {
+            // the next tile: find it (a skipped slot costs one more item
+            // load), start its first slab (high plane) and its row terms
+            while (vn < vtotal) {
+                if (valid(ns, nw)) {
+                    nhave = true;
+                    break;
                 }
+                vn += gridDim.x;
+                if (vn >= vtotal)
+                    break;
+                ns = slot_of(vn);
+                if (ns.in)
+                    nw = items[ns.it];
+            }
+            if (nhave) {
+                copy_slab(a0_of(ns, nw) / 32, b0_of(ns, nw) / 32, 0, buf, NP0);
+                fetch_terms(ns, nw);
             }
         }
         v = vn;
@@ -1239,12 +1352,18 @@ void MfmaRunner::prepare(const Index &ix, const JoinRunner &J, const double *xr,
     if (split)
         fl.ensure(nfrag);
     thr.ensure(Ptot);
+    // split-BF16 full tiles score every tile in BF16 first (refine_tiles):
+    // they also need the rows' terms under the BF16 bound
+    const bool refine = split && KS > KS_HALF;
+    if (refine)
+        thrb.ensure(Ptot);
+    const double Cb = bound_c(d, false);
     if (split)
         LAUNCH(k_frag_prep<true>, grid_for(Ptot, 4), 256, xr, ds, KS, mu.p, pjoin.p, Ptot, C,
-               fh.p, fl.p, thr.p);
+               fh.p, fl.p, thr.p, Cb, refine ? thrb.p : nullptr);
     else
         LAUNCH(k_frag_prep<false>, grid_for(Ptot, 4), 256, xr, ds, KS, mu.p, pjoin.p, Ptot, C,
-               fh.p, fh.p, thr.p);
+               fh.p, fh.p, thr.p, Cb, nullptr);
 
     // work items: super-blocks over the upper neighbour lists
     uint32_t E = 0;
@@ -1302,7 +1421,7 @@ uint64_t MfmaRunner::run(const Index &ix, JoinRunner &J, const double *xr, uint3
     // BF16 band budget: beyond it the BF16 bound is too loose for this data
     // and the join is rerun with split-BF16 (the kernel stops early)
     const uint64_t bf16_budget = std::max<uint64_t>(16ull * n, 1ull << 22);
-    ctr.ensure(8);
+    ctr.ensure(9);
     if (J.pairs.cap < cap_hint)
         J.pairs.ensure(cap_hint);
     *kernel_ms = 0.f;
@@ -1329,18 +1448,19 @@ uint64_t MfmaRunner::run(const Index &ix, JoinRunner &J, const double *xr, uint3
         const uint64_t per_xcd = (slots + 7) / 8;
         const uint32_t grid =
             (uint32_t)std::max<uint64_t>(8, std::min<uint64_t>(per_xcd * 8, MAX_GRID));
-        LAUNCH(k_zero_ctr, 1, 64, ctr.p, 8u);
+        LAUNCH(k_zero_ctr, 1, 64, ctr.p, 9u);
         MSJ_HIP(hipEventRecord(k0, s));
         if (nitems) {
 #define MSJ_MJ(SP, DG)                                                                          \
     do {                                                                                        \
         if (half)                                                                               \
-            MSJ_MJH(SP, DG, true, 256);                                                         \
+            MSJ_MJH(SP, DG, true, false, 256);                                                  \
         else                                                                                    \
-            MSJ_MJH(SP, DG, false, 512);                                                        \
+            MSJ_MJH(SP, DG, false, SP, 512);                                                    \
     } while (0)
-#define MSJ_MJH(SP, DG, HF, NT)                                                                 \
-    LAUNCH((k_mfma_join<SP, DG, HF>), grid, NT, fh.p, SP ? fl.p : fh.p, KS, thr.p, porig.p, pjoin.p, \
+#define MSJ_MJH(SP, DG, HF, RF, NT)                                                             \
+    LAUNCH((k_mfma_join<SP, DG, HF, RF>), grid, NT, fh.p, SP ? fl.p : fh.p, KS, thr.p,          \
+           RF ? thrb.p : nullptr, porig.p, pjoin.p,                                             \
            items.p, smap, per_xcd, lo, hi, qbits, (unsigned long long *)J.pairs.p,              \
            (unsigned long long)J.pairs.cap, ctr.p, (unsigned long long *)band.p, bscore.p,      \
            (unsigned long long)band.cap, sample_every, (unsigned long long)budget)
@@ -1370,7 +1490,7 @@ uint64_t MfmaRunner::run(const Index &ix, JoinRunner &J, const double *xr, uint3
             *launches += 2;
         }
         MSJ_HIP(hipEventRecord(k1, s));
-        unsigned long long h[8];
+        unsigned long long h[9];
         MSJ_HIP(hipMemcpyAsync(h, ctr.p, sizeof(h), hipMemcpyDeviceToHost, s));
         MSJ_HIP(hipStreamSynchronize(s));
         float ms = 0.f;
@@ -1394,7 +1514,17 @@ uint64_t MfmaRunner::run(const Index &ix, JoinRunner &J, const double *xr, uint3
             st->filter_checked = h[3];
             st->filter_violations = h[4];
             st->form_used = split ? MSJ_FORM_MFMA_SPLIT : MSJ_FORM_MFMA_BF16;
-            st->mfma_flops = (double)tiles * TM * TM * 2.0 * (16.0 * KS) * (split ? 3.0 : 1.0);
+            // split full tiles: BF16 products on every tile, the cross
+            // products on the tiles that needed them (h[8])
+            const double tile_flops = (double)TM * TM * 2.0 * (16.0 * KS);
+            const bool refine = split && !half;
+            st->mfma_flops = !split ? tiles * tile_flops
+                             : refine ? tiles * tile_flops + (double)h[8] * tile_flops * 2.0
+                                      : tiles * tile_flops * 3.0;
+            st->split_tiles_ppm = !split ? 0u
+                                  : refine ? (uint32_t)std::min<uint64_t>(
+                                                 1000000ull, tiles ? h[8] * 1000000ull / tiles : 0)
+                                           : 1000000u;
             return R;
         }
         if (nband > band.cap) {
