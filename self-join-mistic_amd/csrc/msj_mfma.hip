@@ -626,6 +626,7 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
         static_assert(SPLIT && HALF ? (KL == 1 && SLA * 16 == 8192 && SLB * 16 == 4096)
                                     : (KL == 2 && SLA * 16 == 16384 && SLB * 16 == TN * 64),
                       "fragment read offsets below");
+        bool wneed = true;  // REF: this wave's 128 x 64 block needs the cross products
         // One pass over the k-slabs.  PH 0: the only pass; REF: PH 1 the high
         // planes' products (BF16), PH 2 the cross products of a tile that
         // needs them.
@@ -659,8 +660,11 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
                         fetch_terms(ns, nw);
                     }
                 }
+                // the second REF pass: only waves whose block needs the cross
+                // products read fragments and issue MFMAs (the others free the
+                // matrix cores of their SIMD); all copy and synchronise
 #pragma unroll
-                for (uint32_t kk = 0; kk < KL; ++kk) {
+                for (uint32_t kk = 0; kk < (PH == 2 && !wneed ? 0u : KL); ++kk) {
                     // the k-step's fragments in one block, waited for inside it
                     // (outputs early-clobber: no read can overwrite an address);
                     // the copy of the next slab stays in flight
@@ -779,6 +783,7 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
                             m = fminf(m, fmaf(-2.f, acc[bi][bj][4 * eg + ei], ay[ei] + tbb[bj]));
                 }
             const bool need = sampled || tile_nan || !(m > hi);
+            wneed = __builtin_amdgcn_ballot_w64(need) != 0;
             if (__syncthreads_or(need) != 0) {
                 if (tid == 0)
                     atomicAdd(&ctr[8], 1ull);
@@ -790,7 +795,8 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
             }
         }
 
-        if (refined) {
+        // a wave whose block the BF16 pass settled holds certain rejects only
+        if (refined && wneed) {
             // ---- epilogue.  C[i][j] of MFMA tile (bi, bj) is in lane
             // j + 32h, element e, with i = 8 (e / 4) + 4 h + e % 4.
             // a half tile crossing the diagonal of a node with itself: pairs
