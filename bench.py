@@ -302,10 +302,10 @@ def main():
     else:
         # useful matrix-core work: every unordered candidate pair (self pairs
         # included) scored over d dimensions, 2 flops per product; split-BF16
-        # adds 2 cross products per dimension on the tiles that need them
-        # (all tiles up to d = 128; above, those not settled by their BF16
-        # scores)
-        mult = 1.0 + 2.0 * st["split_tiles_ppm"] / 1e6 if st["form_used"] == 2 else 1.0
+        # adds 2 cross products per dimension where they are needed (every
+        # pair up to d = 128; above, the wave blocks of tiles not settled by
+        # their BF16 scores)
+        mult = 1.0 + 2.0 * st["split_share_ppm"] / 1e6 if st["form_used"] == 2 else 1.0
         flops = (st["num_calcs"] + cfg["n"]) / 2.0 * cfg["d"] * 2.0 * mult
         peak = BF16_PEAK_TFLOPS
         kname = "k_mfma_join + k_band_check (BF16 MFMA; MI355X dense BF16 peak)"
@@ -344,7 +344,7 @@ def main():
             "sampled_pairs_checked": rs["filter_checked"],
             "violations": rs["filter_violations"],
             "mfma_flops_issued": st["mfma_flops"],
-            "split_tiles_fraction": st["split_tiles_ppm"] / 1e6,
+            "split_share": st["split_share_ppm"] / 1e6,
             "mfma_util": st["mfma_flops"] / (k_ms / 1e3) / 1e12 / BF16_PEAK_TFLOPS,
         }
     if not args.no_cpu_baseline:

@@ -114,9 +114,9 @@ typedef struct msj_stats {
     uint64_t filter_violations;  /* of those, certain-accept/reject classes that
                                     FP64 contradicts (0 unless the bound is wrong) */
     uint32_t form_used;          /* msj_form the join actually ran                  */
-    uint32_t split_tiles_ppm;    /* split-BF16 form: tiles that ran the split
-                                    products, per million tiles (the others were
-                                    certain rejects under the BF16 bound)       */
+    uint32_t split_share_ppm;    /* split-BF16 form: share of the tiles' pairs
+                                    whose cross products ran, per million (the
+                                    others were certain rejects in BF16)        */
     double   mfma_flops;         /* MFMA forms: matrix-core flops issued (tiles)    */
 } msj_stats;
 

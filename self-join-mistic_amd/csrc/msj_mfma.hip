@@ -785,8 +785,8 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
             const bool need = sampled || tile_nan || !(m > hi);
             wneed = __builtin_amdgcn_ballot_w64(need) != 0;
             if (__syncthreads_or(need) != 0) {
-                if (tid == 0)
-                    atomicAdd(&ctr[8], 1ull);
+                if (wneed && lane == 0)
+                    atomicAdd(&ctr[8], 1ull);  // wave blocks that take the cross products
                 // both planes again from the first slab (buffer buf is free:
                 // the last slab used the other one)
                 copy_slab(rbA, rbB, 0, buf, NP);
@@ -1521,16 +1521,18 @@ uint64_t MfmaRunner::run(const Index &ix, JoinRunner &J, const double *xr, uint3
             st->filter_violations = h[4];
             st->form_used = split ? MSJ_FORM_MFMA_SPLIT : MSJ_FORM_MFMA_BF16;
             // split full tiles: BF16 products on every tile, the cross
-            // products on the tiles that needed them (h[8])
+            // products on the 128 x 64 wave blocks (8 per tile) that needed
+            // them (h[8])
             const double tile_flops = (double)TM * TM * 2.0 * (16.0 * KS);
             const bool refine = split && !half;
             st->mfma_flops = !split ? tiles * tile_flops
-                             : refine ? tiles * tile_flops + (double)h[8] * tile_flops * 2.0
+                             : refine ? tiles * tile_flops + (double)h[8] * tile_flops / 4.0
                                       : tiles * tile_flops * 3.0;
-            st->split_tiles_ppm = !split ? 0u
-                                  : refine ? (uint32_t)std::min<uint64_t>(
-                                                 1000000ull, tiles ? h[8] * 1000000ull / tiles : 0)
-                                           : 1000000u;
+            st->split_share_ppm =
+                !split ? 0u
+                : refine ? (uint32_t)std::min<uint64_t>(
+                               1000000ull, tiles ? h[8] * 1000000ull / (8ull * tiles) : 0)
+                         : 1000000u;
             return R;
         }
         if (nband > band.cap) {

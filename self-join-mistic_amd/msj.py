@@ -77,7 +77,7 @@ class Stats(C.Structure):
         ("filter_checked", C.c_uint64),
         ("filter_violations", C.c_uint64),
         ("form_used", C.c_uint32),
-        ("split_tiles_ppm", C.c_uint32),
+        ("split_share_ppm", C.c_uint32),
         ("mfma_flops", C.c_double),
     ]
 
@@ -226,7 +226,7 @@ class JoinResult:
             "max_score_err": st.max_score_err, "mean_score_err": st.mean_score_err,
             "filter_checked": st.filter_checked, "filter_violations": st.filter_violations,
             "form_used": st.form_used, "mfma_flops": st.mfma_flops,
-            "split_tiles_ppm": st.split_tiles_ppm,
+            "split_share_ppm": st.split_share_ppm,
         }
         self.on_device = bool(r.on_device)
         if copy_arrays and not r.on_device:
