@@ -84,6 +84,25 @@ def test_filter_fp32_384(form, msj_mod, oracle_mod):
     _assert_same(got, ref)
 
 
+def test_filter_bf16_first_settles_tiles(msj_mod, oracle_mod):
+    """Split-BF16 above d = 128: tiles between rows of different clusters are
+    settled by their BF16 scores, the others take the cross products; the
+    pair set equals the oracle's and the sampled classes hold in FP64."""
+    x = _prep(datagen.clusters(12000, 384, seed=33, ncl=60), msj_mod)
+    eps = 1.26
+    _announce("fp32 clusters 12000x384 BF16 first", x, eps, "split")
+    got = msj_mod.self_join(x, eps, form=msj_mod.FORM_MFMA_SPLIT,
+                            storage=msj_mod.STORAGE_F32, count_work=1)
+    ref = oracle_mod.self_join(x.astype(np.float64), eps)
+    _assert_same(got, ref)
+    st = got.stats
+    assert st["form_used"] == msj_mod.FORM_MFMA_SPLIT
+    assert 0 < st["split_share_ppm"] < 1000000  # some settled in BF16, some not
+    assert st["filter_violations"] == 0
+    print("[case] cross products on %.1f%% of the pairs" % (st["split_share_ppm"] / 1e4),
+          flush=True)
+
+
 @pytest.mark.parametrize("form", FORMS)
 def test_filter_bound_verified(form, msj_mod, oracle_mod):
     """count_work: sampled pairs' filter classes re-decided in FP64."""
