@@ -1,27 +1,37 @@
 #!/usr/bin/env python3
-"""Benchmark: MiSTIC epsilon self-join on one MI355X.
+"""Benchmark: MiSTIC epsilon self-join on one MI355X, FP64 vs MFMA filter.
 
-Default workload (the configuration the headline metric is quoted on):
-2,000,000 points, d = 18, FP64; 8-Gaussian mixture + 10% uniform noise,
-min-max normalised (seed 1002); eps from a 10,000-query seeded sample
-(seed 2002) so the mean neighbour count, self excluded, is 64.
+Default workload: 515,345 points, d = 90, FP64 correlated Gaussian
+(x = Q diag(sqrt(lambda)) z, lambda_i = i^-1.2), min-max normalised (seed
+1003); eps from a 10,000-query seeded sample (seed 2003) so the mean
+neighbour count, self excluded, is 32.  Both forms run on the same data:
+the plain FP64 join and the split-BF16 MFMA filter with its FP64 band
+recheck.
 
 A step is one whole join on data already resident in HBM: node-net build,
 distance kernels and the ordered result arrays (counts, CSR offsets, sorted
-neighbour lists), which stay in HBM.  `value` is candidate distance
-evaluations per second of that whole step.
+neighbour lists), which stay in HBM.  Each form is timed over K steps after
+W warmup joins.  `value` is candidate distance evaluations per second of
+the MFMA-filter form's step (the plain form's figures sit under "forms").
 
 Also printed (same JSON line):
-  roofline      the distance kernel against the FP64 peak: algorithmic flops =
-                3 x (dimensions the early-exit test consumed, counted by a
-                separate counting pass), divided by the kernel's average
-                duration from HIP events on the library's stream;
-  cpu_baseline  the CPU restatement (oracle) on this machine's cores, on a
-                sampled join (every k-th query point in node order, full
-                candidate lists), as candidate evaluations per second;
+  roofline      the dominant distance kernel of the headline form
+                (k_mfma_join): algorithmic flops (2 d per unordered candidate
+                pair and product plane) over its average duration from HIP
+                events on the library's stream, against the dense BF16 MFMA
+                peak; the plain kernel's FP64 figure is under forms.plain;
+                algorithmic_bytes = rows of every candidate tile read once;
+                traffic (counter-measured HBM bytes) is null here, the
+                rocprofv3 summaries under profiles/ hold it;
+  mismatch_rows_between_forms  rows whose neighbour lists differ between the
+                forms, over all rows;
+  cpu_baseline  the CPU restatement (oracle) on every CPU this process may
+                use, on a sampled join (every k-th query point in node order,
+                full candidate lists), as candidate evaluations per second;
   mismatch      rows of that sample whose GPU neighbour list differs.
 
-usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2]
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c3]
+                       [--forms plain,split]
 """
 import argparse
 import json
@@ -131,16 +141,6 @@ def calibrate(x, cfg):
     return json.loads(out.strip().splitlines()[-1])["eps"]
 
 
-def traffic_bytes(config, form):
-    """HBM bytes per launch of the distance kernel from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload, if any."""
-    path = os.path.join(ROOT, "profiles", "k_join_sym_2Mx18_plain_traffic.json")
-    if config != "c2" or form != 0 or not os.path.exists(path):
-        return None
-    with open(path) as f:
-        return json.load(f)["traffic_bytes_per_launch"]
-
-
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -197,13 +197,180 @@ def cpu_baseline(x, eps, num_calcs, gpu_rows, seconds, threads):
     return out, mism
 
 
+def host_cpus():
+    """CPUs this process may use: the smaller of its affinity set and its
+    cgroup CPU quota (a GPU box grants each GPU a share of its cores)."""
+    info = {"os_cpu_count": os.cpu_count()}
+    try:
+        info["affinity"] = len(os.sched_getaffinity(0))
+    except AttributeError:
+        info["affinity"] = os.cpu_count()
+    quota = None
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = float(q) / float(per)
+    except (OSError, ValueError):
+        pass
+    info["cgroup_cpu_quota"] = quota
+    info["omp_num_threads_env"] = os.environ.get("OMP_NUM_THREADS")
+    try:
+        out = subprocess.run(["nproc"], capture_output=True, text=True, timeout=10).stdout
+        info["nproc"] = int(out.strip())
+    except (OSError, ValueError, subprocess.SubprocessError):
+        info["nproc"] = None
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except (OSError, subprocess.SubprocessError):
+        pass
+    info["model"] = model
+    usable = info["affinity"] or 1
+    if quota:
+        usable = max(1, min(usable, int(quota)))
+    if info["nproc"]:
+        usable = min(usable, info["nproc"])
+    return usable, info
+
+
+def kernel_name(st, d):
+    if st["form_used"] == 0:
+        return "k_join_sym" if (d + 3) // 4 * 4 <= 32 else "k_join_hd"
+    return "k_mfma_join"
+
+
+def run_form(x, eps, cfg, form_name, args, storage):
+    """Counting pass (untimed, host output), then `warmup` + `steps` timed
+    joins on resident data.  Returns (summary dict, host result of the
+    counting pass)."""
+    form = FORMS[form_name]
+    verbose = 1 if args.verbose else 0
+    n, d = x.shape
+    with msj.Context(output=msj.OUT_HOST, count_work=1, storage=storage, form=form,
+                     verbose=verbose) as cc:
+        cc.load(x)
+        ref_run = cc.join(eps)
+    rs = ref_run.stats
+    log("[bench] %s counting pass: R=%d nodes=%d candidates=%d dims=%d form_used=%d" % (
+        form_name, ref_run.total_pairs, rs["num_nodes"], rs["num_calcs"], rs["dims_evaluated"],
+        rs["form_used"]))
+    if form != 0:
+        log("[bench] %s filter check: band=%d checked=%d violations=%d max|err|=%.3g "
+            "mean|err|=%.3g" % (form_name, rs["band_pairs"], rs["filter_checked"],
+                                rs["filter_violations"], rs["max_score_err"],
+                                rs["mean_score_err"]))
+    if args.single_pass:
+        r = ref_run
+        kms, mms = [r.stats["ms_join_kernel"]], [r.stats["ms_main_kernel"]]
+        stage = [(r.stats["ms_index"], r.stats["ms_join"], r.stats["ms_output"])]
+        ms_step = r.stats["ms_total"] - r.stats["ms_d2h"]
+        steps, warmup = 1, 0
+    else:
+        steps, warmup = args.steps, args.warmup
+        # msj_ctx_join returns after its stream has drained: the clock below
+        # brackets exactly the K joins
+        with msj.Context(output=msj.OUT_DEVICE, storage=storage, form=form,
+                         verbose=verbose) as ctx:
+            ctx.load(x)
+            for w in range(warmup):
+                r = ctx.join(eps, copy_arrays=False)
+                log("[bench] %s warmup %d: %.1f ms" % (form_name, w, r.stats["ms_total"]))
+            kms, mms, stage = [], [], []
+            t0 = time.perf_counter()
+            for k in range(steps):
+                r = ctx.join(eps, copy_arrays=False)
+                kms.append(r.stats["ms_join_kernel"])
+                mms.append(r.stats["ms_main_kernel"])
+                stage.append((r.stats["ms_index"], r.stats["ms_join"], r.stats["ms_output"]))
+            wall = time.perf_counter() - t0
+        ms_step = wall * 1e3 / steps
+    st = r.stats
+    C, R = st["num_calcs"], r.total_pairs
+    k_ms, m_ms = float(np.mean(kms)), float(np.mean(mms))
+    ms_index, ms_join, ms_output = (float(v) for v in np.mean(stage, axis=0))
+    log("[bench] %s: %d steps, %.2f ms/step (index %.2f, join %.2f [kernels %.2f, %s %.2f], "
+        "output %.2f ms)" % (form_name, steps, ms_step, ms_index, ms_join, k_ms,
+                             kernel_name(st, d), m_ms, ms_output))
+    kname = kernel_name(st, d)
+    dpad = (d + 3) // 4 * 4
+    if st["form_used"] == 0:
+        flops = 3.0 * rs["dims_evaluated"]  # 3 flops per FP64 dimension consumed
+        roof = {"bound": "fp64_valu", "peak": FP64_PEAK_TFLOPS,
+                "kernel": kname + " (FP64 vector ALU; MI355X FP64 vector peak)"}
+        alg_bytes = 8.0 * d * st["tile_rows"] + 8.0 * R + 4.0 * n
+    else:
+        # every unordered candidate pair (self pairs included) scored over d
+        # dimensions, 2 flops per product; split-BF16 adds the two cross
+        # products where they ran (every pair up to d = 128; above, the wave
+        # blocks of tiles that the BF16 scores do not settle)
+        mult = 1.0 + 2.0 * st["split_share_ppm"] / 1e6 if st["form_used"] == 2 else 1.0
+        flops = (C + n) / 2.0 * d * 2.0 * mult
+        roof = {"bound": "mfma", "peak": BF16_PEAK_TFLOPS,
+                "kernel": kname + " (BF16 MFMA, FP32 accumulate; MI355X dense BF16 peak)"}
+        planes = 2 if st["form_used"] == 2 else 1
+        alg_bytes = 2.0 * planes * ((dpad + 15) // 16 * 16) * st["tile_rows"] + 8.0 * R
+    achieved = flops / (m_ms / 1e3) / 1e12
+    roof.update({"achieved": achieved, "unit": "TFLOP/s", "frac": achieved / roof["peak"],
+                 "traffic": None, "kernel_ms": m_ms, "algorithmic_flops": flops,
+                 "algorithmic_bytes": alg_bytes,
+                 "algorithmic_bytes_per_s": alg_bytes / (m_ms / 1e3)})
+    out = {
+        "form": form_name, "form_used": st["form_used"], "ms_per_step": ms_step,
+        "steps": steps, "warmup": warmup, "join_time_s": ms_step / 1e3,
+        "evals_per_s": C / (ms_step / 1e3), "pairs_per_s": R / (ms_step / 1e3),
+        "total_pairs": R, "candidate_evals": C, "num_nodes": st["num_nodes"],
+        "ms_index": ms_index, "ms_join": ms_join, "ms_output": ms_output,
+        "ms_join_kernels": k_ms, "roofline": roof,
+        "dtype": {0: "f64", 1: "bf16 MFMA (f32 acc) + f64 band",
+                  2: "split-bf16 MFMA (f32 acc) + f64 band"}[st["form_used"]],
+    }
+    if st["form_used"] != 0:
+        out["filter"] = {
+            "band_pairs": rs["band_pairs"],
+            "band_fraction": rs["band_pairs"] / max(1.0, (C + n) / 2.0),
+            "max_score_err": rs["max_score_err"], "mean_score_err": rs["mean_score_err"],
+            "sampled_pairs_checked": rs["filter_checked"],
+            "violations": rs["filter_violations"],
+            "mfma_flops_issued": st["mfma_flops"],
+            "split_share": st["split_share_ppm"] / 1e6,
+            "mfma_issued_frac_of_peak": st["mfma_flops"] / (m_ms / 1e3) / 1e12 / BF16_PEAK_TFLOPS,
+            "ms_band_kernel": k_ms - m_ms,
+        }
+    return out, ref_run
+
+
+def same_result(a, b):
+    """Rows whose neighbour lists differ between two host results."""
+    if a.total_pairs == b.total_pairs and np.array_equal(a.counts, b.counts) \
+            and np.array_equal(a.nbrs, b.nbrs):
+        return 0
+    bad = 0
+    for row in range(a.n):
+        a0, a1 = int(a.offsets[row]), int(a.offsets[row + 1])
+        b0, b1 = int(b.offsets[row]), int(b.offsets[row + 1])
+        if a.counts[row] != b.counts[row] or not np.array_equal(a.nbrs[a0:a1], b.nbrs[b0:b1]):
+            bad += 1
+    return bad
+
+
+DEFAULT_FORMS = {"c1": "plain", "c2": "plain", "c3": "plain,split", "c4": "split"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c5"])
-    ap.add_argument("--form", default="plain", choices=sorted(FORMS))
+    ap.add_argument("--config", default="c3", choices=sorted(CONFIGS) + ["c5"])
+    ap.add_argument("--forms", default=None,
+                    help="comma list of forms to time (plain, bf16, split); default: "
+                         "plain,split for c3, plain for c1/c2, split for c4")
+    ap.add_argument("--form", default=None, choices=sorted(FORMS),
+                    help="time one form only (same as --forms FORM)")
     ap.add_argument("--eps", type=float, default=None, help="override the calibrated eps")
     ap.add_argument("--calibrate", action="store_true",
                     help="recompute eps from the seeded 10,000-query sample, print it, exit")
@@ -216,7 +383,8 @@ def main():
     ap.add_argument("--verbose", action="store_true",
                     help="library progress lines (layers, join item ranges) on stderr")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="CPU baseline threads (0: every CPU this process may use)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -226,6 +394,10 @@ def main():
     if args.config == "c5":
         return run_c5(args)
     cfg = CONFIGS[args.config]
+    forms = (args.form or args.forms or DEFAULT_FORMS[args.config]).split(",")
+    for f in forms:
+        if f not in FORMS:
+            ap.error("unknown form %r" % f)
     log("[bench] generating %s: n=%d d=%d" % (args.config, cfg["n"], cfg["d"]))
     x = make_data(cfg)
     if args.rows:
@@ -247,110 +419,54 @@ def main():
         print(json.dumps({"config": args.config, "eps": eps, "eps_hex": eps_hex}), flush=True)
         return
 
-    # counting pass (untimed): algorithmic FP64 work of the distance kernel
     storage = msj.STORAGE_F32 if cfg.get("storage") == "f32" else msj.STORAGE_F64
-    form = FORMS[args.form]
-    verbose = 1 if args.verbose else 0
-    with msj.Context(output=msj.OUT_HOST, count_work=1, storage=storage, form=form,
-                     verbose=verbose) as cc:
-        cc.load(x)
-        t_count = time.perf_counter()
-        ref_run = cc.join(eps)
-        t_count = time.perf_counter() - t_count
-    dims_eval = ref_run.stats["dims_evaluated"]
-    gpu_rows = (ref_run.counts, ref_run.offsets, ref_run.nbrs, ref_run.node_order)
-    log("[bench] counting pass: R=%d nodes=%d candidates=%d dims=%d" % (
-        ref_run.total_pairs, ref_run.stats["num_nodes"], ref_run.stats["num_calcs"], dims_eval))
-    if form != 0:
-        rs = ref_run.stats
-        log("[bench] filter check: form_used=%d band=%d checked=%d violations=%d "
-            "max|err|=%.3g mean|err|=%.3g" % (rs["form_used"], rs["band_pairs"],
-                                              rs["filter_checked"], rs["filter_violations"],
-                                              rs["max_score_err"], rs["mean_score_err"]))
-
-    # msj_ctx_join returns after its stream has drained: the clock below
-    # brackets exactly the K joins.
-    if args.single_pass:
-        r = ref_run
-        kms = [r.stats["ms_join_kernel"]]
-        stage = [(r.stats["ms_index"], r.stats["ms_join"], r.stats["ms_output"])]
-        ms_step = r.stats["ms_total"] - r.stats["ms_d2h"]
-        args.steps, args.warmup = 1, 0
-    else:
-        with msj.Context(output=msj.OUT_DEVICE, storage=storage, form=form,
-                         verbose=verbose) as ctx:
-            ctx.load(x)
-            for w in range(args.warmup):
-                r = ctx.join(eps, copy_arrays=False)
-                log("[bench] warmup %d: %.1f ms" % (w, r.stats["ms_total"]))
-            kms, stage = [], []
-            t0 = time.perf_counter()
-            for k in range(args.steps):
-                r = ctx.join(eps, copy_arrays=False)
-                kms.append(r.stats["ms_join_kernel"])
-                stage.append((r.stats["ms_index"], r.stats["ms_join"], r.stats["ms_output"]))
-            wall = time.perf_counter() - t0
-        ms_step = wall * 1e3 / args.steps
-    st = r.stats
-    C = st["num_calcs"]
-    R = r.total_pairs
-    value = C / (ms_step / 1e3)
-    k_ms = float(np.mean(kms))
-    if st["form_used"] == 0:
-        flops = 3.0 * dims_eval
-        peak, kname = FP64_PEAK_TFLOPS, "k_join_sym (FP64 vector ALU; MI355X FP64 peak)"
-    else:
-        # useful matrix-core work: every unordered candidate pair (self pairs
-        # included) scored over d dimensions, 2 flops per product; split-BF16
-        # adds 2 cross products per dimension where they are needed (every
-        # pair up to d = 128; above, the wave blocks of tiles not settled by
-        # their BF16 scores)
-        mult = 1.0 + 2.0 * st["split_share_ppm"] / 1e6 if st["form_used"] == 2 else 1.0
-        flops = (st["num_calcs"] + cfg["n"]) / 2.0 * cfg["d"] * 2.0 * mult
-        peak = BF16_PEAK_TFLOPS
-        kname = "k_mfma_join + k_band_check (BF16 MFMA; MI355X dense BF16 peak)"
-    achieved = flops / (k_ms / 1e3) / 1e12
-    log("[bench] %d steps: %.2f ms/step (index %.2f, join %.2f [kernel %.2f], output %.2f ms)"
-        % (args.steps, ms_step, *np.mean(stage, axis=0)[:2], k_ms, np.mean(stage, axis=0)[2]))
-
+    results, hosts = {}, {}
+    for f in forms:
+        results[f], hosts[f] = run_form(x, eps, cfg, f, args, storage)
+    # headline: the MFMA-filter form when one was timed, else the plain form
+    head = next((f for f in reversed(forms) if results[f]["form_used"] != 0), forms[0])
+    H = results[head]
     line = {
-        "metric": METRIC, "value": value, "unit": "distance evals/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
+        "metric": METRIC, "value": H["evals_per_s"], "unit": "distance evals/s", "n_gpus": 1,
+        "steps": H["steps"], "warmup": H["warmup"], "ms_per_step": H["ms_per_step"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": {0: "f64", 1: "bf16 MFMA (f32 acc) + f64 band",
-                  2: "split-bf16 MFMA (f32 acc) + f64 band"}[st["form_used"]],
-        "mean_neighbours": R / cfg["n"] - 1.0,
+        "dtype": H["dtype"], "mean_neighbours": H["total_pairs"] / cfg["n"] - 1.0,
         "data": "synthetic (seeded, generated in-process)",
         "config": {"workload": cfg["workload"], "n": cfg["n"], "d": cfg["d"], "eps": eps,
-                   "eps_hex": eps_hex, "form": args.form, "parallelism": "none (1 GPU)",
+                   "eps_hex": eps_hex, "form": head, "forms_timed": forms,
+                   "parallelism": "none (1 GPU)",
                    "timing": "one counting join (work counters on)" if args.single_pass
-                   else "%d joins on resident data" % args.steps},
-        "join_time_s": ms_step / 1e3, "pairs_per_s": R / (ms_step / 1e3), "total_pairs": R,
-        "candidate_evals": C, "num_nodes": st["num_nodes"],
-        "ms_index": float(np.mean([s[0] for s in stage])),
-        "ms_join": float(np.mean([s[1] for s in stage])),
-        "ms_output": float(np.mean([s[2] for s in stage])),
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak,
-                     "unit": "TFLOP/s", "frac": achieved / peak,
-                     "traffic": traffic_bytes(args.config, st["form_used"]),
-                     "kernel": kname, "kernel_ms": k_ms, "algorithmic_flops": flops},
+                   else "%d joins on resident data per form" % H["steps"]},
+        "join_time_s": H["join_time_s"], "pairs_per_s": H["pairs_per_s"],
+        "total_pairs": H["total_pairs"], "candidate_evals": H["candidate_evals"],
+        "num_nodes": H["num_nodes"], "ms_index": H["ms_index"], "ms_join": H["ms_join"],
+        "ms_output": H["ms_output"], "roofline": H["roofline"], "forms": results,
     }
-    if st["form_used"] != 0:
-        rs = ref_run.stats
-        line["filter"] = {
-            "band_pairs": rs["band_pairs"],
-            "band_fraction": rs["band_pairs"] / max(1.0, (C + cfg["n"]) / 2.0),
-            "max_score_err": rs["max_score_err"], "mean_score_err": rs["mean_score_err"],
-            "sampled_pairs_checked": rs["filter_checked"],
-            "violations": rs["filter_violations"],
-            "mfma_flops_issued": st["mfma_flops"],
-            "split_share": st["split_share_ppm"] / 1e6,
-            "mfma_util": st["mfma_flops"] / (k_ms / 1e3) / 1e12 / BF16_PEAK_TFLOPS,
-        }
+    if "filter" in H:
+        line["filter"] = H["filter"]
+    if "plain" in results and head != "plain":
+        line["speedup_filter_vs_plain"] = results["plain"]["ms_per_step"] / H["ms_per_step"]
+        line["speedup_filter_vs_plain_join_kernels"] = (results["plain"]["ms_join_kernels"]
+                                                        / H["ms_join_kernels"])
+    # full-size parity between the forms (every row, both host results)
+    if len(forms) > 1:
+        base = hosts[forms[0]]
+        line["mismatch_rows_between_forms"] = {
+            f: same_result(base, hosts[f]) for f in forms[1:]}
+        line["node_order_equal_between_forms"] = all(
+            np.array_equal(base.node_order, hosts[f].node_order) for f in forms[1:])
+        log("[bench] forms compared on every row: %s" % line["mismatch_rows_between_forms"])
     if not args.no_cpu_baseline:
-        log("[bench] CPU baseline (oracle, %d threads, ~%.0f s)" % (args.cpu_threads,
-                                                                  args.cpu_seconds))
-        cb, mism = cpu_baseline(x, eps, C, gpu_rows, args.cpu_seconds, args.cpu_threads)
+        threads, hostinfo = host_cpus()
+        if args.cpu_threads:
+            threads = args.cpu_threads
+        log("[bench] CPU baseline (oracle, %d threads of %s, ~%.0f s)" % (
+            threads, hostinfo, args.cpu_seconds))
+        ref = hosts[forms[0]]
+        gpu_rows = (ref.counts, ref.offsets, ref.nbrs, ref.node_order)
+        cb, mism = cpu_baseline(x, eps, H["candidate_evals"], gpu_rows, args.cpu_seconds,
+                                threads)
+        cb["host"] = hostinfo
         line["cpu_baseline"] = cb
         line["mismatch"] = mism
         log("[bench] CPU %.3g evals/s; sampled-row mismatch %d" % (cb["value"], mism))
@@ -447,7 +563,7 @@ def run_c5(args):
         "speedup_vs_sequential": seq_ms / ms_step, "mismatch_jobs": mism,
         "tuned": {"class_ds": st["class_ds"], "epi_log2": st["class_epi_log2"],
                   "grid_blocks": st["grid_blocks"], "work_items": st["work_items"]},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+        "roofline": {"bound": "fp64_valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
                      "kernel": "k_join_batch (persistent, FP64 vector ALU; MI355X FP64 peak)",
                      "kernel_ms": ms_join, "algorithmic_flops": flops},
@@ -455,21 +571,24 @@ def run_c5(args):
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle  # noqa: WPS433 - the CPU baseline
+        threads, hostinfo = host_cpus()
+        if args.cpu_threads:
+            threads = args.cpu_threads
         # about 2e8 candidate evaluations/s on the box's cores at d = 57..90
         budget = max(1.0, 2e8 * args.cpu_seconds)
         every = max(1, int(calcs // budget))
         done, tj = 0.0, 0.0
         log("[bench] CPU baseline (oracle, %d threads): every %d-th query" % (
-            args.cpu_threads, every))
+            threads, every))
         for k, (x, e) in enumerate(zip(datas, eps)):
-            r = oracle.self_join(x, e, sample_every=every, nthreads=args.cpu_threads)
+            r = oracle.self_join(x, e, sample_every=every, nthreads=threads)
             done += r.evals_done
             tj += r.t_join
             if k % 8 == 7:
                 log("[bench] CPU baseline: %d/64 instances, %.1f s of join" % (k + 1, tj))
         line["cpu_baseline"] = {
             "value": done / tj if tj > 0 else 0.0, "unit": "distance evals/s",
-            "cores": args.cpu_threads, "kind": "port",
+            "cores": threads, "host": hostinfo, "kind": "port",
             "sample": "oracle (C/OpenMP restatement) on all 64 instances: node nets + join of "
                       "every %d-th query point in node order (%.3g candidate evaluations, "
                       "%.1f s of join)" % (every, done, tj)}

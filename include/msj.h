@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define MSJ_ABI_VERSION 1u
+#define MSJ_ABI_VERSION 2u
 #define MSJ_MAX_LAYERS 16
 
 typedef enum msj_status {
@@ -118,6 +118,11 @@ typedef struct msj_stats {
                                     whose cross products ran, per million (the
                                     others were certain rejects in BF16)        */
     double   mfma_flops;         /* MFMA forms: matrix-core flops issued (tiles)    */
+    double   ms_main_kernel;     /* the dominant distance kernel alone, HIP events:
+                                    k_join_sym / k_join_hd (plain) or k_mfma_join   */
+    uint64_t tile_rows;          /* sum over adjacent node pairs X <= Y of |X|+|Y|:
+                                    rows read once per candidate tile; 8*d bytes
+                                    each is the join's algorithmic HBM traffic     */
 } msj_stats;
 
 typedef struct msj_result {

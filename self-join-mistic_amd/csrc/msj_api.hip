@@ -246,6 +246,7 @@ uint64_t page_to_host(msj_ctx *c, void *dst, const void *src, size_t bytes, size
 {
     if (bytes == 0)
         return 0;
+    page = std::min(page, bytes);  // a short transfer pins no more than it moves
     c->pin[0].ensure(page);
     c->pin[1].ensure(page);
     MSJ_HIP(hipStreamWaitEvent(c->copy_stream, ready, 0));
@@ -322,17 +323,20 @@ void do_join(msj_ctx *c, double eps, msj_result *out)
         uint64_t budget = (uint64_t)(freeb / 4) / sizeof(uint64_t);
         cap = std::max<uint64_t>(std::min(want, budget), 1);
     }
-    float kms = 0.f;
+    float kms = 0.f, mms = 0.f;
     uint64_t launches = 0, dims = 0;
     uint64_t R;
     if (use_mfma) {
         R = c->mfma.run(c->ix, c->join, c->xr.p, n, d, eps, cap, c->prm.count_work != 0, s,
-                        c->ev[2], c->ev[3], &launches, &kms, &st);
+                        c->ev[2], c->ev[7], c->ev[3], &launches, &kms, &mms, &st);
     } else {
         R = c->join.run_plain(c->ix, c->xr.p, n, d, eps, cap, s, c->ev[2], c->ev[3], &launches,
                               &kms, c->prm.count_work ? &dims : nullptr, c->prm.verbose != 0);
+        mms = kms;
         st.form_used = MSJ_FORM_PLAIN;
     }
+    st.ms_main_kernel = mms;
+    st.tile_rows = c->join.tile_rows;
     st.dims_evaluated = dims;
     c->last_R = R;
     if (R >= 0xFFFFFFFFull * 16)

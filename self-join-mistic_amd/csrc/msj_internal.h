@@ -192,7 +192,8 @@ struct JobView {
 
 struct JoinRunner {
     DevBuf<uint32_t> itcnt, itoff, uoff, uadj, ucum;
-    DevBuf<uint64_t> ulen, tot64;
+    DevBuf<uint64_t> ulen, tot64, trow;
+    DevBuf<unsigned long long> tsum;
     DevBuf<JoinItem> items;
     DevBuf<unsigned long long> counter, work_ctr;
     DevBuf<uint64_t> pairs;      // (row << qbits | neighbour) keys
@@ -200,6 +201,7 @@ struct JoinRunner {
     DevBuf<uint64_t> jkeys;
     DevBuf<double> mm;
     uint32_t nitems = 0;
+    uint64_t tile_rows = 0;      // sum over upper entries (X, Y) of |X| + |Y|
     Scanner scan;
     RadixSorter sorter;
     // Join order of the points: nodes as in ix, points inside a node sorted
@@ -279,9 +281,12 @@ struct MfmaRunner {
                  uint32_t d, double eps, bool split_bf16, hipStream_t s);
     // Filter + band recheck; writes the result keys into J.pairs (grown and
     // rerun on overflow) and returns their number.
+    // kernel_ms: filter + band kernels; main_ms: the filter kernel alone
+    // (events k0 .. kmid).
     uint64_t run(const Index &ix, JoinRunner &J, const double *xr, uint32_t n, uint32_t d,
                  double eps, uint64_t cap_hint, bool diag, hipStream_t s, hipEvent_t k0,
-                 hipEvent_t k1, uint64_t *launches, float *kernel_ms, msj_stats *st);
+                 hipEvent_t kmid, hipEvent_t k1, uint64_t *launches, float *kernel_ms,
+                 float *main_ms, msj_stats *st);
     uint64_t last_band = 0;
 };
 

@@ -114,12 +114,14 @@ __global__ __launch_bounds__(256, 4) void k_join_hd(JobView J,
         add_dims(dims, work);
 }
 
-// Upper neighbour lists: entries Y >= X of X's adjacency, and their sizes.
+// Upper neighbour lists: entries Y >= X of X's adjacency, and their sizes;
+// trow[x] = sum over those entries of |X| + |Y| (rows of X's candidate tiles).
 __global__ __launch_bounds__(256) void k_upper_count(const uint32_t *__restrict__ nstart,
                                                      const uint32_t *__restrict__ aoff,
                                                      const uint32_t *__restrict__ adj, uint32_t M,
                                                      uint32_t *__restrict__ cnt,
-                                                     uint64_t *__restrict__ len)
+                                                     uint64_t *__restrict__ len,
+                                                     uint64_t *__restrict__ trow)
 {
     uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= M)
@@ -135,6 +137,7 @@ __global__ __launch_bounds__(256) void k_upper_count(const uint32_t *__restrict_
     }
     cnt[x] = k;
     len[x] = L;
+    trow[x] = (uint64_t)k * (nstart[x + 1] - nstart[x]) + L;
 }
 
 // ... and, per entry, the offset of its first point inside the node's
@@ -479,12 +482,18 @@ void JoinRunner::plan(const Index &ix, uint32_t n, uint32_t d, hipStream_t s, bo
     ulen.ensure(M);
     uoff.ensure((size_t)M + 1);
     tot64.ensure((size_t)M + 1);
+    trow.ensure(M);
+    tsum.ensure(1);
     LAUNCH(k_upper_count, grid_for(M, 256), 256, ix.nstart.p, ix.aoff.p, ix.adj.p, M, itcnt.p,
-           ulen.p);
+           ulen.p, trow.p);
+    reduce_sum_u64(trow.p, M, tsum.p, s);
     scan.excl_u32_to_u32(itcnt.p, uoff.p, M, s);
     uint32_t ue = 0;
+    unsigned long long trows = 0;
     MSJ_HIP(hipMemcpyAsync(&ue, uoff.p + M, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    MSJ_HIP(hipMemcpyAsync(&trows, tsum.p, sizeof(trows), hipMemcpyDeviceToHost, s));
     MSJ_HIP(hipStreamSynchronize(s));
+    tile_rows = trows;
     uadj.ensure(std::max<uint32_t>(ue, 1));
     ucum.ensure(std::max<uint32_t>(ue, 1));
     LAUNCH(k_upper_write, grid_for(M, 256), 256, ix.nstart.p, ix.aoff.p, ix.adj.p, M, uoff.p,

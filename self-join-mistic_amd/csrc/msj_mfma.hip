@@ -1416,7 +1416,8 @@ void MfmaRunner::prepare(const Index &ix, const JoinRunner &J, const double *xr,
 
 uint64_t MfmaRunner::run(const Index &ix, JoinRunner &J, const double *xr, uint32_t n, uint32_t d,
                          double eps, uint64_t cap_hint, bool diag, hipStream_t s, hipEvent_t k0,
-                         hipEvent_t k1, uint64_t *launches, float *kernel_ms, msj_stats *st)
+                         hipEvent_t kmid, hipEvent_t k1, uint64_t *launches, float *kernel_ms,
+                         float *main_ms, msj_stats *st)
 {
     const double eps2 = eps * eps;  // launcher.cu:3200
     const uint32_t ds = padded_dim(d);
@@ -1431,6 +1432,7 @@ uint64_t MfmaRunner::run(const Index &ix, JoinRunner &J, const double *xr, uint3
     if (J.pairs.cap < cap_hint)
         J.pairs.ensure(cap_hint);
     *kernel_ms = 0.f;
+    *main_ms = 0.f;
     for (int attempt = 0; attempt < 4; ++attempt) {
         // count_work: verify every pair of one item in sample_every (at least
         // 61, about 5e7 sampled pairs at most)
@@ -1483,6 +1485,7 @@ uint64_t MfmaRunner::run(const Index &ix, JoinRunner &J, const double *xr, uint3
             }
 #undef MSJ_MJ
 #undef MSJ_MJH
+            MSJ_HIP(hipEventRecord(kmid, s));
             if (diag)
                 LAUNCH(k_band_check<true>, 2048, 256, xr, ds, eps2, J.jperm.p,
                        (const unsigned long long *)band.p, bscore.p,
@@ -1494,14 +1497,18 @@ uint64_t MfmaRunner::run(const Index &ix, JoinRunner &J, const double *xr, uint3
                        (unsigned long long)band.cap, qbits, (unsigned long long *)J.pairs.p,
                        (unsigned long long)J.pairs.cap, ctr.p);
             *launches += 2;
+        } else {
+            MSJ_HIP(hipEventRecord(kmid, s));
         }
         MSJ_HIP(hipEventRecord(k1, s));
         unsigned long long h[9];
         MSJ_HIP(hipMemcpyAsync(h, ctr.p, sizeof(h), hipMemcpyDeviceToHost, s));
         MSJ_HIP(hipStreamSynchronize(s));
-        float ms = 0.f;
+        float ms = 0.f, mms = 0.f;
         MSJ_HIP(hipEventElapsedTime(&ms, k0, k1));
+        MSJ_HIP(hipEventElapsedTime(&mms, k0, kmid));
         *kernel_ms += ms;
+        *main_ms += mms;
         const uint64_t R = h[0], nband = h[1];
         if (!split && h[7] > bf16_budget) {
             // escalate: the same items with split-BF16 operands

@@ -79,6 +79,8 @@ class Stats(C.Structure):
         ("form_used", C.c_uint32),
         ("split_share_ppm", C.c_uint32),
         ("mfma_flops", C.c_double),
+        ("ms_main_kernel", C.c_double),
+        ("tile_rows", C.c_uint64),
     ]
 
 
@@ -227,6 +229,7 @@ class JoinResult:
             "filter_checked": st.filter_checked, "filter_violations": st.filter_violations,
             "form_used": st.form_used, "mfma_flops": st.mfma_flops,
             "split_share_ppm": st.split_share_ppm,
+            "ms_main_kernel": st.ms_main_kernel, "tile_rows": st.tile_rows,
         }
         self.on_device = bool(r.on_device)
         if copy_arrays and not r.on_device:
