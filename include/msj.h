@@ -55,6 +55,16 @@ typedef enum msj_form {
     MSJ_FORM_MFMA_SPLIT = 2  /* split-BF16 MFMA Gram filter + FP64 band recheck */
 } msj_form;
 
+/* Candidate generator: which pairs the distance test is applied to.  Every
+ * generator yields the same pair set (the distance test decides); they differ
+ * in how many candidates they evaluate. */
+typedef enum msj_index {
+    MSJ_INDEX_NODENET = 0,   /* the reference's node net (nodes.cu:5-356)        */
+    MSJ_INDEX_BRUTE = 1,     /* every pair: one node holding all rows; the
+                                verifier's ground truth (nodes.cu:749-885)     */
+    MSJ_INDEX_COSS = 2       /* the reference's tree/COSS bins (tree.cu:3-996)   */
+} msj_index;
+
 /* Where msj_ctx_join leaves the result arrays. */
 typedef enum msj_output {
     MSJ_OUT_HOST = 0,   /* library-allocated host arrays in msj_result          */
@@ -77,7 +87,7 @@ typedef struct msj_params {
     uint32_t count_work;         /* 1: plain form counts the FP64 dimensions
                                     evaluated; MFMA forms verify their filter
                                     classes in FP64 (both slower)                 */
-    uint32_t reserved0;
+    uint32_t index;              /* msj_index: candidate generator (default node net) */
     uint64_t page_pairs;         /* MSJ_OUT_HOST: neighbour ids per page of the
                                     pinned double buffer (0: 2^24)                 */
 } msj_params;
@@ -154,8 +164,22 @@ void msj_result_free(msj_result *r);
 /* Context API: keeps the data resident in HBM across joins. */
 int  msj_ctx_create(const msj_params *p, msj_ctx **ctx);
 void msj_ctx_destroy(msj_ctx *ctx);
+/* Replaces the context's parameters for later joins (form, candidate
+ * generator, node-net knobs, output, ...); device and storage must stay as
+ * they were at creation (MSJ_E_INVAL otherwise).  The loaded data stays. */
+int  msj_ctx_set_params(msj_ctx *ctx, const msj_params *p);
 /* Uploads n x d row-major values of p->storage precision. */
 int  msj_ctx_load(msj_ctx *ctx, const void *data, uint32_t n, uint32_t d);
+/* main.cu:42-50 + 106-122 in one call: n rows of d FP64 values in file
+ * order (dimensions NOT yet reordered) are streamed to the device through a
+ * bounded pinned buffer (rows may be a read-only mapping of the input file,
+ * larger than host memory), the reference's per-dimension sigma is computed
+ * on the device (bit-identical to stddev(), utils.cu:22-53) and the
+ * dimensions are kept in decreasing-sigma order at p->storage precision.
+ * order[d] / sigma[d] (may be NULL) receive that order and the sorted
+ * sigmas, as msj_dim_order returns them. */
+int  msj_ctx_load_unordered(msj_ctx *ctx, const double *rows, uint32_t n, uint32_t d,
+                            uint32_t *order, double *sigma);
 /* Runs the whole join on the resident data.  With MSJ_OUT_DEVICE the result
  * arrays are device pointers valid until the next join on, or the
  * destruction of, this context; msj_result_free does not free them. */

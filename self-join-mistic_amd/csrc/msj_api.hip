@@ -135,6 +135,8 @@ void validate_params(const msj_params &p)
         throw Error(MSJ_E_INVAL, "unknown output mode");
     if (p.form != MSJ_FORM_PLAIN && p.form != MSJ_FORM_MFMA_BF16 && p.form != MSJ_FORM_MFMA_SPLIT)
         throw Error(MSJ_E_INVAL, "unknown form");
+    if (p.index != MSJ_INDEX_NODENET && p.index != MSJ_INDEX_BRUTE && p.index != MSJ_INDEX_COSS)
+        throw Error(MSJ_E_INVAL, "unknown candidate generator (index)");
 }
 
 template <class F>
@@ -165,6 +167,52 @@ __global__ void k_soa(const unsigned char *__restrict__ rm, unsigned char *__res
             ((double *)soa)[idx] = ((const double *)rm)[(size_t)i * d + j];
         else
             ((float *)soa)[idx] = ((const float *)rm)[(size_t)i * d + j];
+    }
+}
+
+// Rows [row0, row0 + m) of an n-row table, row-major FP64 in `chunk`, into
+// the FP64 columns soa[j * n + row].
+__global__ void k_rows_to_soa(const double *__restrict__ chunk, uint32_t m, uint32_t d,
+                              uint32_t row0, uint32_t n, double *__restrict__ soa)
+{
+    const size_t total = (size_t)m * d, stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += stride) {
+        const uint32_t j = (uint32_t)(idx / m), i = (uint32_t)(idx - (size_t)j * m);
+        soa[(size_t)j * n + row0 + i] = chunk[(size_t)i * d + j];
+    }
+}
+
+// xs[j * n + i] = soa[order[j] * n + i] at the storage precision (FP32:
+// rounded to nearest, as a host (float) cast).
+template <class T>
+__global__ void k_permute_cols(const double *__restrict__ soa, const uint32_t *__restrict__ order,
+                               uint32_t n, uint32_t d, T *__restrict__ xs)
+{
+    const size_t total = (size_t)n * d, stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += stride) {
+        const uint32_t j = (uint32_t)(idx / n), i = (uint32_t)(idx - (size_t)j * n);
+        xs[idx] = (T)soa[(size_t)order[j] * n + i];
+    }
+}
+
+// utils.cu:42-48: dimensions by decreasing sigma; thrust's stable ascending
+// radix order of the doubles, then reversed (ties: higher index first).
+void order_by_sigma(const double *dev, uint32_t d, uint32_t *order, double *sigma)
+{
+    auto rkey = [](double x) {
+        uint64_t u;
+        std::memcpy(&u, &x, 8);
+        return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+    };
+    std::vector<uint32_t> asc(d);
+    for (uint32_t i = 0; i < d; ++i)
+        asc[i] = i;
+    std::stable_sort(asc.begin(), asc.end(),
+                     [&](uint32_t a, uint32_t b) { return rkey(dev[a]) < rkey(dev[b]); });
+    for (uint32_t i = 0; i < d; ++i) {
+        order[i] = asc[d - 1 - i];
+        if (sigma)
+            sigma[i] = dev[asc[d - 1 - i]];
     }
 }
 
@@ -236,6 +284,85 @@ void do_load(msj_ctx *c, const void *data, uint32_t n, uint32_t d)
     c->staging.release();
 }
 
+// main.cu:42-50 + 106-122 on the device: n rows of d FP64 values in file
+// order stream through the pinned double buffer (the host copies chunk k+1
+// while chunk k crosses the link and is scattered into FP64 columns), the
+// per-column sigma is summed on the device in the reference's order, and
+// the columns are kept in decreasing-sigma order at the storage precision.
+// `rows` may be a read-only file mapping: it is read once, sequentially.
+void do_load_unordered(msj_ctx *c, const double *rows, uint32_t n, uint32_t d,
+                       uint32_t *order_out, double *sigma_out)
+{
+    if (d == 0)
+        throw Error(MSJ_E_INVAL, "d must be positive");
+    if (n > 0 && !rows)
+        throw Error(MSJ_E_INVAL, "data is NULL");
+    if ((uint64_t)n * d >= (1ull << 40))
+        throw Error(MSJ_E_OVERFLOW, "n*d too large");
+    DeviceGuard g(c->prm.device);
+    hipStream_t s = c->stream;
+    c->f32 = c->prm.storage == MSJ_STORAGE_F32;
+    c->n = 0;
+    c->d = d;
+    std::vector<uint32_t> order(d);
+    std::vector<double> sig(d, 0.0);
+    if (n == 0) {
+        for (uint32_t j = 0; j < d; ++j)
+            order[j] = j;
+    } else {
+        msj::DevBuf<double> soa, chunk[2], dsig;
+        msj::DevBuf<uint32_t> dorder;
+        soa.ensure((size_t)n * d);
+        const size_t chunk_bytes = (size_t)16 << 20;
+        const uint32_t R = (uint32_t)std::max<size_t>(1, chunk_bytes / (sizeof(double) * d));
+        for (auto &b : chunk)
+            b.ensure((size_t)R * d);
+        c->pin[0].ensure((size_t)R * d * sizeof(double));
+        c->pin[1].ensure((size_t)R * d * sizeof(double));
+        const uint32_t nchunks = (uint32_t)((n + (uint64_t)R - 1) / R);
+        for (uint32_t k = 0; k < nchunks; ++k) {
+            const uint32_t row0 = k * R, m = std::min<uint32_t>(R, n - row0);
+            const size_t bytes = (size_t)m * d * sizeof(double);
+            if (k >= 2)  // the copy out of this pinned buffer (chunk k-2) is done
+                MSJ_HIP(hipEventSynchronize(c->page_ev[k & 1]));
+            std::memcpy(c->pin[k & 1].p, rows + (size_t)row0 * d, bytes);
+            MSJ_HIP(hipMemcpyAsync(chunk[k & 1].p, c->pin[k & 1].p, bytes, hipMemcpyHostToDevice,
+                                   s));
+            MSJ_HIP(hipEventRecord(c->page_ev[k & 1], s));
+            const size_t cnt = (size_t)m * d;
+            msj::check_grid(msj::grid_stride(cnt, 256), 256);
+            hipLaunchKernelGGL(k_rows_to_soa, dim3(msj::grid_stride(cnt, 256)), dim3(256), 0, s,
+                               chunk[k & 1].p, m, d, row0, n, soa.p);
+            MSJ_HIP(hipGetLastError());
+        }
+        dsig.ensure(d);
+        msj::column_sigma(soa.p, n, d, 1, n, dsig.p, s);
+        std::vector<double> dev(d);
+        MSJ_HIP(hipMemcpyAsync(dev.data(), dsig.p, sizeof(double) * d, hipMemcpyDeviceToHost, s));
+        MSJ_HIP(hipStreamSynchronize(s));
+        order_by_sigma(dev.data(), d, order.data(), sig.data());
+        dorder.ensure(d);
+        MSJ_HIP(hipMemcpyAsync(dorder.p, order.data(), sizeof(uint32_t) * d,
+                               hipMemcpyHostToDevice, s));
+        const size_t total = (size_t)n * d;
+        c->xs.ensure(total * (c->f32 ? 4 : 8));
+        msj::check_grid(msj::grid_stride(total, 256), 256);
+        if (c->f32)
+            hipLaunchKernelGGL(k_permute_cols<float>, dim3(msj::grid_stride(total, 256)),
+                               dim3(256), 0, s, soa.p, dorder.p, n, d, (float *)c->xs.p);
+        else
+            hipLaunchKernelGGL(k_permute_cols<double>, dim3(msj::grid_stride(total, 256)),
+                               dim3(256), 0, s, soa.p, dorder.p, n, d, (double *)c->xs.p);
+        MSJ_HIP(hipGetLastError());
+        MSJ_HIP(hipStreamSynchronize(s));
+    }
+    c->n = n;
+    if (order_out)
+        std::memcpy(order_out, order.data(), sizeof(uint32_t) * d);
+    if (sigma_out)
+        std::memcpy(sigma_out, sig.data(), sizeof(double) * d);
+}
+
 // Device -> host copy in pages through two pinned buffers on the copy
 // stream: the DMA of page k+1 runs while the host copies page k out of the
 // other buffer (the reference's batch loop pages results the same way,
@@ -303,7 +430,12 @@ void do_join(msj_ctx *c, double eps, msj_result *out)
     // ---- index (subsystems a, b)
     MSJ_HIP(hipEventRecord(c->ev[0], s));
     msj::LayerStats ls;
-    c->builder.build(c->xs.p, c->f32, n, d, eps, c->prm, c->ix, ls, s);
+    if (c->prm.index == MSJ_INDEX_BRUTE)
+        msj::build_single_node(n, c->ix, ls, s);
+    else if (c->prm.index == MSJ_INDEX_COSS)
+        throw Error(MSJ_E_UNSUPPORTED, "the tree/COSS candidate generator is not built");
+    else
+        c->builder.build(c->xs.p, c->f32, n, d, eps, c->prm, c->ix, ls, s);
     c->join.order(c->ix, c->xs.p, c->f32, n, d, s);
     c->xr.ensure((size_t)n * msj::padded_dim(d));
     msj::gather_node_rows(c->xs.p, c->xr.p, c->f32, c->join.jperm.p, n, d, s);
@@ -414,9 +546,14 @@ void do_join(msj_ctx *c, double eps, msj_result *out)
 
 }  // namespace
 
+int msj::guarded_call(void (*fn)(void *), void *arg)
+{
+    return guarded([&] { fn(arg); });
+}
+
 extern "C" {
 
-const char *msj_version(void) { return "msj 0.1.0 (gfx950, HIP)"; }
+const char *msj_version(void) { return "msj 0.2.0 (gfx950, HIP)"; }
 
 const char *msj_strerror(int status)
 {
@@ -616,6 +753,28 @@ int msj_ctx_load(msj_ctx *c, const void *data, uint32_t n, uint32_t d)
     });
 }
 
+int msj_ctx_set_params(msj_ctx *c, const msj_params *p)
+{
+    return guarded([&] {
+        if (!c || !p)
+            throw Error(MSJ_E_INVAL, "ctx or params is NULL");
+        validate_params(*p);
+        if (p->device != c->prm.device || p->storage != c->prm.storage)
+            throw Error(MSJ_E_INVAL, "device and storage are fixed when the context is created");
+        c->prm = *p;
+    });
+}
+
+int msj_ctx_load_unordered(msj_ctx *c, const double *rows, uint32_t n, uint32_t d,
+                           uint32_t *order, double *sigma)
+{
+    return guarded([&] {
+        if (!c)
+            throw Error(MSJ_E_INVAL, "ctx is NULL");
+        do_load_unordered(c, rows, n, d, order, sigma);
+    });
+}
+
 int msj_ctx_join(msj_ctx *c, double eps, msj_result *out)
 {
     return guarded([&] {
@@ -636,8 +795,24 @@ int msj_self_join(const void *data, uint32_t n, uint32_t d, double eps, const ms
     prm.output = MSJ_OUT_HOST;  // the context dies with this call
     if (out)
         std::memset(out, 0, sizeof(*out));
+    // argument errors come back before any device is touched
+    int rc = guarded([&] {
+        validate_params(prm);
+        if (!out)
+            throw Error(MSJ_E_INVAL, "out is NULL");
+        if (d == 0)
+            throw Error(MSJ_E_INVAL, "d must be positive");
+        if (n > 0 && !data)
+            throw Error(MSJ_E_INVAL, "data is NULL");
+        if ((uint64_t)n * d >= (1ull << 40))
+            throw Error(MSJ_E_OVERFLOW, "n*d too large");
+        if (!(eps > 0.0) || !std::isfinite(eps))
+            throw Error(MSJ_E_INVAL, "eps must be positive and finite");
+    });
+    if (rc != MSJ_OK)
+        return rc;
     msj_ctx *c = nullptr;
-    int rc = msj_ctx_create(&prm, &c);
+    rc = msj_ctx_create(&prm, &c);
     if (rc != MSJ_OK)
         return rc;
     rc = msj_ctx_load(c, data, n, d);
@@ -695,21 +870,7 @@ int msj_dim_order(const double *A, uint32_t n, uint32_t d, uint32_t *order, doub
             });
         for (auto &th : pool)
             th.join();
-        auto rkey = [](double x) {
-            uint64_t u;
-            std::memcpy(&u, &x, 8);
-            return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
-        };
-        std::vector<uint32_t> asc(d);
-        for (uint32_t i = 0; i < d; ++i)
-            asc[i] = i;
-        std::stable_sort(asc.begin(), asc.end(),
-                         [&](uint32_t a, uint32_t b) { return rkey(dev[a]) < rkey(dev[b]); });
-        for (uint32_t i = 0; i < d; ++i) {
-            order[i] = asc[d - 1 - i];
-            if (sigma)
-                sigma[i] = dev[asc[d - 1 - i]];
-        }
+        order_by_sigma(dev.data(), d, order, sigma);
     });
 }
 

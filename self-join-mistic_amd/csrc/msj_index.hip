@@ -92,22 +92,6 @@ inline uint64_t radix_key_f64(double x)
     return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
 }
 
-// Device twins of the conversions.
-__device__ inline uint32_t dev_cvt_u32_sat(double q)  // cvt.rzi.u32.f64
-{
-    if (!(q > 0.0))
-        return 0u;
-    if (q >= 4294967295.0)
-        return 0xFFFFFFFFu;
-    return (uint32_t)q;
-}
-__device__ inline uint32_t dev_cvt_u32_x86(double q)
-{
-    if (!(q > -9223372036854775808.0 && q < 9223372036854775808.0))
-        return 0u;
-    return (uint32_t)(uint64_t)(int64_t)q;
-}
-
 constexpr int KEY_RP = 32;  // register accumulators of the key kernel
 
 __global__ __launch_bounds__(256) void k_iota(uint32_t *p, uint32_t n)
@@ -146,15 +130,15 @@ __global__ __launch_bounds__(256) void k_rp_distmat(const T *__restrict__ xs, ui
 }
 
 // utils.cu:29-41 per column: sequential mean, then sequential sum of squares.
-// One block per column.  The additions stay in row order, made by one
-// thread; the block's other threads stream the column through a
-// double-buffered LDS tile, so the next tile's global loads are in flight
-// while the current one is summed.
+// One block per column; element (r, c) is A[r * rs + c * cs].  The additions
+// stay in row order, made by one thread; the block's other threads stream
+// the column through a double-buffered LDS tile, so the next tile's global
+// loads are in flight while the current one is summed.
 constexpr uint32_t SIG_TILE = 2048;  // rows per LDS tile
 constexpr uint32_t SIG_BLOCK = 256;
 
 __global__ __launch_bounds__(SIG_BLOCK) void k_col_sigma(const double *__restrict__ A,
-                                                          uint32_t rows, uint32_t cols,
+                                                          uint32_t rows, size_t rs, size_t cs,
                                                           double *__restrict__ sigma)
 {
     constexpr uint32_t PER = SIG_TILE / SIG_BLOCK;
@@ -170,7 +154,7 @@ __global__ __launch_bounds__(SIG_BLOCK) void k_col_sigma(const double *__restric
 #pragma unroll
             for (uint32_t k = 0; k < PER; ++k) {
                 const uint32_t r = t * SIG_TILE + k * SIG_BLOCK + tid;
-                v[k] = r < rows ? A[(size_t)r * cols + c] : 0.0;
+                v[k] = r < rows ? A[(size_t)r * rs + (size_t)c * cs] : 0.0;
             }
         };
         auto put = [&](uint32_t t) {
@@ -290,12 +274,9 @@ __global__ __launch_bounds__(256) void k_keys(const T *__restrict__ xs, uint32_t
 #pragma unroll
     for (int t = 0; t < KEY_RP; ++t)
         if ((uint32_t)t < nrp)
-            keys[(size_t)t * n + i] = dev_cvt_u32_sat(floor(sqrt(acc[t]) / eps));
-    for (uint32_t a = 0; a < naxis; ++a) {
-        double x = (double)xs[(size_t)a * n + i];
-        double q = ceil((x + 0.0000001) / eps);
-        keys[(size_t)(nrp + a) * n + i] = dev_cvt_u32_x86(q);
-    }
+            keys[(size_t)t * n + i] = rp_key(acc[t], eps);
+    for (uint32_t a = 0; a < naxis; ++a)
+        keys[(size_t)(nrp + a) * n + i] = axis_key((double)xs[(size_t)a * n + i], eps);
 }
 
 // Composite (candidate | parent node | key) for candidates [t0, t0+tcount).
@@ -620,7 +601,8 @@ void build_impl(IndexBuilder &B, const T *xs, uint32_t n, uint32_t d, double eps
         if (S > 0)
             LAUNCH(k_rp_distmat<T>, grid_for((size_t)S * ntest, 256), 256, xs, n, d, S,
                    B.test_rp.p, ntest, B.distmat.p);
-        LAUNCH(k_col_sigma, ntest, SIG_BLOCK, B.distmat.p, S, ntest, B.sigma.p);
+        LAUNCH(k_col_sigma, ntest, SIG_BLOCK, B.distmat.p, S, (size_t)ntest, (size_t)1,
+               B.sigma.p);
         LAUNCH(k_col_max<T>, d, 256, xs, n, d, S, B.maxc.p);
         MSJ_HIP(hipMemcpyAsync(h_sig.data(), B.sigma.p, ntest * sizeof(double),
                                hipMemcpyDeviceToHost, s));
@@ -659,6 +641,11 @@ void build_impl(IndexBuilder &B, const T *xs, uint32_t n, uint32_t d, double eps
 
         // ---- (2) keys of every candidate partition
         LAUNCH(k_keys<T>, grid_for(n, 256), 256, xs, n, d, B.rpT.p, nrp, naxis, eps, B.keys.p);
+        if (B.probe && B.probe->keys && B.probe->layer == l) {
+            MSJ_HIP(hipMemcpyAsync(B.probe->keys, B.keys.p, sizeof(uint32_t) * ncand * n,
+                                   hipMemcpyDeviceToHost, s));
+            MSJ_HIP(hipStreamSynchronize(s));
+        }
         reduce_max_u32(B.keys.p, (size_t)ncand * n, B.u32scalar.p, s);
         const uint32_t kmax = d2h_scalar(B.u32scalar.p, s);
         const int kb = bits_for(kmax);
@@ -718,6 +705,15 @@ void build_impl(IndexBuilder &B, const T *xs, uint32_t n, uint32_t d, double eps
         LAUNCH(k_new_nodes, grid_for(n, 256), 256, B.ck.p, B.flags.p, B.runidx.p, n, kb,
                B.nsplit.p, B.cstart.p, B.cparent.p, B.ckey.p, B.pnode2.p);
         LAUNCH(k_cfirst, grid_for(mnew, 256), 256, B.cparent.p, mnew, mold, B.cfirst.p);
+        if (B.probe && B.probe->node_keys) {
+            B.probe->cparent[l].resize(mnew);
+            B.probe->ckey[l].resize(mnew);
+            MSJ_HIP(hipMemcpyAsync(B.probe->cparent[l].data(), B.cparent.p, sizeof(uint32_t) * mnew,
+                                   hipMemcpyDeviceToHost, s));
+            MSJ_HIP(hipMemcpyAsync(B.probe->ckey[l].data(), B.ckey.p, sizeof(uint32_t) * mnew,
+                                   hipMemcpyDeviceToHost, s));
+            MSJ_HIP(hipStreamSynchronize(s));
+        }
 
         // ---- (5) adjacency of the new layer
         B.deg.ensure(mnew);
@@ -767,7 +763,45 @@ void build_impl(IndexBuilder &B, const T *xs, uint32_t n, uint32_t d, double eps
     swap_buf(out.ncalcs, B.ncalcs2);
 }
 
+__global__ void k_single_node(uint32_t n, uint32_t *nstart, uint32_t *aoff, uint32_t *adj,
+                              uint64_t *sumsz, uint64_t *ncalcs)
+{
+    if (threadIdx.x == 0) {
+        nstart[0] = 0;
+        nstart[1] = n;
+        aoff[0] = 0;
+        aoff[1] = 1;
+        adj[0] = 0;
+        sumsz[0] = n;
+        ncalcs[0] = (uint64_t)n * n;
+    }
+}
+
 }  // namespace
+
+void column_sigma(const double *A, uint32_t rows, uint32_t cols, size_t rs, size_t cs,
+                  double *sigma, hipStream_t s)
+{
+    if (cols)
+        LAUNCH(k_col_sigma, cols, SIG_BLOCK, A, rows, rs, cs, sigma);
+}
+
+void build_single_node(uint32_t n, Index &out, LayerStats &st, hipStream_t s)
+{
+    out.perm.ensure(n);
+    out.nstart.ensure(2);
+    out.aoff.ensure(2);
+    out.adj.ensure(1);
+    out.sumsz.ensure(1);
+    out.ncalcs.ensure(1);
+    LAUNCH(k_iota, grid_for(n, 256), 256, out.perm.p, n);
+    LAUNCH(k_single_node, 1, 64, n, out.nstart.p, out.aoff.p, out.adj.p, out.sumsz.p,
+           out.ncalcs.p);
+    out.M = 1;
+    out.edges = 1;
+    out.num_calcs = (uint64_t)n * n;
+    st = LayerStats{};
+}
 
 void IndexBuilder::build(const void *xs, bool f32, uint32_t n, uint32_t d, double eps,
                          const msj_params &p, Index &out, LayerStats &st, hipStream_t s)

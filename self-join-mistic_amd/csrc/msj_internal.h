@@ -12,6 +12,7 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "msj.h"
 
@@ -25,6 +26,11 @@ struct Error : std::runtime_error {
 };
 
 void hip_check(hipError_t e, const char *what, const char *file, int line);
+
+// Runs fn(arg) and maps an Error / std::bad_alloc / std::exception to an
+// msj_status, recording its text for msj_last_error() (the C ABI's rule:
+// status codes, never an abort or an exception across the boundary).
+int guarded_call(void (*fn)(void *), void *arg);
 #define MSJ_HIP(x) ::msj::hip_check((x), #x, __FILE__, __LINE__)
 
 // Device allocation that only grows; released on destruction.
@@ -149,7 +155,40 @@ struct GlibcRand {
     int32_t next();
 };
 
+// kernel.cu:895-905 key of a reference-point partition from the FP64 squared
+// distance D: floor(sqrt(D) / eps), IEEE sqrt and division, converted as
+// cvt.rzi.u32.f64 does (truncate, saturate, NaN -> 0).
+__device__ inline uint32_t rp_key(double D, double eps)
+{
+    const double q = floor(sqrt(D) / eps);
+    if (!(q > 0.0))
+        return 0u;
+    if (q >= 4294967295.0)
+        return 0xFFFFFFFFu;
+    return (uint32_t)q;
+}
+
+// nodes.cu:130 key of an axis partition: ceil((x + 1e-7) / eps), converted
+// as x86-64 host code does ((unsigned) of a double: cvttsd2si to 64 bits,
+// keep the low half; out-of-range inputs give 0x8000000000000000).
+__device__ inline uint32_t axis_key(double x, double eps)
+{
+    const double q = ceil((x + 0.0000001) / eps);
+    if (!(q > -9223372036854775808.0 && q < 9223372036854775808.0))
+        return 0u;
+    return (uint32_t)(uint64_t)(int64_t)q;
+}
+
+// Test probe of an index build (host buffers, filled when set).
+struct IndexProbe {
+    uint32_t layer = ~0u;        // layer whose candidate keys are copied out
+    uint32_t *keys = nullptr;    // [ncand * n] keys of every candidate, row order
+    bool node_keys = false;      // keep each layer's (parent, key) of the children
+    std::vector<uint32_t> cparent[MSJ_MAX_LAYERS], ckey[MSJ_MAX_LAYERS];
+};
+
 struct IndexBuilder {
+    IndexProbe *probe = nullptr;
     // per-layer scratch
     DevBuf<uint32_t> keys;       // [ncand*n] candidate keys, row order
     DevBuf<double> test_rp, distmat, sigma, maxc, rpT;
@@ -168,6 +207,16 @@ struct IndexBuilder {
     void build(const void *xs, bool f32, uint32_t n, uint32_t d, double eps, const msj_params &p,
                Index &out, LayerStats &st, hipStream_t s);
 };
+
+// utils.cu:22-41 population sigma of `cols` columns, each summed sequentially
+// in row order (bit-identical to the host loop); element (r, c) is
+// A[r * rs + c * cs] (row-major: rs = cols, cs = 1; SoA: rs = 1, cs = rows).
+void column_sigma(const double *A, uint32_t rows, uint32_t cols, size_t rs, size_t cs,
+                  double *sigma, hipStream_t s);
+
+// Candidate generator of the brute force: one node holding every row (in
+// row order), adjacent to itself only, so every pair is a candidate.
+void build_single_node(uint32_t n, Index &out, LayerStats &st, hipStream_t s);
 
 // ------------------------------------------------------------------- join
 // One wave: queries [qbeg, qend) of node x (offsets inside the node) against

@@ -18,13 +18,15 @@ OK, E_INVAL, E_NOMEM, E_HIP, E_OVERFLOW, E_UNSUPPORTED, E_NODEV = 0, -1, -2, -3,
 STORAGE_F64, STORAGE_F32 = 0, 1
 FORM_PLAIN, FORM_MFMA_BF16, FORM_MFMA_SPLIT = 0, 1, 2
 OUT_HOST, OUT_DEVICE = 0, 1
+INDEX_NODENET, INDEX_BRUTE, INDEX_COSS = 0, 1, 2
 
-# Every symbol include/msj.h declares.
+# Every symbol the C headers under include/ declare for libmsj.so.
 EXPORTS = (
     "msj_version", "msj_strerror", "msj_last_error", "msj_default_params",
     "msj_self_join", "msj_result_free", "msj_ctx_create", "msj_ctx_destroy",
     "msj_ctx_load", "msj_ctx_join", "msj_dim_order", "msj_batch_create", "msj_batch_destroy",
     "msj_batch_load", "msj_batch_run", "msj_join_batch",
+    "msj_probe_index", "msj_probe_key_math", "msj_ctx_load_unordered", "msj_ctx_set_params",
 )
 
 
@@ -43,7 +45,7 @@ class Params(C.Structure):
         ("output", C.c_uint32),
         ("verbose", C.c_uint32),
         ("count_work", C.c_uint32),
-        ("reserved0", C.c_uint32),
+        ("index", C.c_uint32),
         ("page_pairs", C.c_uint64),
     ]
 
@@ -129,6 +131,19 @@ class Result(C.Structure):
     ]
 
 
+class IndexProbe(C.Structure):
+    _fields_ = [
+        ("layer", C.c_uint32),
+        ("ncand", C.c_uint32),
+        ("num_nodes", C.c_uint32),
+        ("num_layers", C.c_uint32),
+        ("keys", C.POINTER(C.c_uint32)),
+        ("node_keys", C.POINTER(C.c_uint32)),
+        ("node_start", C.POINTER(C.c_uint32)),
+        ("node_order", C.POINTER(C.c_uint32)),
+    ]
+
+
 class MsjError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__("msj error %d (%s): %s" % (code, _strerror(code), msg))
@@ -157,6 +172,10 @@ def lib():
         L.msj_ctx_destroy.argtypes = [C.c_void_p]
         L.msj_ctx_load.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]
         L.msj_ctx_join.argtypes = [C.c_void_p, C.c_double, C.POINTER(Result)]
+        L.msj_ctx_set_params.argtypes = [C.c_void_p, C.POINTER(Params)]
+        L.msj_ctx_load_unordered.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_uint32,
+                                             C.c_uint32, C.POINTER(C.c_uint32),
+                                             C.POINTER(C.c_double)]
         L.msj_dim_order.argtypes = [C.POINTER(C.c_double), C.c_uint32, C.c_uint32,
                                     C.POINTER(C.c_uint32), C.POINTER(C.c_double)]
         L.msj_batch_create.argtypes = [C.POINTER(Params), C.POINTER(C.c_void_p)]
@@ -165,6 +184,10 @@ def lib():
         L.msj_batch_run.argtypes = [C.c_void_p, C.POINTER(Result), C.POINTER(BatchStats)]
         L.msj_join_batch.argtypes = [C.POINTER(Job), C.c_uint32, C.POINTER(Params),
                                      C.POINTER(Result)]
+        L.msj_probe_index.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_double,
+                                      C.POINTER(Params), C.POINTER(IndexProbe)]
+        L.msj_probe_key_math.argtypes = [C.POINTER(C.c_double), C.c_uint64, C.c_double,
+                                         C.POINTER(C.c_double), C.POINTER(C.c_uint32), C.c_int]
         _lib = L
     return _lib
 
@@ -273,6 +296,45 @@ def self_join(data, eps, **kw):
         lib().msj_result_free(C.byref(r))
 
 
+def probe_index(data, eps, layer=0, **kw):
+    """msj_probe_index: the GPU node-net build's candidate keys at `layer`
+    ([ncand, n], row order), the final nodes' key tuples ([M, layers]), node
+    boundaries and node order.  Runs no join."""
+    p = default_params(**kw)
+    data = _as_input(data, p.storage)
+    n, d = data.shape
+    ncand = p.rp_per_layer + min(p.numidim, d)
+    keys = np.zeros(ncand * n, dtype=np.uint32)
+    nkeys = np.zeros(n * p.maxrp, dtype=np.uint32)
+    nstart = np.zeros(n + 1, dtype=np.uint32)
+    order = np.zeros(n, dtype=np.uint32)
+    pr = IndexProbe()
+    pr.layer = layer
+    up = C.POINTER(C.c_uint32)
+    pr.keys = keys.ctypes.data_as(up)
+    pr.node_keys = nkeys.ctypes.data_as(up)
+    pr.node_start = nstart.ctypes.data_as(up)
+    pr.node_order = order.ctypes.data_as(up)
+    _check(lib().msj_probe_index(data.ctypes.data_as(C.c_void_p), n, d, float(eps), C.byref(p),
+                                 C.byref(pr)))
+    M, L = pr.num_nodes, pr.num_layers
+    return {"keys": keys[:pr.ncand * n].reshape(pr.ncand, n),
+            "node_keys": nkeys[:M * L].reshape(M, L), "node_start": nstart[:M + 1],
+            "node_order": order, "num_nodes": M, "num_layers": L}
+
+
+def probe_key_math(D, eps, device=0):
+    """msj_probe_key_math: (sqrt(D)/eps, key) computed on the GPU."""
+    D = np.ascontiguousarray(D, dtype=np.float64)
+    q = np.zeros(D.shape[0], dtype=np.float64)
+    k = np.zeros(D.shape[0], dtype=np.uint32)
+    dp = C.POINTER(C.c_double)
+    _check(lib().msj_probe_key_math(D.ctypes.data_as(dp), D.shape[0], float(eps),
+                                    q.ctypes.data_as(dp),
+                                    k.ctypes.data_as(C.POINTER(C.c_uint32)), device))
+    return q, k
+
+
 class Context:
     """Data resident in HBM across joins (msj_ctx_*)."""
 
@@ -288,6 +350,32 @@ class Context:
         n, d = data.shape
         _check(lib().msj_ctx_load(self.h, data.ctypes.data_as(C.c_void_p), n, d))
         self.n, self.d = n, d
+
+    def set_params(self, **kw):
+        """msj_ctx_set_params: change knobs (form, index, ...) for later joins."""
+        p = Params.from_buffer_copy(self.params)
+        for k, v in kw.items():
+            if not hasattr(p, k):
+                raise AttributeError("msj_params has no field %r" % k)
+            setattr(p, k, v)
+        _check(lib().msj_ctx_set_params(self.h, C.byref(p)))
+        self.params = p
+
+    def load_unordered(self, rows):
+        """msj_ctx_load_unordered: FP64 rows in file order; the dimension
+        order (utils.cu:22-53) is computed and applied on the GPU.  Returns
+        (order, sigma)."""
+        rows = np.ascontiguousarray(rows, dtype=np.float64)
+        if rows.ndim != 2:
+            raise ValueError("rows must be 2-D (n, d)")
+        n, d = rows.shape
+        order = np.zeros(d, dtype=np.uint32)
+        sigma = np.zeros(d, dtype=np.float64)
+        _check(lib().msj_ctx_load_unordered(self.h, rows.ctypes.data_as(C.POINTER(C.c_double)),
+                                            n, d, order.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                            sigma.ctypes.data_as(C.POINTER(C.c_double))))
+        self.n, self.d = n, d
+        return order, sigma
 
     def join(self, eps, copy_arrays=True):
         r = Result()

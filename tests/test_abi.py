@@ -11,14 +11,18 @@ import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADER = os.path.join(ROOT, "include", "msj.h")
+# C headers of libmsj.so (msj_nodelauncher.h belongs to the reference-side shim)
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("msj.h", "msj_probe.h")]
 PARAMS_H = os.path.join(ROOT, "include", "msj_params.h")
 
 
 def _declared():
-    txt = open(HEADER).read()
-    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(msj_[a-z_]+)\s*\(", txt)))
+    names = set()
+    for h in HEADERS:
+        txt = open(h).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        names |= set(re.findall(r"\b(msj_[a-z_0-9]+)\s*\(", txt))
+    return sorted(names)
 
 
 def test_exports_every_declared_symbol(msj_mod):
@@ -40,8 +44,10 @@ def test_struct_layout_matches_c(tmp_path, msj_mod):
         "msj_result": [f[0] for f in msj_mod.Result._fields_],
         "msj_job": [f[0] for f in msj_mod.Job._fields_],
         "msj_batch_stats": [f[0] for f in msj_mod.BatchStats._fields_],
+        "msj_index_probe": [f[0] for f in msj_mod.IndexProbe._fields_],
     }
-    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "msj.h"', "int main(void){"]
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "msj.h"',
+             '#include "msj_probe.h"', "int main(void){"]
     for st, fl in fields.items():
         lines.append('printf("%s %%zu\\n", sizeof(%s));' % (st, st))
         for f in fl:
@@ -54,7 +60,8 @@ def test_struct_layout_matches_c(tmp_path, msj_mod):
     got = dict(line.split() for line in subprocess.run(
         [str(exe)], capture_output=True, text=True, check=True).stdout.splitlines())
     py = {"msj_params": msj_mod.Params, "msj_stats": msj_mod.Stats, "msj_result": msj_mod.Result,
-          "msj_job": msj_mod.Job, "msj_batch_stats": msj_mod.BatchStats}
+          "msj_job": msj_mod.Job, "msj_batch_stats": msj_mod.BatchStats,
+          "msj_index_probe": msj_mod.IndexProbe}
     for st, cls in py.items():
         assert int(got[st]) == C.sizeof(cls), st
         for f in fields[st]:
