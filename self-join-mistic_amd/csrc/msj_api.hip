@@ -29,6 +29,8 @@ void set_error(const std::string &m) { g_last_error = m; }
 
 namespace msj {
 
+thread_local Arena *g_arena = nullptr;
+
 void hip_check(hipError_t e, const char *what, const char *file, int line)
 {
     if (e == hipSuccess)
@@ -89,11 +91,7 @@ struct msj_ctx {
     bool f32 = false;
     msj::DevBuf<unsigned char> staging;  // row-major upload
     msj::DevBuf<unsigned char> xs;       // SoA, row order
-    msj::DevBuf<double> xr;              // FP64 rows, node order
-    msj::Index ix;
-    msj::IndexBuilder builder;
-    msj::JoinRunner join;
-    msj::MfmaRunner mfma;
+    msj::JoinCore core;                  // index, join and their device state
     msj::DevBuf<uint32_t> r_counts;
     msj::DevBuf<uint64_t> r_offsets;
     msj::DevBuf<uint32_t> r_nbrs;
@@ -119,7 +117,9 @@ struct DeviceGuard {
     }
 };
 
-void validate_params(const msj_params &p)
+}  // namespace
+
+void msj::validate_params(const msj_params &p)
 {
     if (p.struct_size != sizeof(msj_params))
         throw Error(MSJ_E_INVAL, "msj_params.struct_size mismatch: call msj_default_params first");
@@ -138,6 +138,8 @@ void validate_params(const msj_params &p)
     if (p.index != MSJ_INDEX_NODENET && p.index != MSJ_INDEX_BRUTE && p.index != MSJ_INDEX_COSS)
         throw Error(MSJ_E_INVAL, "unknown candidate generator (index)");
 }
+
+namespace {
 
 template <class F>
 int guarded(F &&f)
@@ -217,6 +219,18 @@ void order_by_sigma(const double *dev, uint32_t d, uint32_t *order, double *sigm
 }
 
 }  // namespace
+
+void msj::rows_to_soa_device(const void *rows, uint32_t n, uint32_t d, bool f32, void *xs,
+                             hipStream_t s)
+{
+    const size_t total = (size_t)n * d;
+    if (total == 0)
+        return;
+    msj::check_grid(msj::grid_stride(total, 256), 256);
+    hipLaunchKernelGGL(k_soa, dim3(msj::grid_stride(total, 256)), dim3(256), 0, s,
+                       (const unsigned char *)rows, (unsigned char *)xs, n, d, f32 ? 4u : 8u);
+    MSJ_HIP(hipGetLastError());
+}
 
 void msj::load_soa(const void *data, uint32_t n, uint32_t d, bool f32,
                    msj::DevBuf<unsigned char> &xs, msj::DevBuf<unsigned char> &staging,
@@ -427,26 +441,7 @@ void do_join(msj_ctx *c, double eps, msj_result *out)
         return;
     }
 
-    // ---- index (subsystems a, b)
-    MSJ_HIP(hipEventRecord(c->ev[0], s));
-    msj::LayerStats ls;
-    if (c->prm.index == MSJ_INDEX_BRUTE)
-        msj::build_single_node(n, c->ix, ls, s);
-    else if (c->prm.index == MSJ_INDEX_COSS)
-        throw Error(MSJ_E_UNSUPPORTED, "the tree/COSS candidate generator is not built");
-    else
-        c->builder.build(c->xs.p, c->f32, n, d, eps, c->prm, c->ix, ls, s);
-    c->join.order(c->ix, c->xs.p, c->f32, n, d, s);
-    c->xr.ensure((size_t)n * msj::padded_dim(d));
-    msj::gather_node_rows(c->xs.p, c->xr.p, c->f32, c->join.jperm.p, n, d, s);
-    const bool use_mfma = c->prm.form != MSJ_FORM_PLAIN && msj::mfma_filter_applies(eps, n, d);
-    c->join.plan(c->ix, n, d, s, !use_mfma);
-    MSJ_HIP(hipEventRecord(c->ev[1], s));
-    if (use_mfma)  // part of the join: filter rows, thresholds, work items
-        c->mfma.prepare(c->ix, c->join, c->xr.p, n, d, eps, c->prm.form == MSJ_FORM_MFMA_SPLIT,
-                        s);
-
-    // ---- join (subsystem c)
+    // ---- index, join, ordered results (subsystems a-d)
     uint64_t cap = 0;
     {
         size_t freeb = 0, totalb = 0;
@@ -455,68 +450,19 @@ void do_join(msj_ctx *c, double eps, msj_result *out)
         uint64_t budget = (uint64_t)(freeb / 4) / sizeof(uint64_t);
         cap = std::max<uint64_t>(std::min(want, budget), 1);
     }
-    float kms = 0.f, mms = 0.f;
-    uint64_t launches = 0, dims = 0;
-    uint64_t R;
-    if (use_mfma) {
-        R = c->mfma.run(c->ix, c->join, c->xr.p, n, d, eps, cap, c->prm.count_work != 0, s,
-                        c->ev[2], c->ev[7], c->ev[3], &launches, &kms, &mms, &st);
-    } else {
-        R = c->join.run_plain(c->ix, c->xr.p, n, d, eps, cap, s, c->ev[2], c->ev[3], &launches,
-                              &kms, c->prm.count_work ? &dims : nullptr, c->prm.verbose != 0);
-        mms = kms;
-        st.form_used = MSJ_FORM_PLAIN;
-    }
-    st.ms_main_kernel = mms;
-    st.tile_rows = c->join.tile_rows;
-    st.dims_evaluated = dims;
+    const uint64_t R = c->core.run(c->xs.p, c->f32, n, d, eps, c->prm, cap, s, c->ev, st);
     c->last_R = R;
-    if (R >= 0xFFFFFFFFull * 16)
-        throw Error(MSJ_E_OVERFLOW, "result pair count too large");
-    MSJ_HIP(hipEventRecord(c->ev[4], s));
-
-    // ---- ordered results (subsystem d)
     c->r_counts.ensure(n);
     c->r_offsets.ensure((size_t)n + 1);
     c->r_nbrs.ensure(std::max<uint64_t>(R, 1));
-    c->join.finish(n, R, c->r_counts.p, c->r_offsets.p, c->r_nbrs.p, s);
-    MSJ_HIP(hipEventRecord(c->ev[5], s));
-    MSJ_HIP(hipStreamSynchronize(s));
-
-    st.ms_index = elapsed(c->ev[0], c->ev[1]);
-    st.ms_join = elapsed(c->ev[1], c->ev[4]);
-    st.ms_join_kernel = kms;
-    st.ms_output = elapsed(c->ev[4], c->ev[5]);
-    st.join_launches = launches;
-    st.num_nodes = c->ix.M;
-    st.num_layers = ls.layers;
-    st.num_candidates = ls.ncand;
-    st.num_pages = 0;
-    st.num_calcs = c->ix.num_calcs;
-    st.total_pairs = R;
-    for (uint32_t l = 0; l < ls.layers; ++l) {
-        st.chosen[l] = ls.chosen[l];
-        st.nodes_per_layer[l] = ls.nodes[l];
-        st.calcs_per_layer[l] = ls.calcs[l];
-        st.sse[l] = ls.sse[l];
-    }
+    c->core.finish(n, R, c->r_counts.p, c->r_offsets.p, c->r_nbrs.p, s, c->ev, st);
     out->total_pairs = R;
-
-    // node sizes / numCalcs for the reference's batch record
-    {
-        std::vector<uint32_t> hstart((size_t)c->ix.M + 1);
-        std::vector<uint64_t> hcalcs(c->ix.M);
-        MSJ_HIP(hipMemcpy(hstart.data(), c->ix.nstart.p, hstart.size() * 4, hipMemcpyDeviceToHost));
-        MSJ_HIP(hipMemcpy(hcalcs.data(), c->ix.ncalcs.p, hcalcs.size() * 8, hipMemcpyDeviceToHost));
-        reference_batch_plan(hstart, hcalcs, n, eps, c->ix.num_calcs, &st.ref_num_batches,
-                             &st.ref_var_calc);
-    }
 
     if (dev_out) {
         out->counts = c->r_counts.p;
         out->offsets = c->r_offsets.p;
         out->nbrs = c->r_nbrs.p;
-        out->node_order = c->ix.perm.p;
+        out->node_order = c->core.ix.perm.p;
     } else {
         MSJ_HIP(hipEventRecord(c->ev[6], s));
         out->counts = (uint32_t *)std::malloc(sizeof(uint32_t) * n);
@@ -534,7 +480,7 @@ void do_join(msj_ctx *c, double eps, msj_result *out)
                      c->ev[6]);
         st.num_pages = (uint32_t)page_to_host(c, out->nbrs, c->r_nbrs.p, sizeof(uint32_t) * R,
                                               page, c->ev[6]);
-        page_to_host(c, out->node_order, c->ix.perm.p, sizeof(uint32_t) * n, page, c->ev[6]);
+        page_to_host(c, out->node_order, c->core.ix.perm.p, sizeof(uint32_t) * n, page, c->ev[6]);
         st.ms_d2h = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() -
                                                               t_copy)
                         .count();
@@ -545,6 +491,86 @@ void do_join(msj_ctx *c, double eps, msj_result *out)
 }
 
 }  // namespace
+
+uint64_t msj::JoinCore::run(const void *xs, bool f32, uint32_t n, uint32_t d, double eps,
+                            const msj_params &prm, uint64_t cap, hipStream_t s, hipEvent_t *ev,
+                            msj_stats &st)
+{
+    // ---- index (subsystems a, b)
+    MSJ_HIP(hipEventRecord(ev[0], s));
+    LayerStats ls;
+    if (prm.index == MSJ_INDEX_BRUTE)
+        build_single_node(n, ix, ls, s);
+    else if (prm.index == MSJ_INDEX_COSS)
+        throw Error(MSJ_E_UNSUPPORTED, "the tree/COSS candidate generator is not built");
+    else
+        builder.build(xs, f32, n, d, eps, prm, ix, ls, s);
+    join.order(ix, xs, f32, n, d, s);
+    xr.ensure((size_t)n * padded_dim(d));
+    gather_node_rows(xs, xr.p, f32, join.jperm.p, n, d, s);
+    const bool use_mfma = prm.form != MSJ_FORM_PLAIN && mfma_filter_applies(eps, n, d);
+    join.plan(ix, n, d, s, !use_mfma);
+    MSJ_HIP(hipEventRecord(ev[1], s));
+    if (use_mfma)  // part of the join: filter rows, thresholds, work items
+        mfma.prepare(ix, join, xr.p, n, d, eps, prm.form == MSJ_FORM_MFMA_SPLIT, s);
+
+    // ---- join (subsystem c)
+    float kms = 0.f, mms = 0.f;
+    uint64_t launches = 0, dims = 0;
+    uint64_t R;
+    if (use_mfma) {
+        R = mfma.run(ix, join, xr.p, n, d, eps, cap, prm.count_work != 0, s, ev[2], ev[7], ev[3],
+                     &launches, &kms, &mms, &st);
+    } else {
+        R = join.run_plain(ix, xr.p, n, d, eps, cap, s, ev[2], ev[3], &launches, &kms,
+                           prm.count_work ? &dims : nullptr, prm.verbose != 0);
+        mms = kms;
+        st.form_used = MSJ_FORM_PLAIN;
+    }
+    if (R >= 0xFFFFFFFFull * 16)
+        throw Error(MSJ_E_OVERFLOW, "result pair count too large");
+    MSJ_HIP(hipEventRecord(ev[4], s));
+    st.ms_main_kernel = mms;
+    st.tile_rows = join.tile_rows;
+    st.dims_evaluated = dims;
+    st.ms_join_kernel = kms;
+    st.join_launches = launches;
+    st.num_nodes = ix.M;
+    st.num_layers = ls.layers;
+    st.num_candidates = ls.ncand;
+    st.num_pages = 0;
+    st.num_calcs = ix.num_calcs;
+    st.total_pairs = R;
+    for (uint32_t l = 0; l < ls.layers; ++l) {
+        st.chosen[l] = ls.chosen[l];
+        st.nodes_per_layer[l] = ls.nodes[l];
+        st.calcs_per_layer[l] = ls.calcs[l];
+        st.sse[l] = ls.sse[l];
+    }
+    // node sizes / numCalcs for the reference's batch record
+    std::vector<uint32_t> hstart((size_t)ix.M + 1);
+    std::vector<uint64_t> hcalcs(ix.M);
+    MSJ_HIP(hipMemcpyAsync(hstart.data(), ix.nstart.p, hstart.size() * 4, hipMemcpyDeviceToHost,
+                           s));
+    MSJ_HIP(hipMemcpyAsync(hcalcs.data(), ix.ncalcs.p, hcalcs.size() * 8, hipMemcpyDeviceToHost,
+                           s));
+    MSJ_HIP(hipStreamSynchronize(s));
+    reference_batch_plan(hstart, hcalcs, n, eps, ix.num_calcs, &st.ref_num_batches,
+                         &st.ref_var_calc);
+    return R;
+}
+
+void msj::JoinCore::finish(uint32_t n, uint64_t R, uint32_t *counts, uint64_t *offsets,
+                           uint32_t *nbrs, hipStream_t s, hipEvent_t *ev, msj_stats &st)
+{
+    // ---- ordered results (subsystem d)
+    join.finish(n, R, counts, offsets, nbrs, s);
+    MSJ_HIP(hipEventRecord(ev[5], s));
+    MSJ_HIP(hipStreamSynchronize(s));
+    st.ms_index = elapsed(ev[0], ev[1]);
+    st.ms_join = elapsed(ev[1], ev[4]);
+    st.ms_output = elapsed(ev[4], ev[5]);
+}
 
 int msj::guarded_call(void (*fn)(void *), void *arg)
 {
@@ -603,7 +629,7 @@ int msj_ctx_create(const msj_params *p, msj_ctx **out)
             prm = *p;
         else
             msj_default_params(&prm);
-        validate_params(prm);
+        msj::validate_params(prm);
         int ndev = 0;
         if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
             (void)hipGetLastError();
@@ -669,7 +695,7 @@ int msj_batch_create(const msj_params *p, msj_batch **out)
             prm = *p;
         else
             msj_default_params(&prm);
-        validate_params(prm);
+        msj::validate_params(prm);
         int ndev = 0;
         if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
             (void)hipGetLastError();
@@ -758,7 +784,7 @@ int msj_ctx_set_params(msj_ctx *c, const msj_params *p)
     return guarded([&] {
         if (!c || !p)
             throw Error(MSJ_E_INVAL, "ctx or params is NULL");
-        validate_params(*p);
+        msj::validate_params(*p);
         if (p->device != c->prm.device || p->storage != c->prm.storage)
             throw Error(MSJ_E_INVAL, "device and storage are fixed when the context is created");
         c->prm = *p;
@@ -797,7 +823,7 @@ int msj_self_join(const void *data, uint32_t n, uint32_t d, double eps, const ms
         std::memset(out, 0, sizeof(*out));
     // argument errors come back before any device is touched
     int rc = guarded([&] {
-        validate_params(prm);
+        msj::validate_params(prm);
         if (!out)
             throw Error(MSJ_E_INVAL, "out is NULL");
         if (d == 0)

@@ -532,6 +532,7 @@ void swap_buf(DevBuf<T> &a, DevBuf<T> &b)
 {
     std::swap(a.p, b.p);
     std::swap(a.cap, b.cap);
+    std::swap(a.borrowed, b.borrowed);
 }
 
 template <class T>

@@ -33,11 +33,39 @@ void hip_check(hipError_t e, const char *what, const char *file, int line);
 int guarded_call(void (*fn)(void *), void *arg);
 #define MSJ_HIP(x) ::msj::hip_check((x), #x, __FILE__, __LINE__)
 
-// Device allocation that only grows; released on destruction.
+// A fixed device region handed out front to back (256-byte aligned).  While
+// one is installed on the calling thread (see ArenaScope), DevBuf takes its
+// memory from it instead of hipMalloc; nothing is ever returned to it.
+struct Arena {
+    char *base = nullptr;
+    size_t cap = 0, used = 0;
+    void *take(size_t bytes)
+    {
+        const size_t at = (used + 255) & ~(size_t)255;
+        if (at > cap || bytes > cap - at)
+            throw Error(MSJ_E_NOMEM, "allocation-free scratch exhausted: " +
+                                         std::to_string(at + bytes) + " bytes needed, " +
+                                         std::to_string(cap) + " available");
+        used = at + bytes;
+        return base + at;
+    }
+};
+extern thread_local Arena *g_arena;
+
+// Installs an arena on this thread for the lifetime of the scope.
+struct ArenaScope {
+    Arena *prev;
+    explicit ArenaScope(Arena *a) : prev(g_arena) { g_arena = a; }
+    ~ArenaScope() { g_arena = prev; }
+};
+
+// Device allocation that only grows; released on destruction.  Under an
+// installed Arena the memory is borrowed from it (and never freed here).
 template <class T>
 struct DevBuf {
     T *p = nullptr;
     size_t cap = 0;
+    bool borrowed = false;
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
@@ -47,6 +75,12 @@ struct DevBuf {
         if (n <= cap)
             return;
         release();
+        if (g_arena) {
+            p = (T *)g_arena->take(n * sizeof(T));
+            borrowed = true;
+            cap = n;
+            return;
+        }
         hipError_t e = hipMalloc((void **)&p, n * sizeof(T));
         if (e != hipSuccess) {
             p = nullptr;
@@ -58,10 +92,11 @@ struct DevBuf {
     }
     void release()
     {
-        if (p)
+        if (p && !borrowed)
             (void)hipFree(p);
         p = nullptr;
         cap = 0;
+        borrowed = false;
     }
 };
 
@@ -339,8 +374,34 @@ struct MfmaRunner {
     uint64_t last_band = 0;
 };
 
+// One join's device state and its two host steps (the body of
+// nodeLauncher5, launcher.cu:3093-3532).
+struct JoinCore {
+    Index ix;
+    IndexBuilder builder;
+    JoinRunner join;
+    MfmaRunner mfma;
+    DevBuf<double> xr;           // FP64 rows in join order
+    // Candidate generation and distance kernels on xs (SoA, row order);
+    // returns the pair count R (keys in join.pairs).  Events ev[0..7] on
+    // stream s time the stages; st receives the statistics.
+    uint64_t run(const void *xs, bool f32, uint32_t n, uint32_t d, double eps,
+                 const msj_params &prm, uint64_t cap, hipStream_t s, hipEvent_t *ev,
+                 msj_stats &st);
+    // Sorts the pairs into counts[n], offsets[n+1], nbrs[R] and synchronises.
+    void finish(uint32_t n, uint64_t R, uint32_t *counts, uint64_t *offsets, uint32_t *nbrs,
+                hipStream_t s, hipEvent_t *ev, msj_stats &st);
+};
+
 // The filter's bounds hold for 2^-50 <= eps^2 <= 2^60, n < 2^30, d <= 4096.
 bool mfma_filter_applies(double eps, uint32_t n, uint32_t d);
+
+// Checks every field of msj_params (MSJ_E_INVAL with the reason).
+void validate_params(const msj_params &p);
+
+// Row-major device rows (n x d, 4- or 8-byte values) -> SoA device copy xs.
+void rows_to_soa_device(const void *rows, uint32_t n, uint32_t d, bool f32, void *xs,
+                        hipStream_t s);
 
 // Row-major host data (n x d, 4- or 8-byte values) -> SoA device copy xs.
 void load_soa(const void *data, uint32_t n, uint32_t d, bool f32, DevBuf<unsigned char> &xs,
