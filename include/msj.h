@@ -232,6 +232,29 @@ int  msj_batch_run(msj_batch *b, msj_result *outs, msj_batch_stats *st);
 /* One-shot: create, load, run, destroy. */
 int  msj_join_batch(const msj_job *jobs, uint32_t njobs, const msj_params *p, msj_result *outs);
 
+/* ---- Allocation-free join.  No counterpart in the reference: for callers
+ * that cannot allocate device memory (serving loops that must not call
+ * hipMalloc, code running between its own allocations).  The same join as
+ * msj_self_join, with the caller's device buffers in and out:
+ *   rows        device, n x d row-major, dims sigma-ordered, p->storage
+ *   counts      device [n]; offsets device [n+1]; node_order device [n]
+ *   nbrs        device [nbrs_cap]; a larger result returns MSJ_E_OVERFLOW
+ *               with stats->total_pairs = R (call again with room for R)
+ *   stats       host, may be NULL
+ * Every other device buffer comes from one file-scope device array of
+ * MSJ_NOALLOC_SCRATCH_BYTES that each device holds from the library's first
+ * kernel launch on it until the process ends; inputs whose buffers do not
+ * fit return MSJ_E_OVERFLOW (the data copies alone need n*d*(4 or 8) +
+ * n*roundup(d,4)*8 bytes) or MSJ_E_NOMEM (scratch exhausted, with the sizes
+ * in msj_last_error).  Calls on one device share the array: they run one at
+ * a time (a per-device lock), each clearing what the previous one used.
+ * Synchronous on the device's null stream; p->output is ignored. */
+#define MSJ_NOALLOC_SCRATCH_BYTES (256ull << 20)
+int  msj_self_join_noalloc(const void *rows, uint32_t n, uint32_t d, double eps,
+                           const msj_params *p, uint32_t *counts, uint64_t *offsets,
+                           uint32_t *nbrs, uint64_t nbrs_cap, uint32_t *node_order,
+                           msj_stats *stats);
+
 /* utils.cu:22-53 stddev(): dimension order by decreasing population standard
  * deviation (ties: higher index first), as main.cu:111 uses it. */
 int  msj_dim_order(const double *data, uint32_t n, uint32_t d, uint32_t *order, double *sigma);

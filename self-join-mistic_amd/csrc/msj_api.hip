@@ -515,6 +515,13 @@ uint64_t msj::JoinCore::run(const void *xs, bool f32, uint32_t n, uint32_t d, do
         mfma.prepare(ix, join, xr.p, n, d, eps, prm.form == MSJ_FORM_MFMA_SPLIT, s);
 
     // ---- join (subsystem c)
+    if (cap == 0) {
+        // under an arena: the result keys get half of what is left after the
+        // filter's band buffers, the other half is the pair sort's
+        const uint64_t left = g_arena ? (uint64_t)(g_arena->cap - g_arena->used) : 0;
+        const uint64_t spare = std::min<uint64_t>(left / 4, 32ull << 20);
+        cap = std::max<uint64_t>(1, (left - spare) / (2 * sizeof(uint64_t)));
+    }
     float kms = 0.f, mms = 0.f;
     uint64_t launches = 0, dims = 0;
     uint64_t R;

@@ -383,8 +383,9 @@ struct JoinCore {
     MfmaRunner mfma;
     DevBuf<double> xr;           // FP64 rows in join order
     // Candidate generation and distance kernels on xs (SoA, row order);
-    // returns the pair count R (keys in join.pairs).  Events ev[0..7] on
-    // stream s time the stages; st receives the statistics.
+    // returns the pair count R (keys in join.pairs).  cap: initial result-key
+    // capacity (0 under an arena: half of what it has left).  Events
+    // ev[0..7] on stream s time the stages; st receives the statistics.
     uint64_t run(const void *xs, bool f32, uint32_t n, uint32_t d, double eps,
                  const msj_params &prm, uint64_t cap, hipStream_t s, hipEvent_t *ev,
                  msj_stats &st);

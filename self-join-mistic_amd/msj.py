@@ -27,6 +27,7 @@ EXPORTS = (
     "msj_ctx_load", "msj_ctx_join", "msj_dim_order", "msj_batch_create", "msj_batch_destroy",
     "msj_batch_load", "msj_batch_run", "msj_join_batch",
     "msj_probe_index", "msj_probe_key_math", "msj_ctx_load_unordered", "msj_ctx_set_params",
+    "msj_self_join_noalloc",
 )
 
 
@@ -173,6 +174,10 @@ def lib():
         L.msj_ctx_load.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32]
         L.msj_ctx_join.argtypes = [C.c_void_p, C.c_double, C.POINTER(Result)]
         L.msj_ctx_set_params.argtypes = [C.c_void_p, C.POINTER(Params)]
+        L.msj_self_join_noalloc.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_double,
+                                            C.POINTER(Params), C.c_void_p, C.c_void_p,
+                                            C.c_void_p, C.c_uint64, C.c_void_p,
+                                            C.POINTER(Stats)]
         L.msj_ctx_load_unordered.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_uint32,
                                              C.c_uint32, C.POINTER(C.c_uint32),
                                              C.POINTER(C.c_double)]
