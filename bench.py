@@ -500,9 +500,10 @@ def run_c5(args):
         b.load(datas, eps)
         for w in range(max(1, args.warmup)):
             _, st = b.run(copy_arrays=False)
-            log("[bench] warmup %d: %.1f ms (tune %.1f ms, classes %s, epi log2 %s, grid %d)" % (
-                w, st["ms_total"], st["ms_tune"], st["class_ds"], st["class_epi_log2"],
-                st["grid_blocks"]))
+            log("[bench] warmup %d: %.1f ms (tune %.1f ms, solo jobs %d, pooled classes %s, "
+                "epi log2 %s, grid %d)" % (w, st["ms_total"], st["ms_tune"], st["solo_jobs"],
+                                           st["class_ds"], st["class_epi_log2"],
+                                           st["grid_blocks"]))
         stage = []
         t0 = time.perf_counter()
         for k in range(args.steps):
@@ -511,8 +512,9 @@ def run_c5(args):
         wall = time.perf_counter() - t0
     ms_step = wall * 1e3 / args.steps
     ms_index, ms_join, ms_output = (float(v) for v in np.mean(stage, axis=0))
-    log("[bench] batch: %.2f ms/step (index %.2f, persistent join %.2f, output %.2f ms)" % (
-        ms_step, ms_index, ms_join, ms_output))
+    log("[bench] batch: %.2f ms/step (builds done after %.2f ms; join stream %.2f ms of device "
+        "time; output stage busy %.2f ms; the three overlap)" % (ms_step, ms_index, ms_join,
+                                                                 ms_output))
 
     # the same 64 joins as sequential plain launches, data resident
     ctxs = []
@@ -563,9 +565,13 @@ def run_c5(args):
         "speedup_vs_sequential": seq_ms / ms_step, "mismatch_jobs": mism,
         "tuned": {"class_ds": st["class_ds"], "epi_log2": st["class_epi_log2"],
                   "grid_blocks": st["grid_blocks"], "work_items": st["work_items"]},
+        "solo_jobs": st["solo_jobs"],
+        "stages_overlap": "builds (2 builder streams) overlap the joins (join stream) and the "
+                          "outputs (output stream)",
         "roofline": {"bound": "fp64_valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "k_join_batch (persistent, FP64 vector ALU; MI355X FP64 peak)",
+                     "kernel": "join stream: solo k_join_hd / k_join_sym launches + persistent "
+                               "k_join_batch (FP64 vector ALU; MI355X FP64 peak)",
                      "kernel_ms": ms_join, "algorithmic_flops": flops},
     }
     if not args.no_cpu_baseline:

@@ -187,11 +187,16 @@ int  msj_ctx_join(msj_ctx *ctx, double eps, msj_result *out);
 
 /* ---- Batching form: many independent joins of uneven sizes and dimensions.
  * No counterpart in the reference (its main runs one join per process); the
- * per-job semantics are msj_self_join's.  All jobs' node nets are built
- * concurrently on worker streams, then one persistent kernel, replayed from a
- * captured hipGraph, drains a device work queue holding every job's work
- * items; tile shapes (work per item, workgroups per CU) are chosen per row
- * width by a built-in autotuner on the first run and kept by the batch. */
+ * per-job semantics are msj_self_join's.  A run is a pipeline over four
+ * streams: two builder threads build the jobs' node nets (largest first);
+ * a job large enough to fill the GPU is joined as soon as its net is built,
+ * overlapping the builds of the next jobs (the reference's NUMSTREAMS
+ * overlap, params.cuh:63); the smaller jobs share one persistent kernel per
+ * row width, replayed from a captured hipGraph, that drains a device work
+ * queue of their work items; an output thread sorts each job's pairs as its
+ * join completes.  Tile shapes of the persistent kernel (work per item,
+ * workgroups per CU) are chosen per row width by a built-in autotuner on the
+ * first run and kept by the batch. */
 typedef struct msj_job {
     const void *data;            /* host, n x d row-major, dims sigma-ordered,
                                     precision of params.storage                   */
@@ -210,13 +215,16 @@ typedef struct msj_batch_stats {
     uint64_t work_items;         /* items in the device queue                      */
     uint64_t total_pairs;        /* sum over jobs                                   */
     uint64_t num_calcs;          /* sum over jobs                                   */
-    double   ms_index;           /* host clock: all node nets, join orders, plans   */
-    double   ms_join;            /* device time of the persistent launch            */
-    double   ms_output;          /* host clock: all pair sorts and CSR arrays       */
+    double   ms_index;           /* host clock until every node net, join order and
+                                    plan is built (joins run meanwhile)            */
+    double   ms_join;            /* device time of the join stream: the solo joins
+                                    and the persistent launch                      */
+    double   ms_output;          /* host clock spent on pair sorts and CSR arrays
+                                    (overlapping later joins)                      */
     double   ms_tune;            /* host clock: autotuning (first run of a class)   */
     double   ms_total;           /* host clock of msj_batch_run                     */
     uint32_t graph_replays;      /* launches through the captured graph             */
-    uint32_t pad0;
+    uint32_t solo_jobs;          /* jobs joined alone (enough work to fill the GPU)  */
 } msj_batch_stats;
 
 typedef struct msj_batch msj_batch;

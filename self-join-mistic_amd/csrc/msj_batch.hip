@@ -1,38 +1,44 @@
-// msj_batch.hip -- batching form: many independent joins in one persistent
-// launch.
+// msj_batch.hip -- batching form: many independent joins on one GPU.
 //
-// A step (batch_run):
-//   1. node nets: every job's index build, join order, FP64 rows and work
-//      items, spread over worker threads, each with its own stream (the
-//      builders' per-layer host round trips of different jobs overlap);
-//   2. join: per row-width class one persistent kernel (grid = workgroups
-//      per CU x CUs) whose waves pop work items from the class's part of a
-//      device queue (one atomic ticket per item; the part is cut into one
-//      segment per XCD) holding the items of all jobs; an item runs the
-//      plain form's per-item code for its job's row
-//      width (msj_join_items.h), and its result keys go to that job's pair
-//      buffer.  The reset of the counters and the class kernels (concurrent
-//      branches) are captured once as a hipGraph and replayed (the job table
-//      they read is refreshed by a copy before each replay);
-//   3. output: every job's pair sort and CSR arrays, on the worker streams.
+// A step (batch_run) is a pipeline over four streams (the GPU grants a
+// process four hardware queues):
+//   builders (two threads, a stream each): every job's node net, join
+//     order, FP64 rows and work items, largest jobs first;
+//   join stream: a job whose candidate evaluations fill the GPU on their own
+//     (MSJ_SOLO_CALCS or more) is joined as soon as its build is done, by the
+//     plain kernel of msj_self_join, while the builders go on with the next
+//     jobs (the build of job k+1 overlaps the join of job k, as the
+//     reference's NUMSTREAMS batches overlap, params.cuh:63); the other jobs
+//     wait for one persistent kernel per row-width class whose waves pop
+//     work items from the class's part of a device queue (one atomic ticket
+//     per item; the part is cut into one segment per XCD) holding all of
+//     their items; the reset of the queue and the class kernels are captured
+//     once as a hipGraph and replayed (the job table they read is refreshed
+//     by a copy before each replay);
+//   output (one thread, its own stream): as each job's join completes, its
+//     pair sort and CSR arrays (and the copies to host).
 // Per-job results are those of msj_self_join (the same per-item code, the
 // same node nets).
 //
 // Autotuner.  The tile shape of the plain form is the work per item (query
 // rows x candidate columns, set by the lane evaluations per item) and the
-// number of resident workgroups per CU.  For each class not tuned yet, the
-// class's jobs are planned with 2^16, 2^18 and 2^20 evaluations per item and
-// the class kernel is timed alone on a prefix of its queue; then, with the
-// best of these, 1, 2 and 4 workgroups per CU.  The choices are kept by the
-// batch for later runs.
+// number of resident workgroups per CU.  For each class of the persistent
+// launch not tuned yet, its jobs are planned with 2^16, 2^18 and 2^20
+// evaluations per item and the class kernel is timed alone on a prefix of
+// its queue; then, with the best of these, 1, 2 and 4 workgroups per CU.
+// The choices are kept by the batch for later runs.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <exception>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -171,6 +177,52 @@ struct BatchJob {
     DevBuf<uint32_t> nbrs;
     uint64_t R = 0, last_R = 0;
     float ms_index = 0.f;
+    bool solo = false;            // joined alone by the single-join kernel
+    hipEvent_t done = nullptr;    // its join is complete (recorded on the join stream)
+    ~BatchJob()
+    {
+        if (done)
+            (void)hipEventDestroy(done);
+    }
+};
+
+// A job whose candidate evaluations reach this fills the GPU with its own
+// kernel; smaller ones share the persistent launch.
+constexpr uint64_t MSJ_SOLO_CALCS = 1ull << 33;
+
+// A closable queue of job indices between the pipeline's threads.
+struct JobQueue {
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<uint32_t> q;
+    bool closed = false;
+    void push(uint32_t j)
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            q.push_back(j);
+        }
+        cv.notify_one();
+    }
+    void close()
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            closed = true;
+        }
+        cv.notify_all();
+    }
+    // false once closed and drained
+    bool pop(uint32_t &j)
+    {
+        std::unique_lock<std::mutex> lk(mu);
+        cv.wait(lk, [&] { return !q.empty() || closed; });
+        if (q.empty())
+            return false;
+        j = q.front();
+        q.pop_front();
+        return true;
+    }
 };
 
 struct Worker {
@@ -228,14 +280,14 @@ void launch_class(const Cls &c, uint32_t grid, const JobView *views, const uint6
 struct BatchEngine {
     msj_params prm;
     std::vector<std::unique_ptr<BatchJob>> jobs;
-    std::vector<std::unique_ptr<Worker>> workers;
-    hipStream_t stream = nullptr;
-    hipStream_t cstream[MAX_CLASSES] = {};  // one per class: the branches of the graph
-    hipEvent_t ev[2] = {}, fork = nullptr, join_ev[MAX_CLASSES] = {};
+    std::vector<std::unique_ptr<Worker>> workers;  // builders, a stream each
+    hipStream_t stream = nullptr;   // joins: solo kernels and the persistent launch
+    hipStream_t ostream = nullptr;  // outputs
+    hipEvent_t ev[2] = {}, oev[2] = {};
     DevBuf<JobView> views;
     DevBuf<uint64_t> job_off;
     DevBuf<unsigned long long> queues, rcounts;
-    std::vector<uint32_t> order;  // table position -> job (sorted by padded width)
+    std::vector<uint32_t> order;  // table position -> job (the pooled jobs by padded width)
     std::vector<Cls> cls;
     std::map<uint32_t, std::pair<uint64_t, uint32_t>> tuned;  // ds -> (epi, wg)
     int cus = 0;
@@ -268,12 +320,11 @@ struct BatchEngine {
                 std::rethrow_exception(e);
     }
 
-    // Classes and the job table (sorted by padded width) from the plans.
-    void upload_table()
+    // Classes and the job table of the jobs `sel` (sorted by padded width)
+    // from their plans.
+    void upload_table(const std::vector<uint32_t> &sel)
     {
-        order.resize(jobs.size());
-        for (uint32_t j = 0; j < order.size(); ++j)
-            order[j] = j;
+        order = sel;
         std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
             return padded_dim(jobs[a]->d) < padded_dim(jobs[b]->d);
         });
@@ -310,23 +361,19 @@ struct BatchEngine {
 
     uint32_t grid_for_wg(uint32_t wg) const { return (uint32_t)std::max(1, cus) * wg; }
 
-    // Reset, then one persistent kernel per class on its own stream (the
-    // classes run concurrently), joined back on `stream`.
+    // Reset, then one persistent kernel per class, one after the other on
+    // the join stream.
     void enqueue_all(uint64_t limit, int only)
     {
         hipStream_t s = stream;
-        const uint32_t nj = (uint32_t)jobs.size();
+        const uint32_t nj = (uint32_t)order.size();
         LAUNCH(k_batch_reset, grid_for(std::max<uint32_t>(nj, MAX_CLASSES * NSEG), 64), 64, views.p,
                nj, queues.p, MAX_CLASSES * NSEG);
-        MSJ_HIP(hipEventRecord(fork, s));
         for (uint32_t c = 0; c < cls.size(); ++c) {
             if (only >= 0 && (int)c != only)
                 continue;
-            MSJ_HIP(hipStreamWaitEvent(cstream[c], fork, 0));
             launch_class(cls[c], grid_for_wg(cls[c].wg), views.p, job_off.p,
-                         queues.p + c * NSEG, limit, cstream[c]);
-            MSJ_HIP(hipEventRecord(join_ev[c], cstream[c]));
-            MSJ_HIP(hipStreamWaitEvent(s, join_ev[c], 0));
+                         queues.p + c * NSEG, limit, s);
         }
     }
 
@@ -341,12 +388,13 @@ struct BatchEngine {
         return ms;
     }
 
-    // The whole join through the captured graph (captured again when the
-    // table's address, the jobs, the classes or their grids change).
-    float launch_graph()
+    // The persistent launch through the captured graph (captured again when
+    // the table's address, the jobs, the classes or their grids change).
+    void launch_graph()
     {
-        std::vector<uint64_t> key = {(uint64_t)(uintptr_t)views.p, (uint64_t)(uintptr_t)job_off.p,
-                                     jobs.size()};
+        std::vector<uint64_t> key = {(uint64_t)(uintptr_t)views.p, (uint64_t)(uintptr_t)job_off.p};
+        for (uint32_t j : order)
+            key.push_back(j);
         for (auto &c : cls) {
             key.push_back(c.ds);
             key.push_back(c.jb);
@@ -366,38 +414,34 @@ struct BatchEngine {
             MSJ_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
             graph_key = key;
         }
-        MSJ_HIP(hipEventRecord(ev[0], stream));
         MSJ_HIP(hipGraphLaunch(gexec, stream));
-        MSJ_HIP(hipEventRecord(ev[1], stream));
-        MSJ_HIP(hipEventSynchronize(ev[1]));
-        float ms = 0.f;
-        MSJ_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
-        return ms;
     }
 
-    void replan(uint32_t key_ds, uint64_t e)
+    // Plans the pooled jobs of width class key_ds with e evaluations per item.
+    void replan(const std::vector<uint32_t> &pooled, uint32_t key_ds, uint64_t e)
     {
-        parallel([&](uint32_t j, Worker &W) {
+        for (uint32_t j : pooled) {
             BatchJob &B = *jobs[j];
             const uint32_t ds = padded_dim(B.d);
             if ((ds <= 32 ? ds : 0u) == key_ds)
-                B.join.plan(B.ix, B.n, B.d, W.s, true, e);
-        });
+                B.join.plan(B.ix, B.n, B.d, stream, true, e);
+        }
     }
 
-    // Autotuning of the classes not tuned yet: lane evaluations per item
-    // (2^16, 2^18, 2^20, with 2 workgroups per CU), then workgroups per CU
-    // (1, 2, 4); each trial runs the class alone over a prefix of its queue
-    // (an eighth, at least 4096 items), timed per full queue.
-    void tune()
+    // Autotuning of the classes of the persistent launch not tuned yet:
+    // lane evaluations per item (2^16, 2^18, 2^20, with 2 workgroups per
+    // CU), then workgroups per CU (1, 2, 4); each trial runs the class alone
+    // over a prefix of its queue (an eighth, at least 4096 items), timed per
+    // full queue.
+    void tune(const std::vector<uint32_t> &pooled)
     {
-        upload_table();
+        upload_table(pooled);
         for (uint32_t c = 0; c < cls.size(); ++c) {
             const uint32_t key = cls[c].ds <= 32 ? cls[c].ds : 0u;
             if (tuned.count(key))
                 continue;
             auto trial = [&](uint32_t wg) {
-                upload_table();
+                upload_table(pooled);
                 cls[c].wg = wg;
                 uint64_t items = 0;
                 for (uint32_t k = cls[c].jb; k < cls[c].je; ++k)
@@ -409,14 +453,14 @@ struct BatchEngine {
             double best = -1.0;
             uint64_t best_e = 1ull << 18;
             for (uint64_t e : {1ull << 16, 1ull << 18, 1ull << 20}) {
-                replan(key, e);
+                replan(pooled, key, e);
                 const double t = trial(2);
                 if (best < 0 || t < best) {
                     best = t;
                     best_e = e;
                 }
             }
-            replan(key, best_e);
+            replan(pooled, key, best_e);
             best = -1.0;
             uint32_t best_wg = 2;
             for (uint32_t wg : {1u, 2u, 4u}) {
@@ -435,6 +479,9 @@ struct BatchEngine {
         (void)hipSetDevice(prm.device);
         if (stream)
             (void)hipStreamSynchronize(stream);
+        if (ostream)
+            (void)hipStreamSynchronize(ostream);
+        jobs.clear();
         if (gexec)
             (void)hipGraphExecDestroy(gexec);
         if (graph)
@@ -444,17 +491,14 @@ struct BatchEngine {
                 (void)hipStreamSynchronize(w->s);
                 (void)hipStreamDestroy(w->s);
             }
-        for (auto &cs : cstream)
-            if (cs)
-                (void)hipStreamDestroy(cs);
         for (auto &e : ev)
             if (e)
                 (void)hipEventDestroy(e);
-        for (auto &e : join_ev)
+        for (auto &e : oev)
             if (e)
                 (void)hipEventDestroy(e);
-        if (fork)
-            (void)hipEventDestroy(fork);
+        if (ostream)
+            (void)hipStreamDestroy(ostream);
         if (stream)
             (void)hipStreamDestroy(stream);
     }
@@ -465,17 +509,15 @@ BatchEngine *batch_create(const msj_params &p)
     std::unique_ptr<BatchEngine> b(new BatchEngine());
     b->prm = p;
     MSJ_HIP(hipStreamCreate(&b->stream));
-    for (auto &cs : b->cstream)
-        MSJ_HIP(hipStreamCreate(&cs));
+    MSJ_HIP(hipStreamCreate(&b->ostream));
     for (auto &e : b->ev)
         MSJ_HIP(hipEventCreate(&e));
-    for (auto &e : b->join_ev)
-        MSJ_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    MSJ_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+    for (auto &e : b->oev)
+        MSJ_HIP(hipEventCreate(&e));
     int cus = 0;
     MSJ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p.device));
     b->cus = cus;
-    const uint32_t W = 8;
+    const uint32_t W = 2;  // with the join and output streams: four streams
     for (uint32_t w = 0; w < W; ++w) {
         b->workers.emplace_back(new Worker());
         MSJ_HIP(hipStreamCreate(&b->workers.back()->s));
@@ -506,6 +548,7 @@ void batch_load(BatchEngine *b, const msj_job *jobs, uint32_t njobs)
         b->jobs.back()->n = jobs[j].n;
         b->jobs.back()->d = jobs[j].d;
         b->jobs.back()->eps = jobs[j].eps;
+        MSJ_HIP(hipEventCreateWithFlags(&b->jobs.back()->done, hipEventDisableTiming));
     }
     b->parallel([&](uint32_t j, Worker &W) {
         BatchJob &B = *b->jobs[j];
@@ -516,6 +559,79 @@ void batch_load(BatchEngine *b, const msj_job *jobs, uint32_t njobs)
     b->rcounts.ensure(std::max<uint32_t>(njobs, 1));
 }
 
+namespace {
+
+// The output stage of one job whose join is complete: its pair count (a job
+// whose pair buffer overflowed is joined again alone with an exact buffer),
+// the pair sort and CSR arrays, and the result fields.
+void finish_job(BatchEngine *b, uint32_t j, msj_result &o, const msj_batch_stats &S)
+{
+    BatchJob &B = *b->jobs[j];
+    hipStream_t s = b->ostream;
+    const bool dev_out = b->prm.output == MSJ_OUT_DEVICE;
+    unsigned long long R = 0;
+    MSJ_HIP(hipMemcpyAsync(&R, B.join.counter.p, sizeof(R), hipMemcpyDeviceToHost, s));
+    MSJ_HIP(hipStreamSynchronize(s));
+    B.R = R;
+    if (B.R > B.join.pairs.cap) {
+        B.join.pairs.ensure(B.R);
+        uint64_t launches = 0;
+        float kms = 0.f;
+        B.R = B.join.run_plain(B.ix, B.xr.p, B.n, B.d, B.eps, B.R, s, b->oev[0], b->oev[1],
+                               &launches, &kms, nullptr);
+    }
+    B.last_R = B.R;
+    B.counts.ensure(B.n);
+    B.offsets.ensure((size_t)B.n + 1);
+    B.nbrs.ensure(std::max<uint64_t>(B.R, 1));
+    B.join.finish(B.n, B.R, B.counts.p, B.offsets.p, B.nbrs.p, s);
+    o.n = B.n;
+    o.total_pairs = B.R;
+    o.stats.num_nodes = B.ix.M;
+    o.stats.num_layers = B.ls.layers;
+    o.stats.num_candidates = B.ls.ncand;
+    o.stats.num_calcs = B.ix.num_calcs;
+    o.stats.total_pairs = B.R;
+    o.stats.tile_rows = B.join.tile_rows;
+    for (uint32_t l = 0; l < B.ls.layers; ++l) {
+        o.stats.chosen[l] = B.ls.chosen[l];
+        o.stats.nodes_per_layer[l] = B.ls.nodes[l];
+        o.stats.calcs_per_layer[l] = B.ls.calcs[l];
+        o.stats.sse[l] = B.ls.sse[l];
+    }
+    o.stats.ms_index = B.ms_index;
+    o.stats.join_launches = 1;
+    o.stats.form_used = MSJ_FORM_PLAIN;
+    if (dev_out) {
+        MSJ_HIP(hipStreamSynchronize(s));
+        o.on_device = 1;
+        o.counts = B.counts.p;
+        o.offsets = B.offsets.p;
+        o.nbrs = B.nbrs.p;
+        o.node_order = B.ix.perm.p;
+    } else {
+        o.counts = (uint32_t *)std::malloc(sizeof(uint32_t) * B.n);
+        o.offsets = (uint64_t *)std::malloc(sizeof(uint64_t) * ((size_t)B.n + 1));
+        o.nbrs = (uint32_t *)std::malloc(sizeof(uint32_t) * std::max<uint64_t>(B.R, 1));
+        o.node_order = (uint32_t *)std::malloc(sizeof(uint32_t) * B.n);
+        if (!o.counts || !o.offsets || !o.nbrs || !o.node_order)
+            throw std::bad_alloc();
+        MSJ_HIP(hipMemcpyAsync(o.counts, B.counts.p, sizeof(uint32_t) * B.n,
+                               hipMemcpyDeviceToHost, s));
+        MSJ_HIP(hipMemcpyAsync(o.offsets, B.offsets.p, sizeof(uint64_t) * ((size_t)B.n + 1),
+                               hipMemcpyDeviceToHost, s));
+        if (B.R)
+            MSJ_HIP(hipMemcpyAsync(o.nbrs, B.nbrs.p, sizeof(uint32_t) * B.R,
+                                   hipMemcpyDeviceToHost, s));
+        MSJ_HIP(hipMemcpyAsync(o.node_order, B.ix.perm.p, sizeof(uint32_t) * B.n,
+                               hipMemcpyDeviceToHost, s));
+        MSJ_HIP(hipStreamSynchronize(s));
+    }
+    (void)S;
+}
+
+}  // namespace
+
 void batch_run(BatchEngine *b, msj_result *outs, msj_batch_stats *st)
 {
     const double t0 = now_ms();
@@ -523,133 +639,164 @@ void batch_run(BatchEngine *b, msj_result *outs, msj_batch_stats *st)
     std::memset(&S, 0, sizeof(S));
     const uint32_t nj = (uint32_t)b->jobs.size();
     const bool f32 = b->prm.storage == MSJ_STORAGE_F32;
-    const bool dev_out = b->prm.output == MSJ_OUT_DEVICE;
     S.njobs = nj;
     S.workers = (uint32_t)b->workers.size();
     for (uint32_t j = 0; j < nj; ++j)
         std::memset(&outs[j], 0, sizeof(msj_result));
 
-    // 1. node nets, join orders, rows, items
-    b->parallel([&](uint32_t j, Worker &W) {
-        BatchJob &B = *b->jobs[j];
-        const double a = now_ms();
-        B.ls = LayerStats();
-        W.builder.build(B.xs.p, f32, B.n, B.d, B.eps, b->prm, B.ix, B.ls, W.s);
-        B.join.order(B.ix, B.xs.p, f32, B.n, B.d, W.s);
-        B.xr.ensure((size_t)B.n * padded_dim(B.d));
-        gather_node_rows(B.xs.p, B.xr.p, f32, B.join.jperm.p, B.n, B.d, W.s);
-        const uint32_t ds = padded_dim(B.d);
-        const auto it = b->tuned.find(ds <= 32 ? ds : 0u);
-        B.join.plan(B.ix, B.n, B.d, W.s, true, it == b->tuned.end() ? 0 : it->second.first);
-        B.join.counter.ensure(1);
-        const uint64_t cap = std::max<uint64_t>(B.last_R + B.last_R / 8, 64ull * B.n + 4096);
-        if (B.join.pairs.cap < cap)
-            B.join.pairs.ensure(cap);
-        MSJ_HIP(hipStreamSynchronize(W.s));
-        B.ms_index = (float)(now_ms() - a);
+    // build order: largest first (their joins start first)
+    std::vector<uint32_t> border(nj);
+    for (uint32_t j = 0; j < nj; ++j)
+        border[j] = j;
+    std::stable_sort(border.begin(), border.end(),
+                     [&](uint32_t a, uint32_t c) { return b->jobs[a]->n > b->jobs[c]->n; });
+
+    JobQueue built, finished;
+    std::atomic<uint32_t> next{0};
+    std::mutex err_mu;
+    std::exception_ptr err;
+    auto fail = [&](std::exception_ptr e) {
+        std::lock_guard<std::mutex> lk(err_mu);
+        if (!err)
+            err = e;
+    };
+    std::atomic<bool> failed{false};
+
+    // builders: node nets, join orders, rows, items
+    std::vector<std::thread> builders;
+    std::atomic<uint32_t> builders_left{(uint32_t)b->workers.size()};
+    for (uint32_t w = 0; w < b->workers.size(); ++w)
+        builders.emplace_back([&, w] {
+            Worker &W = *b->workers[w];
+            try {
+                MSJ_HIP(hipSetDevice(b->prm.device));
+                for (uint32_t k; !failed && (k = next++) < nj;) {
+                    const uint32_t j = border[k];
+                    BatchJob &B = *b->jobs[j];
+                    const double a = now_ms();
+                    B.ls = LayerStats();
+                    W.builder.build(B.xs.p, f32, B.n, B.d, B.eps, b->prm, B.ix, B.ls, W.s);
+                    B.join.order(B.ix, B.xs.p, f32, B.n, B.d, W.s);
+                    B.xr.ensure((size_t)B.n * padded_dim(B.d));
+                    gather_node_rows(B.xs.p, B.xr.p, f32, B.join.jperm.p, B.n, B.d, W.s);
+                    B.solo = B.ix.num_calcs >= MSJ_SOLO_CALCS;
+                    const uint32_t ds = padded_dim(B.d);
+                    const auto it = b->tuned.find(ds <= 32 ? ds : 0u);
+                    const uint64_t epi = B.solo || it == b->tuned.end() ? 0 : it->second.first;
+                    B.join.plan(B.ix, B.n, B.d, W.s, true, epi);
+                    B.join.counter.ensure(1);
+                    const uint64_t cap = std::max<uint64_t>(B.last_R + B.last_R / 8, 64ull * B.n + 4096);
+                    if (B.join.pairs.cap < cap)
+                        B.join.pairs.ensure(cap);
+                    MSJ_HIP(hipStreamSynchronize(W.s));
+                    B.ms_index = (float)(now_ms() - a);
+                    built.push(j);
+                }
+            } catch (...) {
+                failed = true;
+                fail(std::current_exception());
+            }
+            if (--builders_left == 0)
+                built.close();
+        });
+
+    // output stage: every job as its join completes
+    double out_ms = 0.0;
+    std::thread outputs([&] {
+        try {
+            MSJ_HIP(hipSetDevice(b->prm.device));
+            uint32_t j;
+            while (finished.pop(j)) {
+                if (failed)
+                    continue;
+                MSJ_HIP(hipEventSynchronize(b->jobs[j]->done));
+                const double a = now_ms();
+                finish_job(b, j, outs[j], S);
+                out_ms += now_ms() - a;
+            }
+        } catch (...) {
+            failed = true;
+            fail(std::current_exception());
+        }
     });
-    const double t1 = now_ms();
-    S.ms_index = t1 - t0;
 
-    // autotuning of the row widths not seen before
-    bool need = false;
-    for (auto &J : b->jobs) {
-        const uint32_t ds = padded_dim(J->d);
-        need = need || !b->tuned.count(ds <= 32 ? ds : 0u);
+    // dispatch: solo jobs as their builds finish, then the persistent launch
+    double t_built = t0, t_tune = 0.0;
+    bool first = true;
+    std::vector<uint32_t> pooled;
+    try {
+        hipStream_t s = b->stream;
+        uint32_t j;
+        while (built.pop(j)) {
+            if (failed)
+                continue;
+            BatchJob &B = *b->jobs[j];
+            if (!B.solo) {
+                pooled.push_back(j);
+                continue;
+            }
+            if (first) {
+                MSJ_HIP(hipEventRecord(b->ev[0], s));
+                first = false;
+            }
+            B.join.launch_plain(B.ix, B.xr.p, B.n, B.d, B.eps, s);
+            MSJ_HIP(hipEventRecord(B.done, s));
+            ++S.solo_jobs;
+            finished.push(j);
+        }
+        t_built = now_ms();
+        if (!failed && !pooled.empty()) {
+            // autotuning of the pooled row widths not seen before
+            bool need = false;
+            for (uint32_t k : pooled) {
+                const uint32_t ds = padded_dim(b->jobs[k]->d);
+                need = need || !b->tuned.count(ds <= 32 ? ds : 0u);
+            }
+            if (need) {
+                const double a = now_ms();
+                MSJ_HIP(hipStreamSynchronize(s));  // the solo joins first: tuning times alone
+                b->tune(pooled);
+                t_tune = now_ms() - a;
+            }
+            b->upload_table(pooled);
+            if (first) {
+                MSJ_HIP(hipEventRecord(b->ev[0], s));
+                first = false;
+            }
+            b->launch_graph();
+            S.graph_replays = 1;
+            for (uint32_t k : pooled) {
+                MSJ_HIP(hipEventRecord(b->jobs[k]->done, s));
+                finished.push(k);
+            }
+        }
+        if (!first)
+            MSJ_HIP(hipEventRecord(b->ev[1], s));
+    } catch (...) {
+        failed = true;
+        fail(std::current_exception());
     }
-    if (need) {
-        b->tune();
-        S.ms_tune = now_ms() - t1;
+    finished.close();
+    for (auto &t : builders)
+        t.join();
+    outputs.join();
+    if (err) {
+        for (uint32_t k = 0; k < nj; ++k)
+            if (!outs[k].on_device)
+                msj_result_free(&outs[k]);
+        std::rethrow_exception(err);
     }
-
-    // 2. the persistent join over all jobs, from the captured graph
-    b->upload_table();
-    S.ms_join = b->launch_graph();
-    S.graph_replays = 1;
+    if (!first) {
+        MSJ_HIP(hipEventSynchronize(b->ev[1]));
+        float ms = 0.f;
+        MSJ_HIP(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+        S.ms_join = ms;
+    }
+    const double t_end = now_ms();
+    S.ms_index = t_built - t0;
+    S.ms_tune = t_tune;
     for (auto &c : b->cls)
         S.grid_blocks += b->grid_for_wg(c.wg);
-    std::vector<unsigned long long> Rs(nj);
-    {
-        hipStream_t s = b->stream;
-        LAUNCH(k_batch_counts, grid_for(std::max<uint32_t>(nj, 1), 64), 64, b->views.p, nj,
-               b->rcounts.p);
-        if (nj)
-            MSJ_HIP(hipMemcpyAsync(Rs.data(), b->rcounts.p, nj * sizeof(unsigned long long),
-                                   hipMemcpyDeviceToHost, s));
-        MSJ_HIP(hipStreamSynchronize(s));
-    }
-    std::vector<unsigned long long> R(nj);  // table order -> job order
-    for (uint32_t k = 0; k < nj; ++k)
-        R[b->order[k]] = Rs[k];
-
-    // 3. outputs (a job whose pair buffer overflowed is rerun alone with an
-    // exact buffer)
-    const double t2 = now_ms();
-    b->parallel([&](uint32_t j, Worker &W) {
-        BatchJob &B = *b->jobs[j];
-        B.R = R[j];
-        if (B.R > B.join.pairs.cap) {
-            B.join.pairs.ensure(B.R);
-            hipEvent_t e0, e1;
-            MSJ_HIP(hipEventCreate(&e0));
-            MSJ_HIP(hipEventCreate(&e1));
-            uint64_t launches = 0;
-            float kms = 0.f;
-            B.R = B.join.run_plain(B.ix, B.xr.p, B.n, B.d, B.eps, B.R, W.s, e0, e1, &launches,
-                                   &kms, nullptr);
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
-        }
-        B.last_R = B.R;
-        B.counts.ensure(B.n);
-        B.offsets.ensure((size_t)B.n + 1);
-        B.nbrs.ensure(std::max<uint64_t>(B.R, 1));
-        B.join.finish(B.n, B.R, B.counts.p, B.offsets.p, B.nbrs.p, W.s);
-        msj_result &o = outs[j];
-        o.n = B.n;
-        o.total_pairs = B.R;
-        o.stats.num_nodes = B.ix.M;
-        o.stats.num_layers = B.ls.layers;
-        o.stats.num_candidates = B.ls.ncand;
-        o.stats.num_calcs = B.ix.num_calcs;
-        o.stats.total_pairs = B.R;
-        for (uint32_t l = 0; l < B.ls.layers; ++l) {
-            o.stats.chosen[l] = B.ls.chosen[l];
-            o.stats.nodes_per_layer[l] = B.ls.nodes[l];
-            o.stats.calcs_per_layer[l] = B.ls.calcs[l];
-            o.stats.sse[l] = B.ls.sse[l];
-        }
-        o.stats.ms_index = B.ms_index;
-        o.stats.ms_join = S.ms_join;
-        o.stats.ms_join_kernel = S.ms_join;
-        o.stats.join_launches = 1;
-        o.stats.form_used = MSJ_FORM_PLAIN;
-        if (dev_out) {
-            o.on_device = 1;
-            o.counts = B.counts.p;
-            o.offsets = B.offsets.p;
-            o.nbrs = B.nbrs.p;
-            o.node_order = B.ix.perm.p;
-        } else {
-            o.counts = (uint32_t *)std::malloc(sizeof(uint32_t) * B.n);
-            o.offsets = (uint64_t *)std::malloc(sizeof(uint64_t) * ((size_t)B.n + 1));
-            o.nbrs = (uint32_t *)std::malloc(sizeof(uint32_t) * std::max<uint64_t>(B.R, 1));
-            o.node_order = (uint32_t *)std::malloc(sizeof(uint32_t) * B.n);
-            if (!o.counts || !o.offsets || !o.nbrs || !o.node_order)
-                throw std::bad_alloc();
-            MSJ_HIP(hipMemcpyAsync(o.counts, B.counts.p, sizeof(uint32_t) * B.n,
-                                   hipMemcpyDeviceToHost, W.s));
-            MSJ_HIP(hipMemcpyAsync(o.offsets, B.offsets.p, sizeof(uint64_t) * ((size_t)B.n + 1),
-                                   hipMemcpyDeviceToHost, W.s));
-            if (B.R)
-                MSJ_HIP(hipMemcpyAsync(o.nbrs, B.nbrs.p, sizeof(uint32_t) * B.R,
-                                       hipMemcpyDeviceToHost, W.s));
-            MSJ_HIP(hipMemcpyAsync(o.node_order, B.ix.perm.p, sizeof(uint32_t) * B.n,
-                                   hipMemcpyDeviceToHost, W.s));
-            MSJ_HIP(hipStreamSynchronize(W.s));
-        }
-    });
-    S.ms_output = now_ms() - t2;
     for (uint32_t j = 0; j < nj; ++j) {
         S.total_pairs += b->jobs[j]->R;
         S.num_calcs += b->jobs[j]->ix.num_calcs;
@@ -664,9 +811,13 @@ void batch_run(BatchEngine *b, msj_result *outs, msj_batch_stats *st)
         ++k;
     }
     S.classes = k;
-    S.ms_total = now_ms() - t0;
-    for (uint32_t j = 0; j < nj; ++j)
+    S.ms_total = t_end - t0;
+    S.ms_output = out_ms;
+    for (uint32_t j = 0; j < nj; ++j) {
         outs[j].stats.ms_total = S.ms_total;
+        outs[j].stats.ms_join = S.ms_join;
+        outs[j].stats.ms_join_kernel = S.ms_join;
+    }
     if (st)
         *st = S;
 }

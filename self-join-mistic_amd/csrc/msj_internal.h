@@ -307,6 +307,11 @@ struct JoinRunner {
                        uint64_t cap_hint, hipStream_t s, hipEvent_t k0, hipEvent_t k1,
                        uint64_t *launches, float *kernel_ms, uint64_t *dims_evaluated,
                        bool progress = false);
+    // The plain kernel of the planned items, asynchronously on s (counter
+    // zeroed first); the pair count is *counter once s gets there, and a
+    // count above pairs.cap means a rerun with run_plain.
+    void launch_plain(const Index &ix, const double *xr, uint32_t n, uint32_t d, double eps,
+                      hipStream_t s);
     // Sorts the R pairs and writes counts[n], offsets[n+1], nbrs[R].
     void finish(uint32_t n, uint64_t R, uint32_t *counts, uint64_t *offsets, uint32_t *nbrs,
                 hipStream_t s);

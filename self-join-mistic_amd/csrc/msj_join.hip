@@ -570,6 +570,15 @@ uint64_t JoinRunner::run_plain(const Index &ix, const double *xr, uint32_t n, ui
     throw Error(MSJ_E_OVERFLOW, "result count changed between identical join passes");
 }
 
+void JoinRunner::launch_plain(const Index &ix, const double *xr, uint32_t n, uint32_t d,
+                              double eps, hipStream_t s)
+{
+    counter.ensure(1);
+    LAUNCH(k_zero<unsigned long long>, 1, 64, counter.p, (size_t)1);
+    if (nitems)
+        dispatch_plain(view(ix, xr, n, d, eps), nullptr, s);
+}
+
 void JoinRunner::finish(uint32_t n, uint64_t R, uint32_t *counts, uint64_t *offsets, uint32_t *nbrs,
                         hipStream_t s)
 {

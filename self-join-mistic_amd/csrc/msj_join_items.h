@@ -329,7 +329,12 @@ __device__ inline void item_hd(const JobView &J, const JoinItem &w, double *qs, 
                     double t3 = qb.y - cc[k + 3];
                     double p3 = t3 * t3;
                     s = s + p3;
-                    am &= wave_vote(!(s > J.eps2));
+                    // partial sums only grow: testing every second chunk
+                    // (and at the end of the row) decides every pair as the
+                    // per-chunk test does; the counting build tests every
+                    // chunk, its dimension count follows the per-chunk exit
+                    if (COUNT || (k & 4) || l0 + k + 4 >= ds)
+                        am &= wave_vote(!(s > J.eps2));
                 }
                 sq[q] = s;
                 if (am != am0 && !((am >> lane) & 1ull))

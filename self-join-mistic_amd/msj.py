@@ -113,7 +113,7 @@ class BatchStats(C.Structure):
         ("ms_tune", C.c_double),
         ("ms_total", C.c_double),
         ("graph_replays", C.c_uint32),
-        ("pad0", C.c_uint32),
+        ("solo_jobs", C.c_uint32),
     ]
 
 
@@ -458,7 +458,7 @@ class Batch:
         finally:
             for k in range(self.njobs):
                 lib().msj_result_free(C.byref(outs[k]))
-        stats = {f[0]: getattr(st, f[0]) for f in BatchStats._fields_ if f[0] != "pad0"}
+        stats = {f[0]: getattr(st, f[0]) for f in BatchStats._fields_}
         stats["class_ds"] = list(st.class_ds)[:st.classes]
         stats["class_epi_log2"] = list(st.class_epi_log2)[:st.classes]
         return res, stats

@@ -94,3 +94,25 @@ def test_batch_invalid_job(msj_mod):
     with pytest.raises(msj_mod.MsjError) as e:
         msj_mod.join_batch([x], [0.0])
     assert e.value.code == msj_mod.E_INVAL
+
+
+def test_batch_solo_jobs_overlap_builds(msj_mod):
+    """Jobs with enough candidate evaluations to fill the GPU are joined alone
+    as soon as their node nets are built (while the next nets are built);
+    the rest share the persistent launch.  Every job equals msj_self_join."""
+    specs = [(400000, 18, 0.14, 951), (250000, 8, 0.05, 952), (3000, 57, 0.5, 953),
+             (5000, 8, 0.09, 954), (800, 18, 0.25, 955)]
+    datas, eps = [], []
+    for n, d, e, seed in specs:
+        x = datagen.mixture(n, d, seed)
+        order, _ = msj_mod.dim_order(x)
+        datas.append(datagen.reorder_dims(x, order))
+        eps.append(e)
+    print("\n[case] batch with solo jobs", flush=True)
+    with msj_mod.Batch() as b:
+        b.load(datas, eps)
+        for rnd in range(2):
+            res, st = b.run()
+            assert st["solo_jobs"] >= 1 and st["solo_jobs"] < len(datas)
+            for x, e, r in zip(datas, eps, res):
+                _same(r, msj_mod.self_join(x, e))
