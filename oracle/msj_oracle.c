@@ -15,6 +15,8 @@
  *     split_nodes                nodes.cu:452-633
  *     update_neighbors           nodes.cu:719-747
  *     update_calcs               nodes.cu:677-695
+ *   build_coss                   tree.cu:3-593, 596-759, 886-996 (buildTree,
+ *                                generateRanges / binarySearch)
  *   join                         kernel.cu:992-1046 + 833-865 (nodeByPoint5,
  *                                cachedDistanceCheck), launcher.cu:3093-3532
  *
@@ -61,6 +63,7 @@ void or_default_params(or_params *p)
     p->join = 1;
     p->sample_every = 1;
     p->nthreads = 0;
+    p->index = 0;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -705,6 +708,268 @@ static void build_net(const double *data, uint32_t n, uint32_t d, double eps, co
 }
 
 /* ------------------------------------------------------------------------ */
+/* Tree / COSS bins: buildTree (tree.cu:3-593, with MINSQRS = true,          */
+/* params.cuh:29) and the 3^L neighbour probes of generateRanges /           */
+/* binarySearch (tree.cu:596-759, 886-996).                                  */
+/*                                                                          */
+/* Readings of the reference kept here (documented deviations aside):       */
+/*  - axis candidates are maxRP of them (tree.cu:278), guarded j < d;        */
+/*  - an axis candidate's sum of squares is not reset between layers unless  */
+/*    its average bin count triggers the N^2 penalty (tree.cu:336-339); its  */
+/*    first-layer start value (uninitialised in the reference) is 0;         */
+/*  - a candidate whose layer would exceed 2^32 bins, or whose key span     */
+/*    wraps the unsigned bin count maxBin + 2 - (minBin - 2) (both unsigned  */
+/*    overflow and out-of-bounds writes in the reference) is not eligible;   */
+/*    a layer with no eligible candidate fails the build (returns -3).       */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    uint32_t skip, nb, nonempty;
+    int eligible;
+} coss_cand;
+
+/* Bin address of every point under candidate t in layer cl (tree.cu:224-253). */
+static inline uint64_t coss_addr(uint32_t key, uint32_t skip, uint32_t nb, uint32_t prank, int cl)
+{
+    uint32_t bn = key - skip;
+    if (cl == 0)
+        return bn;
+    return (uint64_t)(uint32_t)((prank - 1u) * nb) + bn;
+}
+
+static int cmp_tuple_len;
+static int cmp_tuples(const uint32_t *a, const uint32_t *b, int L)
+{
+    for (int l = 0; l < L; l++) {
+        if (a[l] < b[l])
+            return -1;
+        if (a[l] > b[l])
+            return 1;
+    }
+    return 0;
+}
+
+static int build_coss(const double *data, uint32_t n, uint32_t d, double eps, const or_params *p,
+                      or_result *res, ovec *final_nodes)
+{
+    or_rng g;
+    rng_seed(&g, p->seed);
+    const uint32_t nrp = p->rp_per_layer;
+    const uint32_t L = p->maxrp < OR_MAXL ? p->maxrp : OR_MAXL;
+    const uint32_t naxis = L < d ? L : d;
+    const uint32_t ncand = nrp + naxis;
+    res->num_candidates = ncand;
+    double *rp = XMALLOC(double, (size_t)nrp * d);
+    uint32_t *keys = XMALLOC(uint32_t, (size_t)ncand * n);
+    uint32_t *prank = XMALLOC(uint32_t, n); /* 1-based rank of the point's bin in the last layer */
+    uint32_t *bnum = XMALLOC(uint32_t, (size_t)n * L); /* pointBinNumbers */
+    double *sumsq = (double *)calloc(ncand, sizeof(double));
+    double *score = XMALLOC(double, ncand);
+    coss_cand *cc = (coss_cand *)calloc(ncand, sizeof(coss_cand));
+    uint32_t prev_nonempty = 0;
+    for (uint32_t i = 0; i < n; i++)
+        prank[i] = 1;
+    for (uint32_t cl = 0; cl < L; cl++) {
+        rp_create(data, nrp, d, n, p->sample_per, &g, rp);
+        /* RP keys: binningKernel (kernel.cu:895-905) */
+        #pragma omp parallel for schedule(static)
+        for (int64_t ii = 0; ii < (int64_t)n; ii++) {
+            const double *x = &data[(size_t)ii * d];
+            for (uint32_t t = 0; t < nrp; t++) {
+                const double *r = &rp[(size_t)t * d];
+                double dist = 0;
+                for (uint32_t j = 0; j < d; j++) {
+                    double df = x[j] - r[j];
+                    dist = fma(df, df, dist);
+                }
+                keys[(size_t)t * n + ii] = cvt_u32_sat(floor(sqrt(dist) / eps));
+            }
+            /* axis keys: ceil((x + 0.0000001) / eps) + 10 (tree.cu:284) */
+            for (uint32_t a = 0; a < naxis; a++)
+                keys[(size_t)(nrp + a) * n + ii] = cvt_u32_x86(ceil((x[a] + 0.0000001) / eps) + 10);
+        }
+        for (uint32_t t = 0; t < nrp; t++)
+            sumsq[t] = 0;
+        #pragma omp parallel for schedule(dynamic, 1)
+        for (int64_t tt = 0; tt < (int64_t)ncand; tt++) {
+            const uint32_t t = (uint32_t)tt;
+            const uint32_t *k = keys + (size_t)t * n;
+            uint32_t maxB, minB;
+            if (t < nrp) { /* tree.cu:181-196 */
+                maxB = minB = k[0];
+                for (uint32_t i = 0; i < n; i++) {
+                    if (k[i] > maxB)
+                        maxB = k[i];
+                    if (k[i] < minB)
+                        minB = k[i];
+                }
+            } else { /* tree.cu:281-287: int bounds seeded from row 0, unsigned compares */
+                const uint32_t a = t - nrp;
+                int32_t maxBi = cvt_i32_x86(ceil((data[a] + 0.000001) / eps) + 10);
+                int32_t minBi = (int32_t)((uint32_t)maxBi - 1u);
+                for (uint32_t i = 0; i < n; i++) {
+                    if (k[i] > (uint32_t)maxBi)
+                        maxBi = (int32_t)k[i];
+                    if (k[i] < (uint32_t)minBi)
+                        minBi = (int32_t)k[i];
+                }
+                maxB = (uint32_t)maxBi;
+                minB = (uint32_t)minBi;
+            }
+            coss_cand c;
+            c.skip = minB - 2u;
+            c.nb = maxB + 2u - c.skip;
+            const uint64_t count64 = cl == 0 ? c.nb : (uint64_t)prev_nonempty * c.nb;
+            const int wraps = (uint64_t)(maxB - minB) + 4u > 0xFFFFFFFFull;
+            c.eligible = !wraps && count64 > 0 && count64 <= 0xFFFFFFFFull;
+            c.nonempty = 0;
+            if (!c.eligible) {
+                cc[t] = c;
+                score[t] = INFINITY;
+                continue;
+            }
+            uint32_t *bins = (uint32_t *)calloc(count64, sizeof(uint32_t));
+            if (!bins)
+                abort();
+            for (uint32_t i = 0; i < n; i++) {
+                uint64_t addr = coss_addr(k[i], c.skip, c.nb, prank[i], cl);
+                if (bins[addr] == 0)
+                    c.nonempty++;
+                bins[addr]++;
+            }
+            const double avg = (double)((uint64_t)n / c.nonempty);
+            if (t >= nrp && avg >= n / 3.0)
+                sumsq[t] = (double)((uint64_t)n * n);
+            double s = sumsq[t];
+            for (uint64_t j = 0; j < count64; j++) {
+                double e = bins[j] - avg;
+                s += e * e;
+            }
+            sumsq[t] = s;
+            score[t] = s;
+            free(bins);
+            cc[t] = c;
+        }
+        uint32_t best = 0; /* MINSQRS: the first smallest (tree.cu:395-402) */
+        for (uint32_t t = 1; t < ncand; t++)
+            if (score[best] > score[t])
+                best = t;
+        if (!cc[best].eligible) {
+            free(rp); free(keys); free(prank); free(bnum); free(sumsq); free(score); free(cc);
+            return -3;
+        }
+        /* apply: ranks of the non-empty bins (tree.cu:416-430), bin numbers */
+        const coss_cand c = cc[best];
+        const uint64_t count64 = cl == 0 ? c.nb : (uint64_t)prev_nonempty * c.nb;
+        uint32_t *rankof = (uint32_t *)calloc(count64, sizeof(uint32_t));
+        const uint32_t *k = keys + (size_t)best * n;
+        uint64_t *addr = XMALLOC(uint64_t, n);
+        for (uint32_t i = 0; i < n; i++) {
+            addr[i] = coss_addr(k[i], c.skip, c.nb, prank[i], cl);
+            rankof[addr[i]] = 1;
+        }
+        uint32_t cnt = 0;
+        for (uint64_t j = 0; j < count64; j++)
+            if (rankof[j])
+                rankof[j] = ++cnt;
+        for (uint32_t i = 0; i < n; i++) {
+            prank[i] = rankof[addr[i]];
+            bnum[(size_t)i * L + cl] = k[i] - c.skip;
+        }
+        free(addr);
+        free(rankof);
+        prev_nonempty = c.nonempty;
+        res->chosen[cl] = best;
+        res->sse[cl] = score[best];
+        res->nodes_per_layer[cl] = c.nonempty;
+    }
+    res->num_layers = L;
+    /* points by final bin (lexicographic in the bin numbers), stable by row
+     * (tree.cu:472-503) */
+    uint32_t *order = XMALLOC(uint32_t, n), *tk = XMALLOC(uint32_t, n), *tv = XMALLOC(uint32_t, n);
+    uint32_t *kk = XMALLOC(uint32_t, n);
+    for (uint32_t i = 0; i < n; i++) {
+        order[i] = i;
+        kk[i] = prank[i];
+    }
+    radix_sort_u32(kk, order, n, tk, tv);
+    free(tk);
+    free(tv);
+    /* nodes = non-empty final bins */
+    ovec out = {0};
+    uint32_t start = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        if (i == n - 1 || kk[i] != kk[i + 1]) {
+            onode nd;
+            memset(&nd, 0, sizeof(nd));
+            nd.index = out.n;
+            nd.npts = i - start + 1;
+            nd.pts = XMALLOC(uint32_t, nd.npts);
+            memcpy(nd.pts, order + start, sizeof(uint32_t) * nd.npts);
+            nd.nkeys = L;
+            for (uint32_t l = 0; l < L; l++)
+                nd.keys[l] = bnum[(size_t)order[start] * L + l];
+            nd.split = 1;
+            ovec_push(&out, nd);
+            start = i + 1;
+        }
+    }
+    free(kk);
+    free(order);
+    /* neighbours: the 3^L probes of binarySearch (tree.cu:960-975), each a
+     * binary search over the sorted bin-number tuples (bSearch, 886-917) */
+    const uint32_t M = out.n;
+    uint32_t *tup = XMALLOC(uint32_t, (size_t)M * L);
+    for (uint32_t v = 0; v < M; v++)
+        memcpy(tup + (size_t)v * L, out.v[v].keys, sizeof(uint32_t) * L);
+    uint64_t probes = 1;
+    for (uint32_t l = 0; l < L; l++)
+        probes *= 3;
+    #pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t vv = 0; vv < (int64_t)M; vv++) {
+        onode *nd = &out.v[vv];
+        uint32_t want[OR_MAXL];
+        uint64_t sum = 0;
+        for (uint64_t i = 0; i < probes; i++) {
+            uint64_t q = i;
+            for (uint32_t l = 0; l < L; l++) {
+                want[l] = nd->keys[l] + (uint32_t)(q % 3) - 1u;
+                q /= 3;
+            }
+            int64_t lo = 0, hi = (int64_t)M - 1, hit = -1;
+            while (lo <= hi) {
+                int64_t mid = (lo + hi) / 2;
+                int c = cmp_tuples(tup + (size_t)mid * L, want, (int)L);
+                if (c == 0) {
+                    hit = mid;
+                    break;
+                }
+                if (c < 0)
+                    lo = mid + 1;
+                else
+                    hi = mid - 1;
+            }
+            if (hit >= 0) {
+                node_push_nbr(nd, (uint32_t)hit);
+                sum += out.v[hit].npts;
+            }
+        }
+        nd->numCalcs = sum * (uint64_t)nd->npts; /* tree.cu:756 */
+    }
+    (void)cmp_tuple_len;
+    free(tup);
+    res->calcs_per_layer[L - 1] = total_calcs(&out);
+    free(rp);
+    free(keys);
+    free(prank);
+    free(bnum);
+    free(sumsq);
+    free(score);
+    free(cc);
+    *final_nodes = out;
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Results                                                                   */
 /* ------------------------------------------------------------------------ */
 typedef struct {
@@ -800,7 +1065,11 @@ int or_self_join(const double *data, uint32_t n, uint32_t d, double eps, const o
 
     double t0 = omp_get_wtime();
     ovec nodes = {0};
-    build_net(data, n, d, eps, &p, out, &nodes);
+    if (p.index == 2) {
+        if (build_coss(data, n, d, eps, &p, out, &nodes) != 0)
+            return -3;
+    } else
+        build_net(data, n, d, eps, &p, out, &nodes);
     uint32_t nn = nodes.n;
     out->num_nodes = nn;
     out->num_calcs = total_calcs(&nodes);

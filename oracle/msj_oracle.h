@@ -38,6 +38,8 @@ typedef struct or_params {
     uint32_t join;               /* 0: build the index only; 1: join               */
     uint32_t sample_every;       /* join only node-order positions % k == 0 (0,1: all) */
     int32_t  nthreads;           /* OpenMP threads, <= 0: runtime default           */
+    uint32_t index;              /* candidate generator: 0 node net (nodes.cu),
+                                    2 tree/COSS bins (tree.cu)                      */
 } or_params;
 
 typedef struct or_result {

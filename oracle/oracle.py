@@ -28,6 +28,7 @@ class Params(C.Structure):
         ("join", C.c_uint32),
         ("sample_every", C.c_uint32),
         ("nthreads", C.c_int32),
+        ("index", C.c_uint32),
     ]
 
 

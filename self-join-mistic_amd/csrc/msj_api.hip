@@ -501,8 +501,6 @@ uint64_t msj::JoinCore::run(const void *xs, bool f32, uint32_t n, uint32_t d, do
     LayerStats ls;
     if (prm.index == MSJ_INDEX_BRUTE)
         build_single_node(n, ix, ls, s);
-    else if (prm.index == MSJ_INDEX_COSS)
-        throw Error(MSJ_E_UNSUPPORTED, "the tree/COSS candidate generator is not built");
     else
         builder.build(xs, f32, n, d, eps, prm, ix, ls, s);
     join.order(ix, xs, f32, n, d, s);

@@ -249,7 +249,8 @@ __global__ __launch_bounds__(256) void k_col_max(const T *__restrict__ xs, uint3
 // kernel.cu:880-909 (32 RP keys) + nodes.cu:128-135 (axis keys).  One thread
 // per point reads its column once; RP coordinates are wave-uniform (scalar
 // loads of rpT[j][0..31]).
-template <class T>
+// COSS: the tree's axis keys instead (tree.cu:284).
+template <class T, bool COSS = false>
 __global__ __launch_bounds__(256) void k_keys(const T *__restrict__ xs, uint32_t n, uint32_t d,
                                               const double *__restrict__ rpT, uint32_t nrp,
                                               uint32_t naxis, double eps,
@@ -276,7 +277,8 @@ __global__ __launch_bounds__(256) void k_keys(const T *__restrict__ xs, uint32_t
         if ((uint32_t)t < nrp)
             keys[(size_t)t * n + i] = rp_key(acc[t], eps);
     for (uint32_t a = 0; a < naxis; ++a)
-        keys[(size_t)(nrp + a) * n + i] = axis_key((double)xs[(size_t)a * n + i], eps);
+        keys[(size_t)(nrp + a) * n + i] = COSS ? coss_axis_key((double)xs[(size_t)a * n + i], eps)
+                                               : axis_key((double)xs[(size_t)a * n + i], eps);
 }
 
 // Composite (candidate | parent node | key) for candidates [t0, t0+tcount).
@@ -551,6 +553,70 @@ T d2h_scalar(const T *dptr, hipStream_t s)
         MSJ_HIP(hipGetLastError());                                                      \
     } while (0)
 
+// Host state of the reference-point generation of one join.
+struct RpHost {
+    std::vector<double> trp, sig, maxc, rp, rpT;
+    std::vector<uint32_t> order;
+    RpHost(uint32_t nrp, uint32_t d)
+        : trp((size_t)2 * nrp * d), sig(2 * nrp), maxc(d), rp((size_t)nrp * d),
+          rpT((size_t)d * KEY_RP), order(2 * nrp)
+    {
+    }
+};
+
+// utils.cu:66-137 createRPArray for one layer: 2*nrp random candidates from
+// the glibc stream, their sample-distance sigma and the sample's max corner
+// on the device, the host's stable sigma order and the box corners; the
+// layer's reference points end up transposed in B.rpT (d x KEY_RP).
+template <class T>
+void layer_rps(IndexBuilder &B, const T *xs, uint32_t n, uint32_t d, uint32_t S, uint32_t nrp,
+               GlibcRand &rng, RpHost &h, hipStream_t s)
+{
+    const uint32_t ntest = nrp * 2;
+    for (size_t i = 0; i < h.trp.size(); ++i)
+        h.trp[i] = (double)rng.next() / (double)2147483647;
+    MSJ_HIP(hipMemcpyAsync(B.test_rp.p, h.trp.data(), h.trp.size() * sizeof(double),
+                           hipMemcpyHostToDevice, s));
+    if (S > 0)
+        LAUNCH(k_rp_distmat<T>, grid_for((size_t)S * ntest, 256), 256, xs, n, d, S, B.test_rp.p,
+               ntest, B.distmat.p);
+    LAUNCH(k_col_sigma, ntest, SIG_BLOCK, B.distmat.p, S, (size_t)ntest, (size_t)1, B.sigma.p);
+    LAUNCH(k_col_max<T>, d, 256, xs, n, d, S, B.maxc.p);
+    MSJ_HIP(hipMemcpyAsync(h.sig.data(), B.sigma.p, ntest * sizeof(double), hipMemcpyDeviceToHost,
+                           s));
+    MSJ_HIP(hipMemcpyAsync(h.maxc.data(), B.maxc.p, d * sizeof(double), hipMemcpyDeviceToHost, s));
+    MSJ_HIP(hipStreamSynchronize(s));
+    // thrust::sort_by_key (stable, IEEE radix order) ascending, reversed
+    for (uint32_t i = 0; i < ntest; ++i)
+        h.order[i] = i;
+    std::stable_sort(h.order.begin(), h.order.end(), [&](uint32_t a, uint32_t b) {
+        return radix_key_f64(h.sig[a]) < radix_key_f64(h.sig[b]);
+    });
+    std::reverse(h.order.begin(), h.order.end());
+    for (uint32_t i = 0; i < nrp / 2; ++i)
+        for (uint32_t j = 0; j < d; ++j)
+            h.rp[(size_t)i * d + j] = h.trp[(size_t)h.order[i] * d + j];
+    // box points: max corner, then corner patterns (utils.cu:104-136)
+    for (uint32_t j = 0; j < d; ++j)
+        h.trp[j] = h.maxc[j];
+    const uint32_t step = host_cvt_u32(std::pow(2.0, (double)d) / ntest);
+    for (uint32_t i = 1; i < ntest; ++i)
+        for (uint32_t j = 0; j < d; ++j) {
+            uint32_t div = (uint32_t)host_cvt_i32(std::pow(2.0, (double)j));
+            uint32_t bit = ((i * step) / div) % 2u;
+            h.trp[(size_t)i * d + j] = (double)bit * h.trp[j];
+        }
+    for (uint32_t i = 0; i < nrp / 2 + nrp % 2; ++i)
+        for (uint32_t j = 0; j < d; ++j)
+            h.rp[(size_t)(nrp / 2 + i) * d + j] = h.trp[(size_t)i * d + j];
+    std::fill(h.rpT.begin(), h.rpT.end(), 0.0);
+    for (uint32_t t = 0; t < nrp; ++t)
+        for (uint32_t j = 0; j < d; ++j)
+            h.rpT[(size_t)j * KEY_RP + t] = h.rp[(size_t)t * d + j];
+    MSJ_HIP(hipMemcpyAsync(B.rpT.p, h.rpT.data(), h.rpT.size() * sizeof(double),
+                           hipMemcpyHostToDevice, s));
+}
+
 template <class T>
 void build_impl(IndexBuilder &B, const T *xs, uint32_t n, uint32_t d, double eps,
                 const msj_params &p, Index &out, LayerStats &st, hipStream_t s)
@@ -589,56 +655,12 @@ void build_impl(IndexBuilder &B, const T *xs, uint32_t n, uint32_t d, double eps
     LAUNCH(k_root, 1, 64, B.nsplit.p, B.aoff1.p, B.adj1.p);
     uint32_t mold = 1;
 
-    std::vector<double> h_trp((size_t)ntest * d), h_sig(ntest), h_max(d), h_rp((size_t)nrp * d),
-        h_rpT((size_t)d * KEY_RP), h_sse;
-    std::vector<uint32_t> order(ntest);
+    RpHost rh(nrp, d);
+    std::vector<double> h_sse;
 
     for (uint32_t l = 0; l < layers; ++l) {
         // ---- (1) reference points, utils.cu:66-137
-        for (size_t i = 0; i < h_trp.size(); ++i)
-            h_trp[i] = (double)rng.next() / (double)2147483647;
-        MSJ_HIP(hipMemcpyAsync(B.test_rp.p, h_trp.data(), h_trp.size() * sizeof(double),
-                               hipMemcpyHostToDevice, s));
-        if (S > 0)
-            LAUNCH(k_rp_distmat<T>, grid_for((size_t)S * ntest, 256), 256, xs, n, d, S,
-                   B.test_rp.p, ntest, B.distmat.p);
-        LAUNCH(k_col_sigma, ntest, SIG_BLOCK, B.distmat.p, S, (size_t)ntest, (size_t)1,
-               B.sigma.p);
-        LAUNCH(k_col_max<T>, d, 256, xs, n, d, S, B.maxc.p);
-        MSJ_HIP(hipMemcpyAsync(h_sig.data(), B.sigma.p, ntest * sizeof(double),
-                               hipMemcpyDeviceToHost, s));
-        MSJ_HIP(hipMemcpyAsync(h_max.data(), B.maxc.p, d * sizeof(double), hipMemcpyDeviceToHost,
-                               s));
-        MSJ_HIP(hipStreamSynchronize(s));
-        // thrust::sort_by_key (stable, IEEE radix order) ascending, reversed
-        for (uint32_t i = 0; i < ntest; ++i)
-            order[i] = i;
-        std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
-            return radix_key_f64(h_sig[a]) < radix_key_f64(h_sig[b]);
-        });
-        std::reverse(order.begin(), order.end());
-        for (uint32_t i = 0; i < nrp / 2; ++i)
-            for (uint32_t j = 0; j < d; ++j)
-                h_rp[(size_t)i * d + j] = h_trp[(size_t)order[i] * d + j];
-        // box points: max corner, then corner patterns (utils.cu:104-136)
-        for (uint32_t j = 0; j < d; ++j)
-            h_trp[j] = h_max[j];
-        const uint32_t step = host_cvt_u32(std::pow(2.0, (double)d) / ntest);
-        for (uint32_t i = 1; i < ntest; ++i)
-            for (uint32_t j = 0; j < d; ++j) {
-                uint32_t div = (uint32_t)host_cvt_i32(std::pow(2.0, (double)j));
-                uint32_t bit = ((i * step) / div) % 2u;
-                h_trp[(size_t)i * d + j] = (double)bit * h_trp[j];
-            }
-        for (uint32_t i = 0; i < nrp / 2 + nrp % 2; ++i)
-            for (uint32_t j = 0; j < d; ++j)
-                h_rp[(size_t)(nrp / 2 + i) * d + j] = h_trp[(size_t)i * d + j];
-        std::fill(h_rpT.begin(), h_rpT.end(), 0.0);
-        for (uint32_t t = 0; t < nrp; ++t)
-            for (uint32_t j = 0; j < d; ++j)
-                h_rpT[(size_t)j * KEY_RP + t] = h_rp[(size_t)t * d + j];
-        MSJ_HIP(hipMemcpyAsync(B.rpT.p, h_rpT.data(), h_rpT.size() * sizeof(double),
-                               hipMemcpyHostToDevice, s));
+        layer_rps(B, xs, n, d, S, nrp, rng, rh, s);
 
         // ---- (2) keys of every candidate partition
         LAUNCH(k_keys<T>, grid_for(n, 256), 256, xs, n, d, B.rpT.p, nrp, naxis, eps, B.keys.p);
@@ -778,6 +800,516 @@ __global__ void k_single_node(uint32_t n, uint32_t *nstart, uint32_t *aoff, uint
     }
 }
 
+
+// ------------------------------------------------------------------------
+// Tree/COSS bins: buildTree with MINSQRS (tree.cu:3-593) and the 3^L
+// neighbour probes of generateRanges / binarySearch (tree.cu:596-759,
+// 886-996).  Every layer scores all candidates (the layer's reference points
+// and min(L, d) axes) by the sum of squared deviations of every bin of the
+// layer (empty ones included) from N / #non-empty, and keeps the first
+// smallest.  A bin address is (rank of the point's previous bin - 1) * nb +
+// (key - skip); the ranks of the non-empty bins carry to the next layer.
+// Candidates whose layer needs 2^32 bins or more, or whose key span wraps
+// the unsigned bin count, are not eligible (the reference overflows there).
+
+struct CossSlot {
+    uint32_t t, skip, nb, pad;
+};
+
+struct CossWalk {
+    uint32_t j, avg;
+    uint64_t count;
+    double s0;
+};
+
+__global__ void k_minmax_init(uint32_t *mm, uint32_t ncand)
+{
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < ncand) {
+        mm[2 * t] = 0xFFFFFFFFu;
+        mm[2 * t + 1] = 0u;
+    }
+}
+
+// tree.cu:181-196 / 281-287: smallest and largest key of every candidate
+// (grid.y = candidate).
+__global__ __launch_bounds__(256) void k_cand_minmax(const uint32_t *__restrict__ keys, uint32_t n,
+                                                     uint32_t *__restrict__ mm)
+{
+    const uint32_t t = blockIdx.y;
+    const uint32_t *k = keys + (size_t)t * n;
+    uint32_t lo = 0xFFFFFFFFu, hi = 0u;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t v = k[i];
+        lo = v < lo ? v : lo;
+        hi = v > hi ? v : hi;
+    }
+    for (int o = WAVE / 2; o > 0; o >>= 1) {
+        const uint32_t a = __shfl_xor(lo, o, WAVE), b = __shfl_xor(hi, o, WAVE);
+        lo = a < lo ? a : lo;
+        hi = b > hi ? b : hi;
+    }
+    if ((threadIdx.x & (WAVE - 1)) == 0) {
+        atomicMin(&mm[2 * t], lo);
+        atomicMax(&mm[2 * t + 1], hi);
+    }
+}
+
+// tree.cu:224-253: (slot << ab) | bin address of every point under the
+// eligible candidates [j0, j0 + jc).
+__global__ __launch_bounds__(256) void k_coss_addr(const uint32_t *__restrict__ keys,
+                                                   const uint32_t *__restrict__ prank,
+                                                   const CossSlot *__restrict__ slots,
+                                                   uint32_t jc, uint32_t n, int first, int ab,
+                                                   uint64_t *__restrict__ ck)
+{
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)jc * n)
+        return;
+    const uint32_t j = (uint32_t)(idx / n), i = (uint32_t)(idx - (size_t)j * n);
+    const CossSlot c = slots[j];
+    const uint32_t bn = keys[(size_t)c.t * n + i] - c.skip;
+    const uint32_t addr = first ? bn : (prank[i] - 1u) * c.nb + bn;
+    ck[idx] = ((uint64_t)j << ab) | addr;
+}
+
+__device__ inline unsigned long long sat_add(unsigned long long a, unsigned long long b)
+{
+    const unsigned long long r = a + b;
+    return r < a ? ~0ull : r;
+}
+
+// Per slot (one block each): sum over its non-empty bins of
+// (count - N / #non-empty)^2 in integers, saturating at 2^64 - 1.
+__global__ __launch_bounds__(256) void k_coss_stats(const uint32_t *__restrict__ runfirst,
+                                                    const uint32_t *__restrict__ starts, uint32_t n,
+                                                    unsigned long long *__restrict__ acc)
+{
+    __shared__ unsigned long long red[256 / WAVE];
+    const uint32_t j = blockIdx.x;
+    const uint32_t r0 = runfirst[j], r1 = runfirst[j + 1];
+    const uint32_t A = n / (r1 - r0);
+    unsigned long long a = 0;
+    for (uint32_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+        const uint32_t c = starts[r + 1] - starts[r];
+        const uint64_t df = c > A ? c - A : A - c;
+        a = sat_add(a, df * df);
+    }
+    for (int o = WAVE / 2; o > 0; o >>= 1)
+        a = sat_add(a, __shfl_xor(a, o, WAVE));
+    if ((threadIdx.x & (WAVE - 1)) == 0)
+        red[threadIdx.x / WAVE] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int w = 0; w < 256 / WAVE; ++w)
+            t = sat_add(t, red[w]);
+        acc[j] = t;
+    }
+}
+
+// tree.cu:272-275 / 387-394 in the reference's own order: the statistic
+// of one candidate as the sequential FP64 sum over all of its bins (used
+// when the exact integer sum reaches 2^53 and rounding makes the order
+// matter).  One thread per candidate; runs of empty bins are added by
+// repeat_add.
+__global__ void k_coss_walk(const CossWalk *__restrict__ w, uint32_t nw,
+                            const uint32_t *__restrict__ runfirst,
+                            const uint32_t *__restrict__ starts, const uint64_t *__restrict__ ck,
+                            uint64_t amask, double *__restrict__ out)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nw)
+        return;
+    const CossWalk W = w[k];
+    const double a = (double)W.avg;
+    const double c = a * a;  // (0 - avg)^2 of an empty bin
+    double s = W.s0;
+    uint64_t pos = 0;
+    for (uint32_t r = runfirst[W.j]; r < runfirst[W.j + 1]; ++r) {
+        const uint32_t st = starts[r];
+        const uint64_t addr = ck[st] & amask;
+        s = repeat_add(s, c, addr - pos);
+        const double e = (double)(starts[r + 1] - st) - a;
+        const double sq = e * e;
+        s = s + sq;
+        pos = addr + 1;
+    }
+    out[k] = repeat_add(s, c, W.count - pos);
+}
+
+// The chosen candidate: bin address (sort key), row, bin number.
+__global__ __launch_bounds__(256) void k_coss_best(const uint32_t *__restrict__ key,
+                                                   const uint32_t *__restrict__ prank, uint32_t n,
+                                                   int first, uint32_t skip, uint32_t nb,
+                                                   uint64_t *__restrict__ ck,
+                                                   uint32_t *__restrict__ vals,
+                                                   uint32_t *__restrict__ bnum)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const uint32_t bn = key[i] - skip;
+    ck[i] = first ? bn : (prank[i] - 1u) * nb + bn;
+    vals[i] = i;
+    bnum[i] = bn;
+}
+
+// tree.cu:416-430: 1-based rank of every point's non-empty bin.
+__global__ __launch_bounds__(256) void k_coss_rank(const uint32_t *__restrict__ flags,
+                                                   const uint32_t *__restrict__ runidx,
+                                                   const uint32_t *__restrict__ vals, uint32_t n,
+                                                   uint32_t *__restrict__ prank)
+{
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n)
+        prank[vals[q]] = runidx[q] + flags[q];
+}
+
+__global__ __launch_bounds__(256) void k_u32_to_u64(const uint32_t *__restrict__ a, uint32_t n,
+                                                    uint64_t *__restrict__ b)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        b[i] = a[i];
+}
+
+// Bin-number tuple of every final bin (node order = lexicographic order).
+__global__ __launch_bounds__(256) void k_coss_tuples(const uint32_t *__restrict__ perm,
+                                                     const uint32_t *__restrict__ nstart,
+                                                     const uint32_t *__restrict__ bnum, uint32_t n,
+                                                     uint32_t M, uint32_t L,
+                                                     uint32_t *__restrict__ tup)
+{
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= M)
+        return;
+    const uint32_t row = perm[nstart[v]];
+    for (uint32_t l = 0; l < L; ++l)
+        tup[(size_t)v * L + l] = bnum[(size_t)l * n + row];
+}
+
+// tree.cu:886-996: the bins whose bin numbers differ from v's by at most
+// one in every layer (the 3^L probes), found by a depth-first walk over
+// the sorted tuples that narrows one layer at a time, so the hits come out
+// in node order.  WRITE = false counts them (degree, sum of their sizes,
+// numCalcs = |v| * that sum, tree.cu:756); WRITE = true lists them.
+template <bool WRITE>
+__global__ __launch_bounds__(256) void k_coss_adj(const uint32_t *__restrict__ tup, uint32_t M,
+                                                  uint32_t L, const uint32_t *__restrict__ nstart,
+                                                  uint32_t *__restrict__ deg,
+                                                  const uint32_t *__restrict__ aoff,
+                                                  uint32_t *__restrict__ adj,
+                                                  uint64_t *__restrict__ sumsz,
+                                                  uint64_t *__restrict__ ncalcs)
+{
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= M)
+        return;
+    const uint32_t *key = tup + (size_t)v * L;
+    uint32_t lo[MSJ_MAX_LAYERS], hi[MSJ_MAX_LAYERS];
+    uint8_t dig[MSJ_MAX_LAYERS];
+    lo[0] = 0;
+    hi[0] = M;
+    dig[0] = 0;
+    uint32_t cnt = 0, w = WRITE ? aoff[v] : 0;
+    uint64_t sz = 0;
+    int l = 0;
+    while (l >= 0) {
+        if (dig[l] == 3) {
+            if (--l >= 0)
+                ++dig[l];
+            continue;
+        }
+        const uint32_t want = key[l] + (uint32_t)dig[l] - 1u;
+        // [a, b): tuples of [lo, hi) whose layer-l number is `want`
+        uint32_t a = lo[l], e = hi[l];
+        while (a < e) {
+            const uint32_t m = a + (e - a) / 2;
+            if (tup[(size_t)m * L + l] < want)
+                a = m + 1;
+            else
+                e = m;
+        }
+        uint32_t b = a;
+        e = hi[l];
+        while (b < e) {
+            const uint32_t m = b + (e - b) / 2;
+            if (tup[(size_t)m * L + l] <= want)
+                b = m + 1;
+            else
+                e = m;
+        }
+        if (a == b) {
+            ++dig[l];
+            continue;
+        }
+        if ((uint32_t)l + 1 == L) {  // tuples are distinct: one bin
+            if (WRITE)
+                adj[w++] = a;
+            ++cnt;
+            sz += nstart[a + 1] - nstart[a];
+            ++dig[l];
+            continue;
+        }
+        ++l;
+        lo[l] = a;
+        hi[l] = b;
+        dig[l] = 0;
+    }
+    if (!WRITE) {
+        deg[v] = cnt;
+        sumsz[v] = sz;
+        ncalcs[v] = sz * (uint64_t)(nstart[v + 1] - nstart[v]);
+    }
+}
+
+template <class T>
+void build_coss_impl(IndexBuilder &B, const T *xs, uint32_t n, uint32_t d, double eps,
+                     const msj_params &p, Index &out, LayerStats &st, hipStream_t s)
+{
+    const uint32_t nrp = p.rp_per_layer;
+    const uint32_t L = std::min<uint32_t>(p.maxrp, MSJ_MAX_LAYERS);
+    const uint32_t naxis = std::min(L, d);  // maxRP axes (tree.cu:278), j < d
+    const uint32_t ncand = nrp + naxis;
+    const uint32_t ntest = nrp * 2;
+    const uint32_t S = host_cvt_u32((double)(unsigned long long)n * p.sample_per);
+    st = LayerStats{};
+    st.ncand = ncand;
+    st.layers = L;
+
+    GlibcRand rng;
+    rng.seed(p.seed);
+    B.keys.ensure((size_t)ncand * n);
+    B.test_rp.ensure((size_t)ntest * d);
+    B.distmat.ensure((size_t)std::max<uint32_t>(S, 1) * ntest);
+    B.sigma.ensure(ntest);
+    B.maxc.ensure(d);
+    B.rpT.ensure((size_t)d * KEY_RP);
+    B.cmm.ensure(2 * (size_t)ncand);
+    B.cslots.ensure(4 * (size_t)ncand);
+    B.cacc.ensure(ncand);
+    B.cscore.ensure(ncand);
+    B.cwalk.ensure((sizeof(CossWalk) / 4) * (size_t)ncand);
+    B.prank.ensure(n);
+    B.bnum.ensure((size_t)L * n);
+    B.vals.ensure(n);
+    B.u64scalar.ensure(1);
+    B.runfirst.ensure((size_t)ncand + 1);
+
+    // row 0 seeds the axis key bounds: int maxBin = ceil((x + 0.000001) / eps) + 10,
+    // minBin = maxBin - 1 (tree.cu:281-282), compared as unsigned
+    std::vector<T> x0(naxis);
+    for (uint32_t a = 0; a < naxis; ++a)
+        MSJ_HIP(hipMemcpyAsync(&x0[a], xs + (size_t)a * n, sizeof(T), hipMemcpyDeviceToHost, s));
+    MSJ_HIP(hipStreamSynchronize(s));
+    std::vector<uint32_t> seed_max(naxis);
+    for (uint32_t a = 0; a < naxis; ++a)
+        seed_max[a] = (uint32_t)host_cvt_i32(std::ceil(((double)x0[a] + 0.000001) / eps) + 10.0);
+
+    LAUNCH(k_fill<uint32_t>, grid_for(n, 256), 256, B.prank.p, (size_t)n, 1u);
+    RpHost rh(nrp, d);
+    std::vector<double> sumsq(ncand, 0.0), score(ncand);
+    std::vector<uint32_t> mm(2 * (size_t)ncand), ne_of(ncand);
+    std::vector<uint64_t> count_of(ncand);
+    std::vector<CossSlot> slots;
+    std::vector<CossWalk> walks;
+    std::vector<uint32_t> h_rf;
+    std::vector<unsigned long long> h_acc;
+    std::vector<double> h_walk;
+    uint32_t prev_ne = 0;
+
+    for (uint32_t cl = 0; cl < L; ++cl) {
+        // ---- candidates and their keys (createRPArray, binningKernel, tree.cu:284)
+        layer_rps(B, xs, n, d, S, nrp, rng, rh, s);
+        LAUNCH((k_keys<T, true>), grid_for(n, 256), 256, xs, n, d, B.rpT.p, nrp, naxis, eps,
+               B.keys.p);
+        if (B.probe && B.probe->keys && B.probe->layer == cl) {
+            MSJ_HIP(hipMemcpyAsync(B.probe->keys, B.keys.p, sizeof(uint32_t) * ncand * n,
+                                   hipMemcpyDeviceToHost, s));
+            MSJ_HIP(hipStreamSynchronize(s));
+        }
+        LAUNCH(k_minmax_init, 1, 64 * ((ncand + 63) / 64), B.cmm.p, ncand);
+        hipLaunchKernelGGL(k_cand_minmax, dim3(std::min<unsigned>(grid_for(n, 256), 256), ncand),
+                           dim3(256), 0, s, B.keys.p, n, B.cmm.p);
+        MSJ_HIP(hipGetLastError());
+        MSJ_HIP(hipMemcpyAsync(mm.data(), B.cmm.p, sizeof(uint32_t) * 2 * ncand,
+                               hipMemcpyDeviceToHost, s));
+        MSJ_HIP(hipStreamSynchronize(s));
+
+        // ---- eligible candidates: skip = minBin - 2, nb = maxBin + 2 - skip
+        for (uint32_t t = 0; t < nrp; ++t)
+            sumsq[t] = 0.0;
+        slots.clear();
+        uint64_t maxcount = 1;
+        for (uint32_t t = 0; t < ncand; ++t) {
+            uint32_t lo = mm[2 * t], hi = mm[2 * t + 1];
+            if (t >= nrp) {
+                const uint32_t sm = seed_max[t - nrp];
+                hi = std::max(hi, sm);
+                lo = std::min(lo, sm - 1u);
+            }
+            const uint32_t skip = lo - 2u, nb = hi + 2u - skip;
+            const uint64_t count = cl == 0 ? nb : (uint64_t)prev_ne * nb;
+            score[t] = INFINITY;
+            count_of[t] = count;
+            if ((uint64_t)(hi - lo) + 4u > 0xFFFFFFFFull || count == 0 || count > 0xFFFFFFFFull)
+                continue;
+            slots.push_back(CossSlot{t, skip, nb, 0u});
+            maxcount = std::max(maxcount, count);
+        }
+        const uint32_t nel = (uint32_t)slots.size();
+        if (nel == 0)
+            throw Error(MSJ_E_OVERFLOW, "tree/COSS layer " + std::to_string(cl) +
+                                            ": no candidate fits in 2^32 bins");
+        MSJ_HIP(hipMemcpyAsync(B.cslots.p, slots.data(), sizeof(CossSlot) * nel,
+                               hipMemcpyHostToDevice, s));
+        const int ab = bits_for(maxcount - 1);
+        const uint64_t amask = ab >= 64 ? ~0ull : ((1ull << ab) - 1);
+        // groups of slots whose (slot, address) keys fit the u32 run counters
+        const uint32_t G = (uint32_t)std::max<uint64_t>(
+            1, std::min<uint64_t>(nel, (uint64_t)0x7FFFFFFF / n));
+        for (uint32_t j0 = 0; j0 < nel; j0 += G) {
+            const uint32_t jc = std::min(G, nel - j0);
+            const size_t m = (size_t)jc * n;
+            B.ck.ensure(m);
+            B.flags.ensure(m);
+            B.runidx.ensure(m + 1);
+            B.starts.ensure(m + 1);
+            LAUNCH(k_coss_addr, grid_for(m, 256), 256, B.keys.p, B.prank.p,
+                   (const CossSlot *)B.cslots.p + j0, jc, n, cl == 0 ? 1 : 0, ab, B.ck.p);
+            B.sorter.sort(B.ck.p, nullptr, m, ab + bits_for(jc - 1), s);
+            LAUNCH(k_run_flags, grid_for(m, 256), 256, B.ck.p, m, B.flags.p);
+            B.scan.excl_u32_to_u32(B.flags.p, B.runidx.p, m, s);
+            LAUNCH(k_run_starts, grid_for(m, 256), 256, B.flags.p, B.runidx.p, m, B.starts.p);
+            LAUNCH(k_runfirst, 1, 64 * ((jc + 1 + 63) / 64), B.runidx.p, n, jc, B.runfirst.p);
+            LAUNCH(k_coss_stats, jc, 256, B.runfirst.p, B.starts.p, n, B.cacc.p);
+            h_rf.resize(jc + 1);
+            h_acc.resize(jc);
+            MSJ_HIP(hipMemcpyAsync(h_rf.data(), B.runfirst.p, sizeof(uint32_t) * (jc + 1),
+                                   hipMemcpyDeviceToHost, s));
+            MSJ_HIP(hipMemcpyAsync(h_acc.data(), B.cacc.p, sizeof(unsigned long long) * jc,
+                                   hipMemcpyDeviceToHost, s));
+            MSJ_HIP(hipStreamSynchronize(s));
+            walks.clear();
+            for (uint32_t j = 0; j < jc; ++j) {
+                const uint32_t t = slots[j0 + j].t;
+                const uint32_t ne = h_rf[j + 1] - h_rf[j];
+                const uint32_t A = n / ne;  // numPoints / layerBinNonEmpty
+                double s0 = sumsq[t];
+                if (t >= nrp && (double)A >= (double)n / 3.0)
+                    s0 = (double)((uint64_t)n * n);
+                ne_of[t] = ne;
+                // exact in integers below 2^53: then every partial sum of the
+                // reference's FP64 loop is exact and its order does not matter
+                const unsigned __int128 total =
+                    (unsigned __int128)h_acc[j] +
+                    (unsigned __int128)(count_of[t] - ne) * ((uint64_t)A * A) +
+                    (unsigned __int128)(s0 < 9007199254740992.0 ? (uint64_t)s0 : 0);
+                const bool exact = h_acc[j] != ~0ull && s0 < 9007199254740992.0 &&
+                                   s0 == std::floor(s0) && total < ((unsigned __int128)1 << 53);
+                if (exact)
+                    score[t] = (double)(uint64_t)total;
+                else
+                    walks.push_back(CossWalk{j, A, count_of[t], s0});
+            }
+            if (!walks.empty()) {
+                const uint32_t nw = (uint32_t)walks.size();
+                MSJ_HIP(hipMemcpyAsync(B.cwalk.p, walks.data(), sizeof(CossWalk) * nw,
+                                       hipMemcpyHostToDevice, s));
+                LAUNCH(k_coss_walk, grid_for(nw, 64), 64, (const CossWalk *)B.cwalk.p, nw,
+                       B.runfirst.p, B.starts.p, B.ck.p, amask, B.cscore.p);
+                h_walk.resize(nw);
+                MSJ_HIP(hipMemcpyAsync(h_walk.data(), B.cscore.p, sizeof(double) * nw,
+                                       hipMemcpyDeviceToHost, s));
+                MSJ_HIP(hipStreamSynchronize(s));
+                for (uint32_t k = 0; k < nw; ++k)
+                    score[slots[j0 + walks[k].j].t] = h_walk[k];
+            }
+            for (uint32_t j = 0; j < jc; ++j)
+                sumsq[slots[j0 + j].t] = score[slots[j0 + j].t];
+        }
+        uint32_t best = 0;  // MINSQRS: the first smallest (tree.cu:400-404)
+        for (uint32_t t = 1; t < ncand; ++t)
+            if (score[best] > score[t])
+                best = t;
+        const CossSlot *bs = nullptr;
+        for (const CossSlot &c : slots)
+            if (c.t == best)
+                bs = &c;
+        if (!bs)
+            throw Error(MSJ_E_OVERFLOW, "tree/COSS layer " + std::to_string(cl) +
+                                            ": no candidate fits in 2^32 bins");
+
+        // ---- apply the winner: ranks of its non-empty bins, bin numbers
+        B.ck.ensure(n);
+        B.flags.ensure(n);
+        B.runidx.ensure((size_t)n + 1);
+        LAUNCH(k_coss_best, grid_for(n, 256), 256, B.keys.p + (size_t)best * n, B.prank.p, n,
+               cl == 0 ? 1 : 0, bs->skip, bs->nb, B.ck.p, B.vals.p, B.bnum.p + (size_t)cl * n);
+        B.sorter.sort(B.ck.p, B.vals.p, n, bits_for(count_of[best] - 1), s);
+        LAUNCH(k_run_flags, grid_for(n, 256), 256, B.ck.p, (size_t)n, B.flags.p);
+        B.scan.excl_u32_to_u32(B.flags.p, B.runidx.p, n, s);
+        LAUNCH(k_coss_rank, grid_for(n, 256), 256, B.flags.p, B.runidx.p, B.vals.p, n,
+               B.prank.p);
+        const uint32_t ne = d2h_scalar(B.runidx.p + n, s);
+        if (ne != ne_of[best])
+            throw Error(MSJ_E_HIP, "tree/COSS: inconsistent bin counts");
+        prev_ne = ne;
+        st.chosen[cl] = best;
+        st.sse[cl] = score[best];
+        st.nodes[cl] = ne;
+        if (p.verbose)
+            printf("Layer %u sumqrs: %f BinCount: %llu nonEmpty: %u (candidate %u)\n", cl,
+                   score[best], (unsigned long long)count_of[best], ne, best);
+    }
+
+    // ---- points by final bin, stable by row (tree.cu:472-503)
+    const uint32_t M = prev_ne;
+    B.ck.ensure(n);
+    LAUNCH(k_u32_to_u64, grid_for(n, 256), 256, B.prank.p, n, B.ck.p);
+    LAUNCH(k_iota, grid_for(n, 256), 256, B.vals.p, n);
+    B.sorter.sort(B.ck.p, B.vals.p, n, bits_for(M), s);
+    swap_buf(out.perm, B.vals);
+    LAUNCH(k_run_flags, grid_for(n, 256), 256, B.ck.p, (size_t)n, B.flags.p);
+    B.scan.excl_u32_to_u32(B.flags.p, B.runidx.p, n, s);
+    out.nstart.ensure((size_t)M + 1);
+    LAUNCH(k_run_starts, grid_for(n, 256), 256, B.flags.p, B.runidx.p, (size_t)n, out.nstart.p);
+    B.tup.ensure((size_t)M * L);
+    LAUNCH(k_coss_tuples, grid_for(M, 256), 256, out.perm.p, out.nstart.p, B.bnum.p, n, M, L,
+           B.tup.p);
+    if (B.probe && B.probe->node_keys) {
+        B.probe->tuples.resize((size_t)M * L);
+        MSJ_HIP(hipMemcpyAsync(B.probe->tuples.data(), B.tup.p, sizeof(uint32_t) * M * L,
+                               hipMemcpyDeviceToHost, s));
+        MSJ_HIP(hipStreamSynchronize(s));
+    }
+
+    // ---- neighbouring bins (tree.cu:886-996)
+    B.deg.ensure(M);
+    out.sumsz.ensure(M);
+    out.ncalcs.ensure(M);
+    out.aoff.ensure((size_t)M + 1);
+    LAUNCH(k_coss_adj<false>, grid_for(M, 256), 256, B.tup.p, M, L, out.nstart.p, B.deg.p,
+           nullptr, nullptr, out.sumsz.p, out.ncalcs.p);
+    B.tmp64.ensure((size_t)M + 1);
+    B.scan.excl_u32_to_u64(B.deg.p, B.tmp64.p, M, s);
+    const uint64_t edges = d2h_scalar(B.tmp64.p + M, s);
+    if (edges >= 0xFFFFFFFFull)
+        throw Error(MSJ_E_OVERFLOW, "node adjacency exceeds 2^32 entries");
+    B.scan.excl_u32_to_u32(B.deg.p, out.aoff.p, M, s);
+    out.adj.ensure(std::max<uint64_t>(edges, 1));
+    LAUNCH(k_coss_adj<true>, grid_for(M, 256), 256, B.tup.p, M, L, out.nstart.p, nullptr,
+           out.aoff.p, out.adj.p, nullptr, nullptr);
+    reduce_sum_u64(out.ncalcs.p, M, B.u64scalar.p, s);
+    const uint64_t calcs = d2h_scalar(B.u64scalar.p, s);
+    out.M = M;
+    out.edges = (uint32_t)edges;
+    out.num_calcs = calcs;
+    st.calcs[L - 1] = calcs;
+}
+
 }  // namespace
 
 void column_sigma(const double *A, uint32_t rows, uint32_t cols, size_t rs, size_t cs,
@@ -807,7 +1339,12 @@ void build_single_node(uint32_t n, Index &out, LayerStats &st, hipStream_t s)
 void IndexBuilder::build(const void *xs, bool f32, uint32_t n, uint32_t d, double eps,
                          const msj_params &p, Index &out, LayerStats &st, hipStream_t s)
 {
-    if (f32)
+    if (p.index == MSJ_INDEX_COSS) {
+        if (f32)
+            build_coss_impl<float>(*this, (const float *)xs, n, d, eps, p, out, st, s);
+        else
+            build_coss_impl<double>(*this, (const double *)xs, n, d, eps, p, out, st, s);
+    } else if (f32)
         build_impl<float>(*this, (const float *)xs, n, d, eps, p, out, st, s);
     else
         build_impl<double>(*this, (const double *)xs, n, d, eps, p, out, st, s);

@@ -214,13 +214,69 @@ __device__ inline uint32_t axis_key(double x, double eps)
     return (uint32_t)(uint64_t)(int64_t)q;
 }
 
+// tree.cu:284 key of an axis partition of the tree/COSS index:
+// ceil((x + 1e-7) / eps) + 10, converted as x86-64 host code does.
+__device__ inline uint32_t coss_axis_key(double x, double eps)
+{
+    const double q = ceil((x + 0.0000001) / eps) + 10.0;
+    if (!(q > -9223372036854775808.0 && q < 9223372036854775808.0))
+        return 0u;
+    return (uint32_t)(uint64_t)(int64_t)q;
+}
+
 // Test probe of an index build (host buffers, filled when set).
 struct IndexProbe {
     uint32_t layer = ~0u;        // layer whose candidate keys are copied out
     uint32_t *keys = nullptr;    // [ncand * n] keys of every candidate, row order
     bool node_keys = false;      // keep each layer's (parent, key) of the children
     std::vector<uint32_t> cparent[MSJ_MAX_LAYERS], ckey[MSJ_MAX_LAYERS];
+    std::vector<uint32_t> tuples;  // tree/COSS: the nodes' bin numbers, [M * layers]
 };
+
+// s after g steps of s = RN(s + c), for finite s >= 0 and c >= 0 normal,
+// in a few steps per binade of s instead of g.  Inside one binade [2^(e-1),
+// 2^e) with ulp u, c = q*u + r: with r != u/2 every step adds the same
+// rounded multiple of u; with r == u/2 (a tie) the step rounds to an even
+// significand and, once the significand is even, adds the same multiple of
+// u again every step.  So after a step that lands in the binade, the next
+// step that stays in it gives the steady increment, and the run of steps
+// that keeps s below 2^e - u is added at once, exactly, in units of u.
+__host__ __device__ inline double repeat_add(double s, double c, uint64_t g)
+{
+    if (!(c > 0.0))
+        return s;
+    while (g > 0) {
+        s = s + c;
+        if (--g == 0)
+            break;
+        int es, et;
+        const double t = s + c;  // lands even if it was a tie
+        --g;
+        (void)frexp(s, &es);
+        (void)frexp(t, &et);
+        s = t;
+        if (es != et || g == 0)
+            continue;
+        const double t2 = s + c;  // from an even significand: the steady step
+        --g;
+        (void)frexp(t2, &et);
+        const double delta = t2 - s;  // exact when both lie in the binade
+        s = t2;
+        if (es != et)
+            continue;
+        if (delta == 0.0)  // c below half an ulp: s no longer moves
+            break;
+        const double u = ldexp(1.0, es - 53);
+        const uint64_t room = (uint64_t)((ldexp(1.0, es) - u - s) / u);  // exact
+        const uint64_t D = (uint64_t)(delta / u);
+        uint64_t k = room / D;
+        if (k > g)
+            k = g;
+        s = s + ldexp((double)(k * D), es - 53);
+        g -= k;
+    }
+    return s;
+}
 
 struct IndexBuilder {
     IndexProbe *probe = nullptr;
@@ -236,9 +292,14 @@ struct IndexBuilder {
     DevBuf<uint64_t> sumsz2, ncalcs2, tmp64;
     DevBuf<uint32_t> u32scalar;
     DevBuf<unsigned long long> u64scalar;
+    // tree/COSS index
+    DevBuf<uint32_t> cmm, cslots, prank, bnum, tup, cwalk;
+    DevBuf<unsigned long long> cacc;
+    DevBuf<double> cscore;
     Scanner scan;
     RadixSorter sorter;
-    // Builds the node net of `xs` (SoA, row order, storage T) into `out`.
+    // Builds the candidate index of `xs` (SoA, row order, storage T) into
+    // `out`: the node net, or the tree/COSS bins when p.index asks for them.
     void build(const void *xs, bool f32, uint32_t n, uint32_t d, double eps, const msj_params &p,
                Index &out, LayerStats &st, hipStream_t s);
 };

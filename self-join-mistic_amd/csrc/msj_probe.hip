@@ -119,7 +119,25 @@ void probe_index(void *a_)
     if (a.probe->node_order)
         MSJ_HIP(hipMemcpy(a.probe->node_order, ix.perm.p, sizeof(uint32_t) * a.n,
                           hipMemcpyDeviceToHost));
-    if (a.probe->node_keys) {
+    for (uint32_t l = 0; l < MSJ_MAX_LAYERS; ++l) {
+        a.probe->chosen[l] = ls.chosen[l];
+        a.probe->nodes_per_layer[l] = ls.nodes[l];
+        a.probe->sse[l] = ls.sse[l];
+    }
+    a.probe->num_edges = ix.edges;
+    a.probe->num_calcs = ix.num_calcs;
+    if (a.probe->adj_off)
+        MSJ_HIP(hipMemcpy(a.probe->adj_off, ix.aoff.p, sizeof(uint32_t) * ((size_t)M + 1),
+                          hipMemcpyDeviceToHost));
+    if (a.probe->adj && a.probe->adj_cap)
+        MSJ_HIP(hipMemcpy(a.probe->adj, ix.adj.p,
+                          sizeof(uint32_t) * std::min<uint64_t>(a.probe->adj_cap, ix.edges),
+                          hipMemcpyDeviceToHost));
+    if (a.probe->node_keys && prm.index == MSJ_INDEX_COSS) {
+        if (pr.tuples.size() != (size_t)M * L)
+            throw Error(MSJ_E_HIP, "probe: tree/COSS bin numbers missing");
+        std::copy(pr.tuples.begin(), pr.tuples.end(), a.probe->node_keys);
+    } else if (a.probe->node_keys) {
         // walk each final node up its parents: layer l's children record the
         // index of their parent among layer l-1's nodes
         for (uint32_t c = 0; c < M; ++c) {
@@ -186,6 +204,11 @@ int msj_probe_key_math(const double *D, uint64_t m, double eps, double *q, uint3
 {
     KeyMathArgs a{D, m, eps, q, key, device};
     return msj::guarded_call(probe_key_math, &a);
+}
+
+double msj_probe_repeat_add(double s, double c, uint64_t g)
+{
+    return msj::repeat_add(s, c, g);
 }
 
 }  // extern "C"

@@ -26,7 +26,8 @@ EXPORTS = (
     "msj_self_join", "msj_result_free", "msj_ctx_create", "msj_ctx_destroy",
     "msj_ctx_load", "msj_ctx_join", "msj_dim_order", "msj_batch_create", "msj_batch_destroy",
     "msj_batch_load", "msj_batch_run", "msj_join_batch",
-    "msj_probe_index", "msj_probe_key_math", "msj_ctx_load_unordered", "msj_ctx_set_params",
+    "msj_probe_index", "msj_probe_key_math", "msj_probe_repeat_add", "msj_ctx_load_unordered",
+    "msj_ctx_set_params",
     "msj_self_join_noalloc",
 )
 
@@ -142,6 +143,14 @@ class IndexProbe(C.Structure):
         ("node_keys", C.POINTER(C.c_uint32)),
         ("node_start", C.POINTER(C.c_uint32)),
         ("node_order", C.POINTER(C.c_uint32)),
+        ("adj_off", C.POINTER(C.c_uint32)),
+        ("adj", C.POINTER(C.c_uint32)),
+        ("adj_cap", C.c_uint64),
+        ("num_edges", C.c_uint64),
+        ("num_calcs", C.c_uint64),
+        ("chosen", C.c_uint32 * MAX_LAYERS),
+        ("nodes_per_layer", C.c_uint32 * MAX_LAYERS),
+        ("sse", C.c_double * MAX_LAYERS),
     ]
 
 
@@ -191,6 +200,8 @@ def lib():
                                      C.POINTER(Result)]
         L.msj_probe_index.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_double,
                                       C.POINTER(Params), C.POINTER(IndexProbe)]
+        L.msj_probe_repeat_add.argtypes = [C.c_double, C.c_double, C.c_uint64]
+        L.msj_probe_repeat_add.restype = C.c_double
         L.msj_probe_key_math.argtypes = [C.POINTER(C.c_double), C.c_uint64, C.c_double,
                                          C.POINTER(C.c_double), C.POINTER(C.c_uint32), C.c_int]
         _lib = L
@@ -301,31 +312,56 @@ def self_join(data, eps, **kw):
         lib().msj_result_free(C.byref(r))
 
 
-def probe_index(data, eps, layer=0, **kw):
-    """msj_probe_index: the GPU node-net build's candidate keys at `layer`
+def probe_index(data, eps, layer=0, adjacency=False, **kw):
+    """msj_probe_index: the GPU index build's candidate keys at `layer`
     ([ncand, n], row order), the final nodes' key tuples ([M, layers]), node
-    boundaries and node order.  Runs no join."""
+    boundaries, node order, numCalcs and, with adjacency=True (a second
+    build sized by the first), the neighbour-node CSR.  Runs no join."""
     p = default_params(**kw)
     data = _as_input(data, p.storage)
     n, d = data.shape
-    ncand = p.rp_per_layer + min(p.numidim, d)
+    axes = min(p.maxrp, d) if p.index == INDEX_COSS else min(p.numidim, d)
+    ncand = p.rp_per_layer + axes
     keys = np.zeros(ncand * n, dtype=np.uint32)
     nkeys = np.zeros(n * p.maxrp, dtype=np.uint32)
     nstart = np.zeros(n + 1, dtype=np.uint32)
     order = np.zeros(n, dtype=np.uint32)
-    pr = IndexProbe()
-    pr.layer = layer
+    aoff = np.zeros(n + 1, dtype=np.uint32)
     up = C.POINTER(C.c_uint32)
-    pr.keys = keys.ctypes.data_as(up)
-    pr.node_keys = nkeys.ctypes.data_as(up)
-    pr.node_start = nstart.ctypes.data_as(up)
-    pr.node_order = order.ctypes.data_as(up)
-    _check(lib().msj_probe_index(data.ctypes.data_as(C.c_void_p), n, d, float(eps), C.byref(p),
-                                 C.byref(pr)))
+
+    def build(adj):
+        pr = IndexProbe()
+        pr.layer = layer
+        pr.keys = keys.ctypes.data_as(up)
+        pr.node_keys = nkeys.ctypes.data_as(up)
+        pr.node_start = nstart.ctypes.data_as(up)
+        pr.node_order = order.ctypes.data_as(up)
+        pr.adj_off = aoff.ctypes.data_as(up)
+        if adj is not None:
+            pr.adj = adj.ctypes.data_as(up)
+            pr.adj_cap = adj.shape[0]
+        _check(lib().msj_probe_index(data.ctypes.data_as(C.c_void_p), n, d, float(eps),
+                                     C.byref(p), C.byref(pr)))
+        return pr
+
+    pr = build(None)
+    adj = None
+    if adjacency:
+        adj = np.zeros(max(pr.num_edges, 1), dtype=np.uint32)
+        pr = build(adj)
+        adj = adj[:pr.num_edges]
     M, L = pr.num_nodes, pr.num_layers
     return {"keys": keys[:pr.ncand * n].reshape(pr.ncand, n),
             "node_keys": nkeys[:M * L].reshape(M, L), "node_start": nstart[:M + 1],
-            "node_order": order, "num_nodes": M, "num_layers": L}
+            "node_order": order, "num_nodes": M, "num_layers": L,
+            "adj_off": aoff[:M + 1], "adj": adj, "num_edges": pr.num_edges,
+            "num_calcs": pr.num_calcs, "chosen": list(pr.chosen)[:L],
+            "nodes_per_layer": list(pr.nodes_per_layer)[:L], "sse": list(pr.sse)[:L]}
+
+
+def probe_repeat_add(s, c, g):
+    """msj_probe_repeat_add: s after g steps of s = RN(s + c) (host code)."""
+    return lib().msj_probe_repeat_add(float(s), float(c), int(g))
 
 
 def probe_key_math(D, eps, device=0):

@@ -4,7 +4,9 @@ binningKernel, nodes.cu:128-135 axis partitions) and node key tuples
 
   * every candidate key row of every layer equals the oracle's keys from the
     oracle's reference points, bit for bit, on all points;
-  * the final nodes' key tuples equal the oracle's (stopped layers included);
+  * the final nodes' key tuples equal the oracle's (stopped layers included),
+    and so do every node's neighbour set, numCalcs, the chosen candidates
+    and the bits of their statistic;
   * the device FP64 sqrt and division behind each key equal IEEE (numpy's,
     correctly rounded) on 1e8 inputs, many of them within a few ulps of a
     bin edge.
@@ -58,12 +60,21 @@ def test_node_key_tuples(name, make, eps, msj_mod, oracle_mod):
         order, _ = msj_mod.dim_order(x)
         x = datagen.reorder_dims(x, order)
     _announce(name, *x.shape)
-    got = msj_mod.probe_index(x, eps)
+    got = msj_mod.probe_index(x, eps, adjacency=True)
     ref = oracle_mod.self_join(x, eps, join=0)
     assert got["num_nodes"] == ref.num_nodes
     assert np.array_equal(got["node_start"], ref.node_start)
     assert np.array_equal(got["node_order"], ref.node_order)
     assert np.array_equal(got["node_keys"], ref.node_keys)
+    # neighbour-node sets (splitNodes + updateNeighbors, nodes.cu:452-633, 719-747)
+    # and numCalcs
+    assert got["num_calcs"] == ref.num_calcs
+    assert got["num_edges"] == len(ref.adj)
+    for v in range(ref.num_nodes):
+        mine = np.sort(got["adj"][got["adj_off"][v]:got["adj_off"][v + 1]])
+        assert np.array_equal(mine, np.sort(ref.adj[ref.adj_off[v]:ref.adj_off[v + 1]])), v
+    assert got["chosen"] == list(ref.chosen)
+    assert [v.hex() for v in got["sse"]] == [v.hex() for v in ref.sse]
 
 
 def _host_keys(D, eps):

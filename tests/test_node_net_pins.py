@@ -263,3 +263,155 @@ def test_stop_rule_threshold_parameter(offset, oracle_mod):
         assert np.all(second != U32)
     else:
         assert np.all(second == U32) and rows.sum() == 1
+
+
+# ------------------------------------------------------------ tree / COSS
+
+
+def _x86_u32(v):
+    """(unsigned) of a double on x86-64: cvttsd2si to 64 bits, low half;
+    out of range -> 0x8000000000000000 (low half 0)."""
+    v = np.asarray(v, dtype=np.float64)
+    ok = (v > -9.223372036854775808e18) & (v < 9.223372036854775808e18)
+    out = np.zeros(v.shape, dtype=np.int64)
+    out[ok] = np.trunc(v[ok]).astype(np.int64)
+    return (out & 0xFFFFFFFF).astype(np.int64)
+
+
+def _x86_i32(v):
+    """(int) of a double on x86-64: cvttsd2si to 32 bits; out of range ->
+    INT_MIN."""
+    if not (-2147483649.0 < v < 2147483648.0):
+        return -(1 << 31)
+    return int(np.trunc(v))
+
+
+def numpy_coss(x, eps, oracle_mod, layers=6, plant=None):
+    """buildTree (tree.cu:3-593, MINSQRS) and the 3^L probes (tree.cu:886-996)
+    restated in numpy from the oracle's reference points and RP keys."""
+    n, d = x.shape
+    rps = oracle_mod.rp_layers(x, layers=layers)
+    nrp = rps.shape[1]
+    naxis = min(layers, d)
+    ncand = nrp + naxis
+    sumsq = np.zeros(ncand)
+    prank = np.ones(n, dtype=np.int64)
+    prev_nonempty = 0
+    bnum = np.zeros((n, layers), dtype=np.int64)
+    out = {"chosen": [], "sse": [], "nodes": []}
+    M32 = 0xFFFFFFFF
+    for cl in range(layers):
+        K = oracle_mod.keys(x, eps, rps[cl], 0).astype(np.int64)  # RP keys only
+        ax = np.stack([_x86_u32(np.ceil((x[:, a] + 0.0000001) / eps) + 10)
+                       for a in range(naxis)]) if naxis else np.zeros((0, n), np.int64)
+        K = np.concatenate([K, ax])
+        sumsq[:nrp] = 0.0
+        score = np.full(ncand, np.inf)
+        cand = []
+        for t in range(ncand):
+            k = K[t]
+            if t < nrp:
+                maxb, minb = int(k.max()), int(k.min())
+            else:
+                a = t - nrp
+                maxbi = _x86_i32(np.ceil((x[0, a] + 0.000001) / eps) + 10) & M32
+                minbi = (maxbi - 1) & M32
+                maxb, minb = max(maxbi, int(k.max())), min(minbi, int(k.min()))
+            skip = (minb - 2) & M32
+            nb = (maxb + 2 - skip) & M32
+            count = nb if cl == 0 else prev_nonempty * nb
+            cand.append((skip, nb, count))
+            if count == 0 or count > M32 or (maxb - minb) + 4 > M32:
+                continue
+            bn = (k - skip) & M32
+            addr = bn if cl == 0 else ((prank - 1) * nb) % (1 << 32) + bn
+            bins = np.bincount(addr, minlength=count).astype(np.float64)
+            nonempty = int((bins > 0).sum())
+            avg = float(n // nonempty)
+            if t >= nrp and avg >= n / 3.0:
+                sumsq[t] = float(n * n)
+            e = bins - avg
+            acc = np.add.accumulate(np.concatenate([[sumsq[t]], e * e]))
+            sumsq[t] = float(acc[-1])
+            score[t] = sumsq[t]
+        best = int(np.flatnonzero(score == score.min())[0]) if plant != "argmax" else \
+            int(np.flatnonzero(score == score[np.isfinite(score)].max())[0])
+        skip, nb, count = cand[best]
+        k = K[best]
+        bn = (k - skip) & M32
+        addr = bn if cl == 0 else ((prank - 1) * nb) % (1 << 32) + bn
+        uniq = np.unique(addr)
+        prank = np.searchsorted(uniq, addr) + 1
+        bnum[:, cl] = bn
+        prev_nonempty = len(uniq)
+        out["chosen"].append(best)
+        out["sse"].append(float(score[best]))
+        out["nodes"].append(len(uniq))
+    order = np.lexsort((np.arange(n), prank))
+    tuples = bnum[order]
+    starts = np.concatenate([[0], np.flatnonzero(np.any(tuples[1:] != tuples[:-1], axis=1)) + 1])
+    keys = tuples[starts]
+    sizes = np.diff(np.concatenate([starts, [n]]))
+    index = {tuple(t): v for v, t in enumerate(keys.tolist())}
+    adj = []
+    import itertools
+    for v, t in enumerate(keys.tolist()):
+        nb_set = set()
+        for delta in itertools.product((-1, 0, 1), repeat=layers):
+            w = index.get(tuple(a + b for a, b in zip(t, delta)))
+            if w is not None:
+                nb_set.add(w)
+        adj.append(nb_set)
+    calcs = sum(int(sizes[v]) * sum(int(sizes[w]) for w in adj[v]) for v in range(len(keys)))
+    out.update(node_order=order.astype(np.uint32), node_keys=keys.astype(np.uint32),
+               node_start=np.concatenate([starts, [n]]).astype(np.uint32), adj=adj,
+               num_calcs=calcs)
+    return out
+
+
+COSS_CASES = [
+    ("uniform d=2", lambda: datagen.uniform(6000, 2, 71), 0.01),
+    ("mixture d=5", lambda: datagen.mixture(6000, 5, 72), 0.06),
+    ("mixture d=18", lambda: datagen.mixture(5000, 18, 73), 0.2),
+    ("correlated d=40", lambda: datagen.correlated(3000, 40, 74), 0.5),
+    # axis keys below zero wrap the unsigned bin span: those axes are not eligible
+    ("centred d=3", lambda: datagen.uniform(4000, 3, 75) * 2.0 - 1.0, 0.02),
+]
+
+
+@pytest.mark.parametrize("name,gen,eps", COSS_CASES, ids=[c[0] for c in COSS_CASES])
+def test_oracle_coss_equals_numpy_restatement(name, gen, eps, oracle_mod):
+    x = _prep(gen(), oracle_mod)
+    ref = oracle_mod.self_join(x, eps, join=0, index=2)
+    mine = numpy_coss(x, eps, oracle_mod)
+    assert list(ref.chosen) == mine["chosen"]
+    assert [v.hex() for v in ref.sse] == [float(v).hex() for v in mine["sse"]]
+    assert list(ref.nodes_per_layer) == mine["nodes"]
+    assert np.array_equal(ref.node_order, mine["node_order"])
+    assert np.array_equal(ref.node_start, mine["node_start"])
+    assert np.array_equal(ref.node_keys, mine["node_keys"])
+    assert ref.num_calcs == mine["num_calcs"]
+    for v in range(ref.num_nodes):
+        assert set(ref.adj[ref.adj_off[v]:ref.adj_off[v + 1]].tolist()) == mine["adj"][v]
+    # the selection is the smallest statistic: a planted argmax is caught
+    planted = numpy_coss(x, eps, oracle_mod, plant="argmax")
+    assert planted["chosen"] != list(ref.chosen)
+
+
+@pytest.mark.parametrize("name,gen,eps", COSS_CASES, ids=[c[0] for c in COSS_CASES])
+def test_coss_join_is_complete(name, gen, eps, oracle_mod):
+    """Every pair within eps lies in adjacent bins: the join over the tree's
+    3^L neighbourhoods equals brute force."""
+    x = _prep(gen(), oracle_mod)
+    got = oracle_mod.self_join(x, eps, index=2)
+    bf = oracle_mod.brute_force(x, eps)
+    assert np.array_equal(got.counts, bf.counts)
+    assert np.array_equal(got.nbrs, bf.nbrs)
+
+
+def test_coss_fails_when_no_candidate_fits(oracle_mod):
+    """A layer whose every candidate needs 2^32 bins or more (the reference's
+    unsigned bin count would wrap) fails the build instead of guessing."""
+    x = _prep(datagen.uniform(20000, 2, 76), oracle_mod)
+    with pytest.raises(ValueError):
+        oracle_mod.self_join(x, 4e-6, join=0, index=2)
