@@ -133,6 +133,10 @@ typedef struct msj_stats {
     uint64_t tile_rows;          /* sum over adjacent node pairs X <= Y of |X|+|Y|:
                                     rows read once per candidate tile; 8*d bytes
                                     each is the join's algorithmic HBM traffic     */
+    uint64_t filter_checked_bf16;/* split-BF16 form with count_work: of the verified
+                                    pairs, those the BF16 first pass scored as
+                                    certain rejects (the test that lets a block
+                                    skip the cross products)                      */
 } msj_stats;
 
 typedef struct msj_result {

@@ -331,14 +331,18 @@ __global__ __launch_bounds__(256) void k_sblk_fill(const uint32_t *__restrict__ 
 // --------------------------------------------------------------- the filter
 // ctr: [0] result keys, [1] band entries, [2] max error (double bits),
 // [3] verified pairs, [4] violations, [5] error sum (double), [6] pairs
-// decided in FP64, [7] class-0 band entries.  band_budget != 0: workgroups
+// decided in FP64, [7] class-0 band entries, [8] wave blocks that took the
+// cross products, [9] verified class-3 pairs.  band_budget != 0: workgroups
 // stop taking items once [7] exceeds it (the BF16 bound was too loose; the
 // host reruns with split-BF16).
 //
 // Band entries: (join position p << 32 | class << 30 | join position c) and
 // the filter score.  Class 0: band pair, decided in FP64.  With DIAG
 // (count_work): class 1, a certain accept (every one is verified); class 2,
-// a certain reject of a sampled work item (verified).
+// a certain reject of a sampled work item (verified); class 3 (REF), a pair
+// of a sampled work item that the first, BF16 pass scores as a certain
+// reject under the BF16 bound -- the test that lets a wave block skip the
+// cross products (verified).
 //
 // Tile.  A workgroup (NW waves) computes a TM x TN tile (256 x 256, or a
 // 256 x 128 half tile with HALF; REF: split full tiles, BF16 pass first); wave (wr, wc) the 128 x 64 block of rows
@@ -782,6 +786,43 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
                         for (int bj = 0; bj < 2; ++bj)
                             m = fminf(m, fmaf(-2.f, acc[bi][bj][4 * eg + ei], ay[ei] + tbb[bj]));
                 }
+            if constexpr (DIAG) {
+                // sampled: every pair the BF16 score calls a certain reject
+                // goes to the FP64 check as class 3 before the cross products
+                // overwrite the accumulators
+                if (sampled) {
+                    const bool dg = self && cb < qbeg + TM && qbeg < cb + TN;
+#pragma unroll
+                    for (int bi = 0; bi < 4; ++bi)  // constant register indices
+#pragma unroll
+                        for (int e = 0; e < 16; ++e) {
+                            const uint32_t il = 128 * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
+#pragma unroll
+                            for (int bj = 0; bj < 2; ++bj) {
+                                const uint32_t jl = 64 * wc + 32 * bj + (lane & 31);
+                                const uint32_t p = qbeg + il, c = cb + jl;
+                                const bool valid =
+                                    p < w.qend && c < w.cend && (!dg || p <= c);
+                                const float sc =
+                                    fmaf(-2.f, acc[bi][bj][e], tAyb[il] + tByb[jl]);
+                                const bool push = valid && sc > hi;
+                                const uint64_t mb = __builtin_amdgcn_ballot_w64(push);
+                                if (!mb)
+                                    continue;
+                                const uint32_t k = (uint32_t)__popcll(mb);
+                                if (bfill + k > BHITS)
+                                    bflush();
+                                if (push) {
+                                    const uint32_t idx = bfill + (uint32_t)__popcll(mb & lt);
+                                    bb[idx] = ((unsigned long long)jA[il] << 32) |
+                                              (3ull << 30) | jB[jl];
+                                    sb[idx] = sc;
+                                }
+                                bfill += k;
+                            }
+                        }
+                }
+            }
             const bool need = sampled || tile_nan || !(m > hi);
             wneed = __builtin_amdgcn_ballot_w64(need) != 0;
             if (__syncthreads_or(need) != 0) {
@@ -1176,7 +1217,7 @@ __global__ __launch_bounds__(256) void k_band_check(
     const uint32_t lane = threadIdx.x & (WAVE - 1);
     const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (WAVE - lane));
     double emax = 0.0, esum = 0.0;
-    unsigned long long checked = 0, viol = 0, decided = 0;
+    unsigned long long checked = 0, viol = 0, decided = 0, checked3 = 0;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     // whole waves iterate together so the ballots below see every lane
     for (size_t base = (size_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(WAVE - 1)); base < nb;
@@ -1207,11 +1248,11 @@ __global__ __launch_bounds__(256) void k_band_check(
             }
             const bool ok = !(s > eps2);
             const double err = fabs((double)bscore[i] - s);
-            if (err == err) {
+            if (err == err && cls != 3) {  // class 3 holds BF16 scores
                 emax = err > emax ? err : emax;
                 esum += err;
             }
-            if (cls == 0 || cls == 3) {
+            if (cls == 0) {
                 ++decided;
                 emit = ok;
                 two = ok && ja != jb;
@@ -1220,7 +1261,8 @@ __global__ __launch_bounds__(256) void k_band_check(
             }
             if (DIAG && cls != 0) {
                 ++checked;
-                if ((cls == 1 && !ok) || (cls == 2 && ok))
+                checked3 += cls == 3;
+                if ((cls == 1 && !ok) || ((cls == 2 || cls == 3) && ok))
                     ++viol;
             }
         }
@@ -1249,6 +1291,7 @@ __global__ __launch_bounds__(256) void k_band_check(
         checked += __shfl_xor(checked, o, WAVE);
         viol += __shfl_xor(viol, o, WAVE);
         decided += __shfl_xor(decided, o, WAVE);
+        checked3 += __shfl_xor(checked3, o, WAVE);
     }
     if (lane == 0) {
         if (emax > 0.0)
@@ -1261,6 +1304,8 @@ __global__ __launch_bounds__(256) void k_band_check(
             atomicAdd(&ctr[4], viol);
         if (decided)
             atomicAdd(&ctr[6], decided);
+        if (checked3)
+            atomicAdd(&ctr[9], checked3);
     }
 }
 
@@ -1428,7 +1473,7 @@ uint64_t MfmaRunner::run(const Index &ix, JoinRunner &J, const double *xr, uint3
     // BF16 band budget: beyond it the BF16 bound is too loose for this data
     // and the join is rerun with split-BF16 (the kernel stops early)
     const uint64_t bf16_budget = std::max<uint64_t>(16ull * n, 1ull << 22);
-    ctr.ensure(9);
+    ctr.ensure(10);
     if (J.pairs.cap < cap_hint)
         J.pairs.ensure(cap_hint);
     *kernel_ms = 0.f;
@@ -1456,7 +1501,7 @@ uint64_t MfmaRunner::run(const Index &ix, JoinRunner &J, const double *xr, uint3
         const uint64_t per_xcd = (slots + 7) / 8;
         const uint32_t grid =
             (uint32_t)std::max<uint64_t>(8, std::min<uint64_t>(per_xcd * 8, MAX_GRID));
-        LAUNCH(k_zero_ctr, 1, 64, ctr.p, 9u);
+        LAUNCH(k_zero_ctr, 1, 64, ctr.p, 10u);
         MSJ_HIP(hipEventRecord(k0, s));
         if (nitems) {
 #define MSJ_MJ(SP, DG)                                                                          \
@@ -1501,7 +1546,7 @@ uint64_t MfmaRunner::run(const Index &ix, JoinRunner &J, const double *xr, uint3
             MSJ_HIP(hipEventRecord(kmid, s));
         }
         MSJ_HIP(hipEventRecord(k1, s));
-        unsigned long long h[9];
+        unsigned long long h[10];
         MSJ_HIP(hipMemcpyAsync(h, ctr.p, sizeof(h), hipMemcpyDeviceToHost, s));
         MSJ_HIP(hipStreamSynchronize(s));
         float ms = 0.f, mms = 0.f;
@@ -1526,6 +1571,7 @@ uint64_t MfmaRunner::run(const Index &ix, JoinRunner &J, const double *xr, uint3
             st->mean_score_err = nband ? esum / (double)nband : 0.0;
             st->filter_checked = h[3];
             st->filter_violations = h[4];
+            st->filter_checked_bf16 = h[9];
             st->form_used = split ? MSJ_FORM_MFMA_SPLIT : MSJ_FORM_MFMA_BF16;
             // split full tiles: BF16 products on every tile, the cross
             // products on the 128 x 64 wave blocks (8 per tile) that needed

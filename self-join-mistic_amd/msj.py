@@ -85,6 +85,7 @@ class Stats(C.Structure):
         ("mfma_flops", C.c_double),
         ("ms_main_kernel", C.c_double),
         ("tile_rows", C.c_uint64),
+        ("filter_checked_bf16", C.c_uint64),
     ]
 
 
@@ -269,6 +270,7 @@ class JoinResult:
             "form_used": st.form_used, "mfma_flops": st.mfma_flops,
             "split_share_ppm": st.split_share_ppm,
             "ms_main_kernel": st.ms_main_kernel, "tile_rows": st.tile_rows,
+            "filter_checked_bf16": st.filter_checked_bf16,
         }
         self.on_device = bool(r.on_device)
         if copy_arrays and not r.on_device:

@@ -99,40 +99,54 @@ def test_filter_bf16_first_settles_tiles(msj_mod, oracle_mod):
     assert st["form_used"] == msj_mod.FORM_MFMA_SPLIT
     assert 0 < st["split_share_ppm"] < 1000000  # some settled in BF16, some not
     assert st["filter_violations"] == 0
+    # the BF16 first pass's own certain rejects were among the verified pairs
+    assert st["filter_checked_bf16"] > 0
     print("[case] cross products on %.1f%% of the pairs" % (st["split_share_ppm"] / 1e4),
           flush=True)
 
 
+BOUND_CASES = [
+    ("correlated 20000x90 half tiles", lambda: datagen.correlated(20000, 90, 25), 0.77),
+    # full tiles: split-BF16 runs its BF16 pass first (class-3 checks)
+    ("correlated 4000x160 full tiles", lambda: datagen.correlated(4000, 160, 43), 1.6),
+]
+
+
 @pytest.mark.parametrize("form", FORMS)
-def test_filter_bound_verified(form, msj_mod, oracle_mod):
+@pytest.mark.parametrize("name,gen,eps", BOUND_CASES, ids=[c[0] for c in BOUND_CASES])
+def test_filter_bound_verified(name, gen, eps, form, msj_mod, oracle_mod):
     """count_work: sampled pairs' filter classes re-decided in FP64."""
-    x = _prep(datagen.correlated(20000, 90, 25), msj_mod)
-    eps = 0.77
-    _announce("bound check", x, eps, form)
+    x = _prep(gen(), msj_mod)
+    _announce("bound check " + name, x, eps, form)
     got = msj_mod.self_join(x, eps, form=_form(msj_mod, form), count_work=1)
     ref = oracle_mod.self_join(x, eps)
     _assert_same(got, ref)
     st = got.stats
     assert st["filter_checked"] > 100000
     assert st["filter_violations"] == 0
-    print("[case] checked %d sampled pairs, mean |score err| %.3g" % (
-        st["filter_checked"], st["mean_score_err"]), flush=True)
+    if x.shape[1] > 128 and st["form_used"] == msj_mod.FORM_MFMA_SPLIT:
+        assert st["filter_checked_bf16"] > 0
+    print("[case] checked %d sampled pairs (%d BF16-first rejects), mean |score err| %.3g" % (
+        st["filter_checked"], st["filter_checked_bf16"], st["mean_score_err"]), flush=True)
 
 
 @pytest.mark.parametrize("form", FORMS)
-def test_filter_extreme_rows(form, msj_mod, oracle_mod):
+@pytest.mark.parametrize("d", [40, 160])
+def test_filter_extreme_rows(d, form, msj_mod, oracle_mod):
     """Rows far outside the data (|a'|^2 > 2^100) fall in the band; a
-    duplicate pair of them is a result."""
-    x = datagen.mixture(3000, 40, 26)
+    duplicate pair of them is a result.  d = 160 runs full tiles (the
+    split form's BF16-first pass meets the NaN scores there)."""
+    x = datagen.mixture(3000, d, 26)
     x[17] = 1e60
     x[2900] = 1e60
     x[5, :7] = -3e55
     x = _prep(x, msj_mod)
-    eps = 0.5
+    eps = 0.5 if d == 40 else 1.0
     _announce("extreme rows", x, eps, form)
-    got = msj_mod.self_join(x, eps, form=_form(msj_mod, form))
+    got = msj_mod.self_join(x, eps, form=_form(msj_mod, form), count_work=1)
     ref = oracle_mod.self_join(x, eps)
     _assert_same(got, ref)
+    assert got.stats["filter_violations"] == 0
 
 
 @pytest.mark.parametrize("form", FORMS)
