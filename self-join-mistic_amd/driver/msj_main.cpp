@@ -14,7 +14,8 @@
 //
 // Extra options (after the four positional arguments):
 //   --form plain|bf16|split   distance form (default plain)
-//   --index nodenet|brute     candidate generator (default: the node net)
+//   --index nodenet|brute|coss  candidate generator (default: the node net;
+//                             coss: the tree's bins, tree.cu)
 //   --f32                     store coordinates as FP32 (rounded to nearest)
 //   --device N                HIP device ordinal
 //   --seed S                  srand seed of the reference-point stream
@@ -52,7 +53,7 @@ static int usage(const char *argv0)
 {
     std::fprintf(stderr,
                  "usage: %s <file.bin> <dim> <numRP> <eps> [--form plain|bf16|split]\n"
-                 "          [--index nodenet|brute] [--f32] [--device N] [--seed S]\n"
+                 "          [--index nodenet|brute|coss] [--f32] [--device N] [--seed S]\n"
                  "          [--page-pairs K] [--out PREFIX] [--strict] [--dry-run] [--verbose]\n",
                  argv0);
     return 2;

@@ -8,7 +8,8 @@
     result, and exactly the pairs planted (on the CPU) into a saved result;
   * result paging with a small page size (several pinned pages);
   * the device dimension order of msj_ctx_load_unordered (utils.cu:22-53);
-  * the brute-force candidate generator.
+  * the brute-force candidate generator, and the tree/COSS one through the
+    driver (--index coss).
 """
 import json
 import os
@@ -96,6 +97,25 @@ def test_main_end_to_end(name, make, eps, extra, tmp_path, msj_mod, oracle_mod):
     assert np.array_equal(arrays["offsets"], ref.offsets)
     assert np.array_equal(arrays["nbrs"], ref.nbrs)
     assert np.array_equal(arrays["node_order"], ref.node_order)
+
+
+def test_main_tree_index(tmp_path, msj_mod, oracle_mod):
+    """msj_main --index coss: the tree/COSS bins (tree.cu) as the candidate
+    generator; the --out arrays equal the oracle's tree join."""
+    x = datagen.mixture(12000, 6, 86)
+    f = _raw(tmp_path, x)
+    prefix = tmp_path / "coss"
+    eps = 0.07
+    r = run(MAIN, f, 6, 7, repr(eps), "--index", "coss", "--out", prefix)
+    assert r.returncode == 0, r.stderr
+    xo, _ = _ordered(x, msj_mod)
+    ref = oracle_mod.self_join(xo, eps, index=2)
+    assert "Total results Set Size: %d " % ref.total_pairs in r.stdout
+    assert "Final graph has %d nodes, %d calcs total" % (ref.num_nodes, ref.num_calcs) in r.stdout
+    assert np.array_equal(np.fromfile(str(prefix) + ".counts.u32", dtype="<u4"), ref.counts)
+    assert np.array_equal(np.fromfile(str(prefix) + ".nbrs.u32", dtype="<u4"), ref.nbrs)
+    assert np.array_equal(np.fromfile(str(prefix) + ".node_order.u32", dtype="<u4"),
+                          ref.node_order)
 
 
 def test_verify_clean_and_planted(tmp_path):
