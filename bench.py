@@ -509,7 +509,9 @@ def run_c5(args):
         calcs += r.stats["num_calcs"]
     log("[bench] counting pass: candidates=%d dims=%d" % (calcs, dims_eval))
 
-    with msj.Batch(output=msj.OUT_DEVICE) as b:
+    form = args.form or "plain"
+    fcode = FORMS[form]
+    with msj.Batch(output=msj.OUT_DEVICE, form=fcode) as b:
         b.load(datas, eps)
         for w in range(max(1, args.warmup)):
             _, st = b.run(copy_arrays=False)
@@ -533,7 +535,7 @@ def run_c5(args):
     ctxs = []
     try:
         for x in datas:
-            c = msj.Context(output=msj.OUT_DEVICE)
+            c = msj.Context(output=msj.OUT_DEVICE, form=fcode)
             c.load(x)
             ctxs.append(c)
         for c, e in zip(ctxs, eps):  # warm
@@ -546,11 +548,11 @@ def run_c5(args):
     finally:
         for c in ctxs:
             c.close()
-    log("[bench] 64 sequential plain joins: %.2f ms/step" % seq_ms)
+    log("[bench] 64 sequential %s joins: %.2f ms/step" % (form, seq_ms))
 
     # parity: batch (host output) against the sequential single joins
     log("[bench] parity: batch with host output vs 64 single joins")
-    res = msj.join_batch(datas, eps)
+    res = msj.join_batch(datas, eps, form=fcode)
     mism = 0
     for k, (x, e, r) in enumerate(zip(datas, eps, res)):
         s1 = msj.self_join(x, e)
@@ -565,12 +567,15 @@ def run_c5(args):
     line = {
         "metric": METRIC, "value": calcs / (ms_step / 1e3), "unit": "distance evals/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if form == "plain" else
+                 "f64 pooled jobs; %s MFMA filter + f64 band on the solo jobs" % form,
         "data": "synthetic (seeded, generated in-process)",
-        "config": {"workload": "batching form: 64 independent plain FP64 joins, sizes "
+        "config": {"workload": "batching form: 64 independent FP64 joins, sizes "
                                "log-uniform in [1e3, 1e6], d in {8, 18, 57, 90}, mixture data, "
                                "mean 16-64 neighbours", "jobs": len(datas),
                    "total_points": int(sum(p[0] for p in specs)), "form": "batch",
+                   "join_form": form,
                    "parallelism": "none (1 GPU)"},
         "join_time_s": ms_step / 1e3, "pairs_per_s": R / (ms_step / 1e3), "total_pairs": R,
         "candidate_evals": calcs, "ms_index": ms_index, "ms_join": ms_join,

@@ -195,7 +195,9 @@ int  msj_ctx_join(msj_ctx *ctx, double eps, msj_result *out);
  * streams: two builder threads build the jobs' node nets (largest first);
  * a job large enough to fill the GPU is joined as soon as its net is built,
  * overlapping the builds of the next jobs (the reference's NUMSTREAMS
- * overlap, params.cuh:63); the smaller jobs share one persistent kernel per
+ * overlap, params.cuh:63), in params.form (with an MFMA form, by the filter
+ * and its FP64 band recheck where msj_self_join would use them; the
+ * result is the same); the smaller jobs share one persistent plain kernel per
  * row width, replayed from a captured hipGraph, that drains a device work
  * queue of their work items; an output thread sorts each job's pairs as its
  * join completes.  Tile shapes of the persistent kernel (work per item,

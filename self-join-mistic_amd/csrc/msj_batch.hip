@@ -6,7 +6,8 @@
 //     order, FP64 rows and work items, largest jobs first;
 //   join stream: a job whose candidate evaluations fill the GPU on their own
 //     (MSJ_SOLO_CALCS or more) is joined as soon as its build is done, by the
-//     plain kernel of msj_self_join, while the builders go on with the next
+//     kernels of msj_self_join in the batch's form (the plain kernel, or the
+//     MFMA filter and its FP64 band recheck), while the builders go on with the next
 //     jobs (the build of job k+1 overlaps the join of job k, as the
 //     reference's NUMSTREAMS batches overlap, params.cuh:63); the other jobs
 //     wait for one persistent kernel per row-width class whose waves pop
@@ -178,6 +179,9 @@ struct BatchJob {
     uint64_t R = 0, last_R = 0;
     float ms_index = 0.f;
     bool solo = false;            // joined alone by the single-join kernel
+    bool mfma_form = false;       // solo, joined by the MFMA filter (params.form)
+    MfmaRunner mfma;
+    msj_stats mst = {};           // the filter's statistics of the last join
     hipEvent_t done = nullptr;    // its join is complete (recorded on the join stream)
     ~BatchJob()
     {
@@ -283,7 +287,7 @@ struct BatchEngine {
     std::vector<std::unique_ptr<Worker>> workers;  // builders, a stream each
     hipStream_t stream = nullptr;   // joins: solo kernels and the persistent launch
     hipStream_t ostream = nullptr;  // outputs
-    hipEvent_t ev[2] = {}, oev[2] = {};
+    hipEvent_t ev[2] = {}, oev[2] = {}, fev[3] = {};  // fev: a filter join's stages
     DevBuf<JobView> views;
     DevBuf<uint64_t> job_off;
     DevBuf<unsigned long long> queues, rcounts;
@@ -497,6 +501,9 @@ struct BatchEngine {
         for (auto &e : oev)
             if (e)
                 (void)hipEventDestroy(e);
+        for (auto &e : fev)
+            if (e)
+                (void)hipEventDestroy(e);
         if (ostream)
             (void)hipStreamDestroy(ostream);
         if (stream)
@@ -513,6 +520,8 @@ BatchEngine *batch_create(const msj_params &p)
     for (auto &e : b->ev)
         MSJ_HIP(hipEventCreate(&e));
     for (auto &e : b->oev)
+        MSJ_HIP(hipEventCreate(&e));
+    for (auto &e : b->fev)
         MSJ_HIP(hipEventCreate(&e));
     int cus = 0;
     MSJ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p.device));
@@ -569,11 +578,13 @@ void finish_job(BatchEngine *b, uint32_t j, msj_result &o, const msj_batch_stats
     BatchJob &B = *b->jobs[j];
     hipStream_t s = b->ostream;
     const bool dev_out = b->prm.output == MSJ_OUT_DEVICE;
-    unsigned long long R = 0;
-    MSJ_HIP(hipMemcpyAsync(&R, B.join.counter.p, sizeof(R), hipMemcpyDeviceToHost, s));
-    MSJ_HIP(hipStreamSynchronize(s));
-    B.R = R;
-    if (B.R > B.join.pairs.cap) {
+    if (!B.mfma_form) {  // the filter's run already returned its exact count
+        unsigned long long R = 0;
+        MSJ_HIP(hipMemcpyAsync(&R, B.join.counter.p, sizeof(R), hipMemcpyDeviceToHost, s));
+        MSJ_HIP(hipStreamSynchronize(s));
+        B.R = R;
+    }
+    if (!B.mfma_form && B.R > B.join.pairs.cap) {
         B.join.pairs.ensure(B.R);
         uint64_t launches = 0;
         float kms = 0.f;
@@ -602,6 +613,18 @@ void finish_job(BatchEngine *b, uint32_t j, msj_result &o, const msj_batch_stats
     o.stats.ms_index = B.ms_index;
     o.stats.join_launches = 1;
     o.stats.form_used = MSJ_FORM_PLAIN;
+    if (B.mfma_form) {
+        o.stats.join_launches = 2;
+        o.stats.form_used = B.mst.form_used;
+        o.stats.band_pairs = B.mst.band_pairs;
+        o.stats.max_score_err = B.mst.max_score_err;
+        o.stats.mean_score_err = B.mst.mean_score_err;
+        o.stats.filter_checked = B.mst.filter_checked;
+        o.stats.filter_violations = B.mst.filter_violations;
+        o.stats.filter_checked_bf16 = B.mst.filter_checked_bf16;
+        o.stats.mfma_flops = B.mst.mfma_flops;
+        o.stats.split_share_ppm = B.mst.split_share_ppm;
+    }
     if (dev_out) {
         MSJ_HIP(hipStreamSynchronize(s));
         o.on_device = 1;
@@ -680,10 +703,15 @@ void batch_run(BatchEngine *b, msj_result *outs, msj_batch_stats *st)
                     B.xr.ensure((size_t)B.n * padded_dim(B.d));
                     gather_node_rows(B.xs.p, B.xr.p, f32, B.join.jperm.p, B.n, B.d, W.s);
                     B.solo = B.ix.num_calcs >= MSJ_SOLO_CALCS;
+                    B.mfma_form = B.solo && b->prm.form != MSJ_FORM_PLAIN &&
+                                  mfma_filter_applies(B.eps, B.n, B.d);
                     const uint32_t ds = padded_dim(B.d);
                     const auto it = b->tuned.find(ds <= 32 ? ds : 0u);
                     const uint64_t epi = B.solo || it == b->tuned.end() ? 0 : it->second.first;
-                    B.join.plan(B.ix, B.n, B.d, W.s, true, epi);
+                    B.join.plan(B.ix, B.n, B.d, W.s, !B.mfma_form, epi);
+                    if (B.mfma_form)  // rows, planes, thresholds and tiles of the filter
+                        B.mfma.prepare(B.ix, B.join, B.xr.p, B.n, B.d, B.eps,
+                                       b->prm.form == MSJ_FORM_MFMA_SPLIT, W.s);
                     B.join.counter.ensure(1);
                     const uint64_t cap = std::max<uint64_t>(B.last_R + B.last_R / 8, 64ull * B.n + 4096);
                     if (B.join.pairs.cap < cap)
@@ -739,7 +767,18 @@ void batch_run(BatchEngine *b, msj_result *outs, msj_batch_stats *st)
                 MSJ_HIP(hipEventRecord(b->ev[0], s));
                 first = false;
             }
-            B.join.launch_plain(B.ix, B.xr.p, B.n, B.d, B.eps, s);
+            if (B.mfma_form) {
+                // the filter waits for its own counters (and reruns on a
+                // full result buffer) before the next job's dispatch
+                uint64_t launches = 0;
+                float kms = 0.f, mms = 0.f;
+                B.mst = msj_stats{};
+                B.R = B.mfma.run(B.ix, B.join, B.xr.p, B.n, B.d, B.eps, B.join.pairs.cap,
+                                 b->prm.count_work != 0, s, b->fev[0], b->fev[1], b->fev[2],
+                                 &launches, &kms, &mms, &B.mst);
+            } else {
+                B.join.launch_plain(B.ix, B.xr.p, B.n, B.d, B.eps, s);
+            }
             MSJ_HIP(hipEventRecord(B.done, s));
             ++S.solo_jobs;
             finished.push(j);

@@ -116,3 +116,26 @@ def test_batch_solo_jobs_overlap_builds(msj_mod):
             assert st["solo_jobs"] >= 1 and st["solo_jobs"] < len(datas)
             for x, e, r in zip(datas, eps, res):
                 _same(r, msj_mod.self_join(x, e))
+
+
+def test_batch_solo_jobs_in_filter_form(msj_mod):
+    """With an MFMA form the solo jobs run the filter and its FP64 band
+    recheck (as msj_self_join would); pooled jobs stay on the plain
+    persistent kernel.  Every job still equals the plain single join."""
+    datas, eps = [], []
+    for x, e in [(datagen.correlated(100000, 90, 961), 0.77),
+                 (datagen.mixture(3000, 57, 962), 0.5),
+                 (datagen.mixture(5000, 8, 963), 0.09)]:
+        order, _ = msj_mod.dim_order(x)
+        datas.append(datagen.reorder_dims(x, order))
+        eps.append(e)
+    print("\n[case] batch with filter-form solo jobs", flush=True)
+    with msj_mod.Batch(form=msj_mod.FORM_MFMA_SPLIT) as b:
+        b.load(datas, eps)
+        for rnd in range(2):
+            res, st = b.run()
+            assert st["solo_jobs"] == 1
+            assert res[0].stats["form_used"] == msj_mod.FORM_MFMA_SPLIT
+            assert res[1].stats["form_used"] == msj_mod.FORM_PLAIN
+            for x, e, r in zip(datas, eps, res):
+                _same(r, msj_mod.self_join(x, e))
