@@ -345,6 +345,7 @@ struct JoinRunner {
     DevBuf<uint32_t> jperm;      // join order: node order, Morton order inside a node
     DevBuf<uint64_t> jkeys;
     DevBuf<double> mm;
+    DevBuf<double> pca;          // sample, covariance, directions, projections
     uint32_t nitems = 0;
     uint64_t tile_rows = 0;      // sum over upper entries (X, Y) of |X| + |Y|
     Scanner scan;

@@ -30,6 +30,8 @@
 // sort then yields every row's neighbours ascending, the counts and the
 // CSR offsets.
 #include <algorithm>
+#include <cmath>
+#include <vector>
 #include <cstdio>
 
 #include "msj_join_items.h"
@@ -364,6 +366,124 @@ __global__ __launch_bounds__(256) void k_morton_keys(const T *__restrict__ xs, u
     vals[i] = row;
 }
 
+// Principal directions of the rows, for the join order of wide rows.
+constexpr uint32_t PCA_SAMPLE = 4096;  // rows of the covariance sample
+constexpr uint32_t PCA_DIMS = 128;     // leading dimensions it covers
+constexpr uint32_t PCA_K = 3;          // directions (Morton coordinates)
+
+// S[j * ns + i] = column j < dp of sample row i = i * n / ns.
+template <class T>
+__global__ __launch_bounds__(256) void k_pca_sample(const T *__restrict__ xs, uint32_t n,
+                                                    uint32_t ns, uint32_t dp,
+                                                    double *__restrict__ S)
+{
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ns * dp)
+        return;
+    const uint32_t j = e / ns, i = e - j * ns;
+    const uint32_t row = (uint32_t)((uint64_t)i * n / ns);
+    S[e] = (double)xs[(size_t)j * n + row];
+}
+
+// mu[j] = mean of sample column j (one wave per column).
+__global__ __launch_bounds__(256) void k_pca_mean(const double *__restrict__ S, uint32_t ns,
+                                                  uint32_t dp, double *__restrict__ mu)
+{
+    const uint32_t j = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    const uint32_t lane = threadIdx.x & (WAVE - 1);
+    if (j >= dp)
+        return;
+    double a = 0.0;
+    for (uint32_t i = lane; i < ns; i += WAVE)
+        a += S[(size_t)j * ns + i];
+    for (int o = WAVE / 2; o > 0; o >>= 1)
+        a += __shfl_xor(a, o, WAVE);
+    if (lane == 0)
+        mu[j] = a / ns;
+}
+
+// C[a * dp + b] (a <= b) = sum over the sample of the centred products (one
+// wave per pair).
+__global__ __launch_bounds__(256) void k_pca_cov(const double *__restrict__ S, uint32_t ns,
+                                                 uint32_t dp, const double *__restrict__ mu,
+                                                 double *__restrict__ C)
+{
+    const uint32_t w = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    const uint32_t lane = threadIdx.x & (WAVE - 1);
+    if (w >= dp * dp)
+        return;
+    const uint32_t a = w / dp, b = w - a * dp;
+    if (b < a)
+        return;
+    double acc = 0.0;
+    for (uint32_t i = lane; i < ns; i += WAVE)
+        acc += (S[(size_t)a * ns + i] - mu[a]) * (S[(size_t)b * ns + i] - mu[b]);
+    for (int o = WAVE / 2; o > 0; o >>= 1)
+        acc += __shfl_xor(acc, o, WAVE);
+    if (lane == 0)
+        C[w] = acc;
+}
+
+// P[k * n + row] = (x_row - mu) . V_k over the first dp columns.
+template <class T>
+__global__ __launch_bounds__(256) void k_pca_proj(const T *__restrict__ xs, uint32_t n,
+                                                  uint32_t dp, const double *__restrict__ mu,
+                                                  const double *__restrict__ V,
+                                                  double *__restrict__ P)
+{
+    const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= n)
+        return;
+    double p[PCA_K] = {};
+    for (uint32_t j = 0; j < dp; ++j) {
+        const double v = (double)xs[(size_t)j * n + row] - mu[j];
+#pragma unroll
+        for (uint32_t k = 0; k < PCA_K; ++k)
+            p[k] += v * V[k * dp + j];
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < PCA_K; ++k)
+        P[(size_t)k * n + row] = p[k];
+}
+
+// Top PCA_K eigenvectors of the symmetric dp x dp matrix C (upper triangle
+// filled) by power iteration with deflation; false if C is degenerate.
+bool top_directions(std::vector<double> &C, uint32_t dp, std::vector<double> &V)
+{
+    for (uint32_t a = 0; a < dp; ++a)
+        for (uint32_t b = 0; b < a; ++b)
+            C[(size_t)a * dp + b] = C[(size_t)b * dp + a];
+    V.assign((size_t)PCA_K * dp, 0.0);
+    std::vector<double> v(dp), w(dp);
+    for (uint32_t k = 0; k < PCA_K; ++k) {
+        for (uint32_t j = 0; j < dp; ++j)
+            v[j] = 1.0 + 0.001 * j + (j == k ? 1.0 : 0.0);
+        double lam = 0.0;
+        for (int it = 0; it < 100; ++it) {
+            double nn = 0.0;
+            for (uint32_t a = 0; a < dp; ++a) {
+                double t = 0.0;
+                for (uint32_t b = 0; b < dp; ++b)
+                    t += C[(size_t)a * dp + b] * v[b];
+                w[a] = t;
+                nn += t * t;
+            }
+            nn = std::sqrt(nn);
+            if (!(nn > 0.0) || !std::isfinite(nn))
+                return false;
+            lam = nn;
+            for (uint32_t j = 0; j < dp; ++j)
+                v[j] = w[j] / nn;
+        }
+        for (uint32_t j = 0; j < dp; ++j)
+            V[(size_t)k * dp + j] = v[j];
+        for (uint32_t a = 0; a < dp; ++a)  // deflate
+            for (uint32_t b = 0; b < dp; ++b)
+                C[(size_t)a * dp + b] -= lam * v[a] * v[b];
+    }
+    return true;
+}
+
 template <class T>
 __global__ void k_zero(T *p, size_t n)
 {
@@ -430,6 +550,52 @@ void JoinRunner::order(const Index &ix, const void *xs, bool f32, uint32_t n, ui
     const uint32_t crb = std::max<uint32_t>(1u, std::min<uint32_t>(CR_BLOCKS, (n + 255) / 256));
     jperm.ensure(n);
     jkeys.ensure(n);
+    // Wide rows (padded width > 32): order each node along the Morton curve
+    // of the rows' projections on the three leading principal directions (of
+    // a sample, over the first 128 columns).  Rows that sit together in that
+    // order are close in the dominant directions, so the pairs that one wave
+    // or one tile evaluates reach their decisions after similar numbers of
+    // dimensions.  Only the evaluation order changes.
+    if (padded_dim(d) > 32 && n >= 2) {
+        const uint32_t ns = std::min<uint32_t>(n, PCA_SAMPLE);
+        const uint32_t dp = std::min<uint32_t>(d, PCA_DIMS);
+        const size_t oS = 0, oMu = oS + (size_t)ns * dp, oC = oMu + dp, oV = oC + (size_t)dp * dp,
+                     oP = oV + (size_t)PCA_K * dp;
+        pca.ensure(oP + (size_t)PCA_K * n);
+        if (f32)
+            LAUNCH(k_pca_sample<float>, grid_for((size_t)ns * dp, 256), 256, (const float *)xs, n,
+                   ns, dp, pca.p + oS);
+        else
+            LAUNCH(k_pca_sample<double>, grid_for((size_t)ns * dp, 256), 256, (const double *)xs,
+                   n, ns, dp, pca.p + oS);
+        LAUNCH(k_pca_mean, grid_for(dp, 4), 256, pca.p + oS, ns, dp, pca.p + oMu);
+        LAUNCH(k_zero<double>, grid_for((size_t)dp * dp, 256), 256, pca.p + oC, (size_t)dp * dp);
+        LAUNCH(k_pca_cov, grid_for((size_t)dp * dp, 4), 256, pca.p + oS, ns, dp, pca.p + oMu,
+               pca.p + oC);
+        std::vector<double> hC((size_t)dp * dp), hV;
+        MSJ_HIP(hipMemcpyAsync(hC.data(), pca.p + oC, sizeof(double) * hC.size(),
+                               hipMemcpyDeviceToHost, s));
+        MSJ_HIP(hipStreamSynchronize(s));
+        if (top_directions(hC, dp, hV)) {
+            MSJ_HIP(hipMemcpyAsync(pca.p + oV, hV.data(), sizeof(double) * hV.size(),
+                                   hipMemcpyHostToDevice, s));
+            if (f32)
+                LAUNCH(k_pca_proj<float>, grid_for(n, 256), 256, (const float *)xs, n, dp,
+                       pca.p + oMu, pca.p + oV, pca.p + oP);
+            else
+                LAUNCH(k_pca_proj<double>, grid_for(n, 256), 256, (const double *)xs, n, dp,
+                       pca.p + oMu, pca.p + oV, pca.p + oP);
+            hipLaunchKernelGGL(k_col_range<double>, dim3(crb, PCA_K), dim3(256), 0, s,
+                               (const double *)(pca.p + oP), n, mm.p + 6);
+            MSJ_HIP(hipGetLastError());
+            LAUNCH(k_col_range_final, PCA_K, 64, mm.p + 6, crb, mm.p);
+            LAUNCH(k_morton_keys<double>, grid_for(n, 256), 256, (const double *)(pca.p + oP),
+                   n, PCA_K, mm.p, ix.nstart.p, ix.M, ix.perm.p, jkeys.p, jperm.p);
+            sorter.sort(jkeys.p, jperm.p, n, bits_for(ix.M ? ix.M - 1 : 0) + 30, s);
+            MSJ_HIP(hipStreamSynchronize(s));  // hV is read by the copy above
+            return;
+        }
+    }
     if (f32) {
         hipLaunchKernelGGL(k_col_range<float>, dim3(crb, km), dim3(256), 0, s, (const float *)xs,
                            n, mm.p + 6);
