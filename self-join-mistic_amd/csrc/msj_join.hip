@@ -369,7 +369,7 @@ __global__ __launch_bounds__(256) void k_morton_keys(const T *__restrict__ xs, u
 // Principal directions of the rows, for the join order of wide rows.
 constexpr uint32_t PCA_SAMPLE = 4096;  // rows of the covariance sample
 constexpr uint32_t PCA_DIMS = 128;     // leading dimensions it covers
-constexpr uint32_t PCA_K = 3;          // directions (Morton coordinates)
+constexpr uint32_t PCA_K = 6;          // directions computed (at most)
 
 // S[j * ns + i] = column j < dp of sample row i = i * n / ns.
 template <class T>
@@ -424,7 +424,7 @@ __global__ __launch_bounds__(256) void k_pca_cov(const double *__restrict__ S, u
         C[w] = acc;
 }
 
-// P[k * n + row] = (x_row - mu) . V_k over the first dp columns.
+// P[k * n + row] = (x_row - mu) . V_k over the first dp columns, k < PCA_K.
 template <class T>
 __global__ __launch_bounds__(256) void k_pca_proj(const T *__restrict__ xs, uint32_t n,
                                                   uint32_t dp, const double *__restrict__ mu,
@@ -482,6 +482,47 @@ bool top_directions(std::vector<double> &C, uint32_t dp, std::vector<double> &V)
                 C[(size_t)a * dp + b] -= lam * v[a] * v[b];
     }
     return true;
+}
+
+// As k_morton_keys for K coordinates of B = 30 / K bits each, in K columns
+// of xs (double, SoA).
+template <uint32_t K>
+__global__ __launch_bounds__(256) void k_morton_keys_k(const double *__restrict__ xs, uint32_t n,
+                                                       const double *__restrict__ mm,
+                                                       const uint32_t *__restrict__ nstart,
+                                                       uint32_t M,
+                                                       const uint32_t *__restrict__ perm,
+                                                       uint64_t *__restrict__ keys,
+                                                       uint32_t *__restrict__ vals)
+{
+    constexpr uint32_t B = 30 / K;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    uint32_t lo = 0, hi = M;  // node holding position i
+    while (hi - lo > 1) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (nstart[mid] <= i)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const uint32_t row = perm[i];
+    const double top = (double)((1u << B) - 1);
+    uint32_t code = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < K; ++k) {
+        const double a = mm[2 * k], b = mm[2 * k + 1];
+        const double v = xs[(size_t)k * n + row];
+        double t = b > a ? (v - a) / (b - a) * top : 0.0;
+        t = t > 0.0 ? (t < top ? t : top) : 0.0;  // also maps NaN to 0
+        const uint32_t q = (uint32_t)t;
+#pragma unroll
+        for (uint32_t bb = 0; bb < B; ++bb)
+            code |= ((q >> bb) & 1u) << (bb * K + k);
+    }
+    keys[i] = ((uint64_t)lo << 30) | code;
+    vals[i] = row;
 }
 
 template <class T>
@@ -546,13 +587,14 @@ void JoinRunner::order(const Index &ix, const void *xs, bool f32, uint32_t n, ui
                        hipStream_t s)
 {
     const uint32_t km = d < 3 ? d : 3u;
-    mm.ensure(2 * 3 + 2 * 3 * CR_BLOCKS);  // the result, then the partial ranges
+    mm.ensure(2 * PCA_K + 2 * PCA_K * CR_BLOCKS);  // the result, then the partial ranges
     const uint32_t crb = std::max<uint32_t>(1u, std::min<uint32_t>(CR_BLOCKS, (n + 255) / 256));
     jperm.ensure(n);
     jkeys.ensure(n);
     // Wide rows (padded width > 32): order each node along the Morton curve
-    // of the rows' projections on the three leading principal directions (of
-    // a sample, over the first 128 columns).  Rows that sit together in that
+    // of the rows' projections on the leading principal directions (of a
+    // sample, over the first 128 columns): three up to d = 128, six above
+    // (the 256-row tiles of the filter).  Rows that sit together in that
     // order are close in the dominant directions, so the pairs that one wave
     // or one tile evaluates reach their decisions after similar numbers of
     // dimensions.  Only the evaluation order changes.
@@ -585,12 +627,17 @@ void JoinRunner::order(const Index &ix, const void *xs, bool f32, uint32_t n, ui
             else
                 LAUNCH(k_pca_proj<double>, grid_for(n, 256), 256, (const double *)xs, n, dp,
                        pca.p + oMu, pca.p + oV, pca.p + oP);
-            hipLaunchKernelGGL(k_col_range<double>, dim3(crb, PCA_K), dim3(256), 0, s,
-                               (const double *)(pca.p + oP), n, mm.p + 6);
+            const uint32_t K = d > 128 ? 6u : 3u;
+            hipLaunchKernelGGL(k_col_range<double>, dim3(crb, K), dim3(256), 0, s,
+                               (const double *)(pca.p + oP), n, mm.p + 2 * PCA_K);
             MSJ_HIP(hipGetLastError());
-            LAUNCH(k_col_range_final, PCA_K, 64, mm.p + 6, crb, mm.p);
-            LAUNCH(k_morton_keys<double>, grid_for(n, 256), 256, (const double *)(pca.p + oP),
-                   n, PCA_K, mm.p, ix.nstart.p, ix.M, ix.perm.p, jkeys.p, jperm.p);
+            LAUNCH(k_col_range_final, K, 64, mm.p + 2 * PCA_K, crb, mm.p);
+            if (K == 6)
+                LAUNCH(k_morton_keys_k<6>, grid_for(n, 256), 256, (const double *)(pca.p + oP),
+                       n, mm.p, ix.nstart.p, ix.M, ix.perm.p, jkeys.p, jperm.p);
+            else
+                LAUNCH(k_morton_keys_k<3>, grid_for(n, 256), 256, (const double *)(pca.p + oP),
+                       n, mm.p, ix.nstart.p, ix.M, ix.perm.p, jkeys.p, jperm.p);
             sorter.sort(jkeys.p, jperm.p, n, bits_for(ix.M ? ix.M - 1 : 0) + 30, s);
             MSJ_HIP(hipStreamSynchronize(s));  // hV is read by the copy above
             return;
