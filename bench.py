@@ -25,6 +25,9 @@ Also printed (same JSON line):
                 rocprofv3 summaries under profiles/ hold it;
   mismatch_rows_between_forms  rows whose neighbour lists differ between the
                 forms, over all rows;
+  mismatch_rows_vs_brute_force  (default workload) rows whose neighbour list
+                from the node net differs from the brute-force candidate
+                generator's (every pair a candidate), over all rows;
   cpu_baseline  the CPU restatement (oracle) on every CPU this process may
                 use, on a sampled join (every k-th query point in node order,
                 full candidate lists), as candidate evaluations per second;
@@ -469,6 +472,18 @@ def main():
         line["node_order_equal_between_forms"] = all(
             np.array_equal(base.node_order, hosts[f].node_order) for f in forms[1:])
         log("[bench] forms compared on every row: %s" % line["mismatch_rows_between_forms"])
+    # full-size parity of the candidate generator: the node net's join against
+    # the brute-force generator (one node, every pair a candidate, the same
+    # FP64 decisions) on every row, in the fastest timed form (untimed)
+    if args.config == "c3" and not args.rows:
+        fb = head
+        log("[bench] brute-force generator (%s form), all rows" % fb)
+        bf = msj.self_join(x, eps, form=FORMS[fb], index=msj.INDEX_BRUTE,
+                           storage=storage)
+        line["mismatch_rows_vs_brute_force"] = same_result(hosts[fb], bf)
+        line["brute_force_candidate_evals"] = bf.stats["num_calcs"]
+        log("[bench] node net vs brute force on every row: %d rows differ"
+            % line["mismatch_rows_vs_brute_force"])
     if not args.no_cpu_baseline:
         threads, hostinfo = host_cpus()
         if args.cpu_threads:

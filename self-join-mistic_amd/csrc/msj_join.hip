@@ -13,11 +13,14 @@
 // Work decomposition.  A work item is one wave: a range of X's points (the
 // queries) against a range of X's upper candidate list (the points of the
 // neighbour nodes Y >= X, concatenated).  The wave walks that range 64
-// candidates at a time, one candidate per lane with its first DR
-// coordinates in VGPRs, and loops over the queries; a query's coordinates
-// are wave-uniform, so they are scalar loads into SGPRs and reach every lane
-// with no broadcast instruction.  The FP64 data is a node-ordered row-major
-// copy, so a query row is one contiguous scalar load.
+// candidates at a time, one candidate per lane with its coordinates in VGPRs
+// (rows of up to 32 dimensions whole, wider rows 16 dimensions per slab),
+// and loops over the queries, whose rows it stages in its own LDS slice: a
+// query's coordinates are wave-uniform LDS addresses (broadcast reads).
+// Items are ordered candidate-chunk-major, so the query groups that share a
+// candidate chunk run side by side in one workgroup (L1/L2 reuse).  Rows in
+// join order: node order, and inside a node a space-filling-curve order
+// (JoinRunner::order).
 //
 // Decision: exactly the reference's sequence per 4-dimension chunk,
 //   s = fma(d0, d0, s); s = s + d1*d1; s = s + d2*d2; s = s + d3*d3;
@@ -25,10 +28,10 @@
 // with this file compiled without floating-point contraction.  The wave
 // leaves the chunk loop as soon as no lane is still undecided.
 //
-// Emission: one atomic per wave per accepting query (ballot + popcount
-// prefix), not per pair.  Keys are (row << qbits | neighbour row); one radix
-// sort then yields every row's neighbours ascending, the counts and the
-// CSR offsets.
+// Emission: accepted keys (row << qbits | neighbour row) go to a per-wave
+// LDS buffer; the wave reserves global space with one atomic per flush and
+// copies the buffer out with coalesced stores.  One radix sort then yields
+// every row's neighbours ascending, the counts and the CSR offsets.
 #include <algorithm>
 #include <cmath>
 #include <vector>
