@@ -423,6 +423,7 @@ struct MfmaRunner {
     uint64_t nitems = 0;         // super-blocks
     uint64_t tiles = 0;          // 128 x 128 tiles of the last prepare()
     SlotMap smap = {};
+    DevBuf<SlotMap> smapd;       // the same in device memory (the kernel reads it there)
     bool split = false;
     Scanner scan;
     // Padded rows, centre, fragments and thresholds from the node-ordered

@@ -367,7 +367,8 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
     const uint4 *__restrict__ fh, const uint4 *__restrict__ fl, uint32_t KS,
     const float4 *__restrict__ thr, const float2 *__restrict__ thrb,
     const uint32_t *__restrict__ porig,
-    const uint32_t *__restrict__ pjoin, const MItem *__restrict__ items, SlotMap sm,
+    const uint32_t *__restrict__ pjoin, const MItem *__restrict__ items,
+    const SlotMap *__restrict__ smp,
     uint64_t per_xcd, float lo, float hi, int qbits, unsigned long long *__restrict__ out,
     unsigned long long cap, unsigned long long *__restrict__ ctr,
     unsigned long long *__restrict__ band, float *__restrict__ bscore, unsigned long long bcap,
@@ -469,6 +470,9 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
         const uint64_t u = (v % 8u) * per_xcd + v / 8u;
         const uint64_t t = HALF ? u >> 1 : u;
         S.half = HALF ? (uint32_t)(u & 1u) : 0u;
+        // the slot map stays in memory (scalar loads once per tile): held
+        // in SGPRs for the whole kernel it pushed other values into spills
+        const SlotMap &sm = *smp;
         S.in = t < sm.slot_base[SBS];
         uint32_t cls = 0;
         while (S.in && cls + 1 < SBS && t >= sm.slot_base[cls + 1])
@@ -1453,6 +1457,8 @@ void MfmaRunner::prepare(const Index &ix, const JoinRunner &J, const double *xr,
     smap.slot_base[SBS] = slot;
     if (slot >= (1ull << 40))
         throw Error(MSJ_E_OVERFLOW, "filter tile slots exceed 2^40");
+    smapd.ensure(1);
+    MSJ_HIP(hipMemcpyAsync(smapd.p, &smap, sizeof(SlotMap), hipMemcpyHostToDevice, s));
     items.ensure(std::max<uint64_t>(nitems, 1));
     if (nitems)
         LAUNCH(k_sblk_fill, grid_for(nitems, 256), 256, ix.nstart.p, pstart.p, esrc.p, J.uadj.p,
@@ -1514,7 +1520,7 @@ uint64_t MfmaRunner::run(const Index &ix, JoinRunner &J, const double *xr, uint3
 #define MSJ_MJH(SP, DG, HF, RF, NT)                                                             \
     LAUNCH((k_mfma_join<SP, DG, HF, RF>), grid, NT, fh.p, SP ? fl.p : fh.p, KS, thr.p,          \
            RF ? thrb.p : nullptr, porig.p, pjoin.p,                                             \
-           items.p, smap, per_xcd, lo, hi, qbits, (unsigned long long *)J.pairs.p,              \
+           items.p, smapd.p, per_xcd, lo, hi, qbits, (unsigned long long *)J.pairs.p,           \
            (unsigned long long)J.pairs.cap, ctr.p, (unsigned long long *)band.p, bscore.p,      \
            (unsigned long long)band.cap, sample_every, (unsigned long long)budget)
             if (split) {
