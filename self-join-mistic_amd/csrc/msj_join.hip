@@ -489,8 +489,8 @@ bool top_directions(std::vector<double> &C, uint32_t dp, std::vector<double> &V)
 
 // As k_morton_keys for K coordinates of B = 30 / K bits each, in K columns
 // of xs (double, SoA).
-template <uint32_t K>
-__global__ __launch_bounds__(256) void k_morton_keys_k(const double *__restrict__ xs, uint32_t n,
+template <uint32_t K, class T>
+__global__ __launch_bounds__(256) void k_morton_keys_k(const T *__restrict__ xs, uint32_t n,
                                                        const double *__restrict__ mm,
                                                        const uint32_t *__restrict__ nstart,
                                                        uint32_t M,
@@ -516,7 +516,7 @@ __global__ __launch_bounds__(256) void k_morton_keys_k(const double *__restrict_
 #pragma unroll
     for (uint32_t k = 0; k < K; ++k) {
         const double a = mm[2 * k], b = mm[2 * k + 1];
-        const double v = xs[(size_t)k * n + row];
+        const double v = (double)xs[(size_t)k * n + row];
         double t = b > a ? (v - a) / (b - a) * top : 0.0;
         t = t > 0.0 ? (t < top ? t : top) : 0.0;  // also maps NaN to 0
         const uint32_t q = (uint32_t)t;
@@ -636,15 +636,35 @@ void JoinRunner::order(const Index &ix, const void *xs, bool f32, uint32_t n, ui
             MSJ_HIP(hipGetLastError());
             LAUNCH(k_col_range_final, K, 64, mm.p + 2 * PCA_K, crb, mm.p);
             if (K == 6)
-                LAUNCH(k_morton_keys_k<6>, grid_for(n, 256), 256, (const double *)(pca.p + oP),
+                LAUNCH((k_morton_keys_k<6, double>), grid_for(n, 256), 256, (const double *)(pca.p + oP),
                        n, mm.p, ix.nstart.p, ix.M, ix.perm.p, jkeys.p, jperm.p);
             else
-                LAUNCH(k_morton_keys_k<3>, grid_for(n, 256), 256, (const double *)(pca.p + oP),
+                LAUNCH((k_morton_keys_k<3, double>), grid_for(n, 256), 256, (const double *)(pca.p + oP),
                        n, mm.p, ix.nstart.p, ix.M, ix.perm.p, jkeys.p, jperm.p);
             sorter.sort(jkeys.p, jperm.p, n, bits_for(ix.M ? ix.M - 1 : 0) + 30, s);
             MSJ_HIP(hipStreamSynchronize(s));  // hV is read by the copy above
             return;
         }
+    }
+    // rows of 9 to 32 dimensions: the first six columns (five bits each)
+    if (padded_dim(d) > 8) {
+        if (f32) {
+            hipLaunchKernelGGL(k_col_range<float>, dim3(crb, 6), dim3(256), 0, s,
+                               (const float *)xs, n, mm.p + 2 * PCA_K);
+            MSJ_HIP(hipGetLastError());
+            LAUNCH(k_col_range_final, 6, 64, mm.p + 2 * PCA_K, crb, mm.p);
+            LAUNCH((k_morton_keys_k<6, float>), grid_for(n, 256), 256, (const float *)xs, n,
+                   mm.p, ix.nstart.p, ix.M, ix.perm.p, jkeys.p, jperm.p);
+        } else {
+            hipLaunchKernelGGL(k_col_range<double>, dim3(crb, 6), dim3(256), 0, s,
+                               (const double *)xs, n, mm.p + 2 * PCA_K);
+            MSJ_HIP(hipGetLastError());
+            LAUNCH(k_col_range_final, 6, 64, mm.p + 2 * PCA_K, crb, mm.p);
+            LAUNCH((k_morton_keys_k<6, double>), grid_for(n, 256), 256, (const double *)xs, n,
+                   mm.p, ix.nstart.p, ix.M, ix.perm.p, jkeys.p, jperm.p);
+        }
+        sorter.sort(jkeys.p, jperm.p, n, bits_for(ix.M ? ix.M - 1 : 0) + 30, s);
+        return;
     }
     if (f32) {
         hipLaunchKernelGGL(k_col_range<float>, dim3(crb, km), dim3(256), 0, s, (const float *)xs,
