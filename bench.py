@@ -531,9 +531,9 @@ def run_c5(args):
         for w in range(max(1, args.warmup)):
             _, st = b.run(copy_arrays=False)
             log("[bench] warmup %d: %.1f ms (tune %.1f ms, solo jobs %d, pooled classes %s, "
-                "epi log2 %s, grid %d)" % (w, st["ms_total"], st["ms_tune"], st["solo_jobs"],
-                                           st["class_ds"], st["class_epi_log2"],
-                                           st["grid_blocks"]))
+                "epi log2 %s, slab dims %s, grid %d)" % (
+                    w, st["ms_total"], st["ms_tune"], st["solo_jobs"], st["class_ds"],
+                    st["class_epi_log2"], st["class_slab"], st["grid_blocks"]))
         stage = []
         t0 = time.perf_counter()
         for k in range(args.steps):
@@ -597,6 +597,7 @@ def run_c5(args):
         "ms_output": ms_output, "sequential_ms_per_step": seq_ms,
         "speedup_vs_sequential": seq_ms / ms_step, "mismatch_jobs": mism,
         "tuned": {"class_ds": st["class_ds"], "epi_log2": st["class_epi_log2"],
+                  "slab_dims": st["class_slab"],
                   "grid_blocks": st["grid_blocks"], "work_items": st["work_items"]},
         "solo_jobs": st["solo_jobs"],
         "stages_overlap": "builds (2 builder streams) overlap the joins (join stream) and the "

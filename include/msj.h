@@ -201,8 +201,9 @@ int  msj_ctx_join(msj_ctx *ctx, double eps, msj_result *out);
  * row width, replayed from a captured hipGraph, that drains a device work
  * queue of their work items; an output thread sorts each job's pairs as its
  * join completes.  Tile shapes of the persistent kernel (work per item,
- * workgroups per CU) are chosen per row width by a built-in autotuner on the
- * first run and kept by the batch. */
+ * workgroups per CU, and for rows wider than 32 the dimensions per slab)
+ * are chosen per row width by a built-in autotuner on the first run and
+ * kept by the batch. */
 typedef struct msj_job {
     const void *data;            /* host, n x d row-major, dims sigma-ordered,
                                     precision of params.storage                   */
@@ -231,6 +232,8 @@ typedef struct msj_batch_stats {
     double   ms_total;           /* host clock of msj_batch_run                     */
     uint32_t graph_replays;      /* launches through the captured graph             */
     uint32_t solo_jobs;          /* jobs joined alone (enough work to fill the GPU)  */
+    uint32_t class_slab[8];      /* per class: dimensions per slab of the wide-row
+                                    kernel (8 or 16; 0 for widths <= 32)         */
 } msj_batch_stats;
 
 typedef struct msj_batch msj_batch;

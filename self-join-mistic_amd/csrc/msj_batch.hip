@@ -22,12 +22,15 @@
 // same node nets).
 //
 // Autotuner.  The tile shape of the plain form is the work per item (query
-// rows x candidate columns, set by the lane evaluations per item) and the
-// number of resident workgroups per CU.  For each class of the persistent
-// launch not tuned yet, its jobs are planned with 2^16, 2^18 and 2^20
-// evaluations per item and the class kernel is timed alone on a prefix of
-// its queue; then, with the best of these, 1, 2 and 4 workgroups per CU.
-// The choices are kept by the batch for later runs.
+// rows x candidate columns, set by the lane evaluations per item), the
+// number of resident workgroups per CU and, for rows wider than 32, the
+// dimensions per slab (the candidate coordinates a wave holds in registers
+// and the query rows it stages in LDS at a time).  For each class of the
+// persistent launch not tuned yet, its jobs are planned with 2^16, 2^18 and
+// 2^20 evaluations per item and the class kernel is timed alone on a prefix
+// of its queue; then, with the best of these, 1, 2 and 4 workgroups per CU;
+// then, for the wide class, slabs of 8 and 16 dimensions.  The choices are
+// kept by the batch for later runs.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -67,7 +70,7 @@ __device__ inline uint32_t xcd_id()
     return (uint32_t)__builtin_amdgcn_s_getreg(20 | (0 << 6) | (3 << 11)) % NSEG;
 }
 
-template <int DS>
+template <int DS, int SL = HSL>
 __device__ inline void batch_loop(const JobView *__restrict__ views,
                                   const uint64_t *__restrict__ job_off, uint32_t jb, uint32_t je,
                                   unsigned long long *__restrict__ queue, uint64_t limit)
@@ -114,7 +117,7 @@ __device__ inline void batch_loop(const JobView *__restrict__ views,
         if (DS > 0)
             item_sym<(DS > 0 ? DS : 4), false>(J, w, qsh[wv], H, dims);
         else
-            item_hd<false>(J, w, qsh[wv], H, dims);
+            item_hd<false, SL>(J, w, qsh[wv], H, dims);
         if (H.fill)
             H.flush(J);  // the keys belong to this job
     }
@@ -130,14 +133,16 @@ __global__ __launch_bounds__(256) void k_join_batch(const JobView *__restrict__ 
     batch_loop<DS>(views, job_off, jb, je, queue, limit);
 }
 
-// ds > 32: four waves per SIMD, as the single-join kernel.
+// ds > 32: four waves per SIMD, as the single-join kernel; SL dimensions
+// per slab (tuned: 8 or 16).
+template <int SL>
 __global__ __launch_bounds__(256, 4) void k_join_batch_hd(const JobView *__restrict__ views,
                                                           const uint64_t *__restrict__ job_off,
                                                           uint32_t jb, uint32_t je,
                                                           unsigned long long *__restrict__ queue,
                                                           uint64_t limit)
 {
-    batch_loop<0>(views, job_off, jb, je, queue, limit);
+    batch_loop<0, SL>(views, job_off, jb, je, queue, limit);
 }
 
 __global__ void k_batch_reset(const JobView *__restrict__ views, uint32_t njobs,
@@ -248,6 +253,13 @@ struct Cls {
     uint32_t ds, jb, je;
     uint64_t epi;   // tuned lane evaluations per item
     uint32_t wg;    // tuned workgroups per CU
+    uint32_t sl;    // ds > 32: tuned dimensions per slab (8 or 16)
+};
+
+// The autotuner's choice for one row-width class.
+struct Tuned {
+    uint64_t epi;
+    uint32_t wg, sl;
 };
 
 constexpr uint32_t MAX_CLASSES = 8;  // padded widths 4..32 step 4, and ds > 32
@@ -273,8 +285,12 @@ void launch_class(const Cls &c, uint32_t grid, const JobView *views, const uint6
         MSJ_C(32)
 #undef MSJ_C
     default:
-        hipLaunchKernelGGL(k_join_batch_hd, g, blk, 0, s, views, job_off, c.jb, c.je, queue,
-                           limit);
+        if (c.sl == 8)
+            hipLaunchKernelGGL(k_join_batch_hd<8>, g, blk, 0, s, views, job_off, c.jb, c.je,
+                               queue, limit);
+        else
+            hipLaunchKernelGGL(k_join_batch_hd<HSL>, g, blk, 0, s, views, job_off, c.jb, c.je,
+                               queue, limit);
     }
     MSJ_HIP(hipGetLastError());
 }
@@ -293,7 +309,7 @@ struct BatchEngine {
     DevBuf<unsigned long long> queues, rcounts;
     std::vector<uint32_t> order;  // table position -> job (the pooled jobs by padded width)
     std::vector<Cls> cls;
-    std::map<uint32_t, std::pair<uint64_t, uint32_t>> tuned;  // ds -> (epi, wg)
+    std::map<uint32_t, Tuned> tuned;  // ds (0: every ds > 32) -> choice
     int cus = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -340,7 +356,7 @@ struct BatchEngine {
             const uint32_t ds = padded_dim(B.d);
             const uint32_t key = ds <= 32 ? ds : 0u;  // every ds > 32 shares one kernel
             if (nc.empty() || (nc.back().ds <= 32 ? nc.back().ds : 0u) != key)
-                nc.push_back(Cls{ds, k, k, 0, 0});
+                nc.push_back(Cls{ds, k, k, 0, 0, (uint32_t)HSL});
             nc.back().je = k + 1;
             hv.push_back(B.join.view(B.ix, B.xr.p, B.n, B.d, B.eps));
             off.push_back(off.back() + B.join.nitems);
@@ -349,8 +365,9 @@ struct BatchEngine {
             throw Error(MSJ_E_UNSUPPORTED, "too many row-width classes");
         for (auto &c : nc) {
             const auto it = tuned.find(c.ds <= 32 ? c.ds : 0u);
-            c.epi = it == tuned.end() ? 0 : it->second.first;
-            c.wg = it == tuned.end() ? 2 : it->second.second;
+            c.epi = it == tuned.end() ? 0 : it->second.epi;
+            c.wg = it == tuned.end() ? 2 : it->second.wg;
+            c.sl = it == tuned.end() ? (uint32_t)HSL : it->second.sl;
         }
         cls = nc;
         views.ensure(std::max<size_t>(hv.size(), 1));
@@ -404,6 +421,7 @@ struct BatchEngine {
             key.push_back(c.jb);
             key.push_back(c.je);
             key.push_back(c.wg);
+            key.push_back(c.sl);
         }
         if (!gexec || key != graph_key) {
             if (gexec)
@@ -444,9 +462,10 @@ struct BatchEngine {
             const uint32_t key = cls[c].ds <= 32 ? cls[c].ds : 0u;
             if (tuned.count(key))
                 continue;
-            auto trial = [&](uint32_t wg) {
+            auto trial = [&](uint32_t wg, uint32_t sl) {
                 upload_table(pooled);
                 cls[c].wg = wg;
+                cls[c].sl = sl;
                 uint64_t items = 0;
                 for (uint32_t k = cls[c].jb; k < cls[c].je; ++k)
                     items += jobs[order[k]]->join.nitems;
@@ -458,7 +477,7 @@ struct BatchEngine {
             uint64_t best_e = 1ull << 18;
             for (uint64_t e : {1ull << 16, 1ull << 18, 1ull << 20}) {
                 replan(pooled, key, e);
-                const double t = trial(2);
+                const double t = trial(2, HSL);
                 if (best < 0 || t < best) {
                     best = t;
                     best_e = e;
@@ -468,13 +487,24 @@ struct BatchEngine {
             best = -1.0;
             uint32_t best_wg = 2;
             for (uint32_t wg : {1u, 2u, 4u}) {
-                const double t = trial(wg);
+                const double t = trial(wg, HSL);
                 if (best < 0 || t < best) {
                     best = t;
                     best_wg = wg;
                 }
             }
-            tuned[key] = {best_e, best_wg};
+            uint32_t best_sl = HSL;
+            if (key == 0) {  // wide rows: dimensions per slab, 8 or 16
+                best = -1.0;
+                for (uint32_t sl : {8u, (uint32_t)HSL}) {
+                    const double t = trial(best_wg, sl);
+                    if (best < 0 || t < best) {
+                        best = t;
+                        best_sl = sl;
+                    }
+                }
+            }
+            tuned[key] = Tuned{best_e, best_wg, best_sl};
         }
     }
 
@@ -707,7 +737,7 @@ void batch_run(BatchEngine *b, msj_result *outs, msj_batch_stats *st)
                                   mfma_filter_applies(B.eps, B.n, B.d);
                     const uint32_t ds = padded_dim(B.d);
                     const auto it = b->tuned.find(ds <= 32 ? ds : 0u);
-                    const uint64_t epi = B.solo || it == b->tuned.end() ? 0 : it->second.first;
+                    const uint64_t epi = B.solo || it == b->tuned.end() ? 0 : it->second.epi;
                     B.join.plan(B.ix, B.n, B.d, W.s, !B.mfma_form, epi);
                     if (B.mfma_form)  // rows, planes, thresholds and tiles of the filter
                         B.mfma.prepare(B.ix, B.join, B.xr.p, B.n, B.d, B.eps,
@@ -847,6 +877,7 @@ void batch_run(BatchEngine *b, msj_result *outs, msj_batch_stats *st)
             break;
         S.class_ds[k] = c.ds <= 32 ? c.ds : 0u;
         S.class_epi_log2[k] = (uint32_t)bits_for(c.epi) - 1;
+        S.class_slab[k] = c.ds > 32 ? c.sl : 0u;
         ++k;
     }
     S.classes = k;

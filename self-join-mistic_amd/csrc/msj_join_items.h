@@ -224,14 +224,15 @@ __device__ inline void item_sym(const JobView &J, const JoinItem &w, double *qs,
 }
 
 // High-dimensional rows (ds > 32).  Slab-major order: the wave holds the
-// 64 candidates' next HSL dimensions in VGPRs (each loaded once) and
+// 64 candidates' next SL dimensions (HSL, or the batch autotuner's choice)
+// in VGPRs (each loaded once) and
 // advances the partial sums of all of the item's queries (up to HQ, one FP64
 // sum per lane and query in VGPRs) through that slab, chunk by chunk in the
 // reference's order, with the same early-exit test; a query row slab is
 // staged in the wave's LDS slice and read as a broadcast.  Decisions are
 // those of the per-pair sequence: the chunks of every pair run in order,
 // each followed by the s > eps^2 test.
-template <bool COUNT>
+template <bool COUNT, int SL = HSL>
 __device__ inline void item_hd(const JobView &J, const JoinItem &w, double *qs, WaveHits &H,
                                uint64_t &dims)
 {
@@ -271,18 +272,18 @@ __device__ inline void item_hd(const JobView &J, const JoinItem &w, double *qs, 
         for (int q = 0; q < HQ; ++q)
             sq[q] = 0.0;
         const double *crow = J.xr + (size_t)c * ds;
-        for (uint32_t l0 = 0; l0 < ds; l0 += HSL) {
+        for (uint32_t l0 = 0; l0 < ds; l0 += SL) {
             if (!wave_vote(alive != 0))
                 break;
-            // query slab -> LDS (qn rows of HSL dims), candidate slab -> VGPRs
+            // query slab -> LDS (qn rows of SL dims), candidate slab -> VGPRs
             wave_sync();
-            for (uint32_t e = lane; e < HQ * HSL; e += WAVE) {
-                const uint32_t q = e / HSL, j = e % HSL;
+            for (uint32_t e = lane; e < HQ * SL; e += WAVE) {
+                const uint32_t q = e / SL, j = e % SL;
                 qs[e] = (q < qn && l0 + j < ds) ? J.xr[(size_t)(q0 + q) * ds + l0 + j] : 0.0;
             }
-            double cc[HSL];
+            double cc[SL];
 #pragma unroll
-            for (int j = 0; j < HSL; j += 2) {
+            for (int j = 0; j < SL; j += 2) {
                 if (l0 + j < ds) {
                     const double2 v = *(const double2 *)(crow + l0 + j);
                     cc[j] = v.x;
@@ -304,14 +305,14 @@ __device__ inline void item_hd(const JobView &J, const JoinItem &w, double *qs, 
                 const uint64_t am0 = am;
                 // the query's slab in one batch of broadcast reads (one LDS
                 // round trip per query and slab, not per chunk)
-                const double *qrow = qs + q * HSL;  // wave-uniform LDS address
-                double2 qv[HSL / 2];
+                const double *qrow = qs + q * SL;  // wave-uniform LDS address
+                double2 qv[SL / 2];
 #pragma unroll
-                for (int j = 0; j < HSL / 2; ++j)
+                for (int j = 0; j < SL / 2; ++j)
                     qv[j] = *(const double2 *)(qrow + 2 * j);
                 double s = sq[q];
 #pragma unroll
-                for (int k = 0; k < HSL; k += 4) {
+                for (int k = 0; k < SL; k += 4) {
                     if (l0 + k >= ds || !am)
                         continue;
                     if (COUNT && ((am >> lane) & 1ull) && l0 + k < J.d)

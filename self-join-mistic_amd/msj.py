@@ -116,6 +116,7 @@ class BatchStats(C.Structure):
         ("ms_total", C.c_double),
         ("graph_replays", C.c_uint32),
         ("solo_jobs", C.c_uint32),
+        ("class_slab", C.c_uint32 * 8),
     ]
 
 
@@ -499,6 +500,7 @@ class Batch:
         stats = {f[0]: getattr(st, f[0]) for f in BatchStats._fields_}
         stats["class_ds"] = list(st.class_ds)[:st.classes]
         stats["class_epi_log2"] = list(st.class_epi_log2)[:st.classes]
+        stats["class_slab"] = list(st.class_slab)[:st.classes]
         return res, stats
 
     def close(self):

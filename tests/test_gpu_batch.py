@@ -114,6 +114,11 @@ def test_batch_solo_jobs_overlap_builds(msj_mod):
         for rnd in range(2):
             res, st = b.run()
             assert st["solo_jobs"] >= 1 and st["solo_jobs"] < len(datas)
+            # the tuned slab width of the wide-row class (class_ds 0: every
+            # padded width above 32), 0 for the others
+            for ds, sl in zip(st["class_ds"], st["class_slab"]):
+                assert sl in ((8, 16) if ds == 0 else (0,))
+            assert 0 in st["class_ds"]
             for x, e, r in zip(datas, eps, res):
                 _same(r, msj_mod.self_join(x, e))
 
