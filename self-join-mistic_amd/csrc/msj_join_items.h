@@ -225,17 +225,25 @@ __device__ inline void item_sym(const JobView &J, const JoinItem &w, double *qs,
 
 // High-dimensional rows (ds > 32).  Slab-major order: the wave holds the
 // 64 candidates' next SL dimensions (HSL, or the batch autotuner's choice)
-// in VGPRs (each loaded once) and
-// advances the partial sums of all of the item's queries (up to HQ, one FP64
-// sum per lane and query in VGPRs) through that slab, chunk by chunk in the
-// reference's order, with the same early-exit test; a query row slab is
-// staged in the wave's LDS slice and read as a broadcast.  Decisions are
-// those of the per-pair sequence: the chunks of every pair run in order,
-// each followed by the s > eps^2 test.
+// in VGPRs (each loaded once) and advances the partial sums of all of the
+// item's queries (up to HQ, one FP64 sum per lane and query in VGPRs)
+// through that slab, chunk by chunk in the reference's order; a query row
+// slab is staged in the wave's LDS slice and read as a broadcast.  The
+// undecided pairs of query q are a wave mask amq[q] (wave-uniform, SGPRs).
+//
+// Decisions are those of the per-pair sequence (each chunk followed by the
+// s > eps^2 test): partial sums never decrease, so a pair is accepted iff
+// its full sum passes the test, and any schedule of tests that ends with
+// the full sum decides the same.  The join tests after every second chunk;
+// chunks past ds (the last slab of a width that is not a multiple of SL)
+// add (0 - 0)^2 = 0 exactly, so a slab runs its chunks in pairs with no
+// per-chunk guard.  The counting build tests every chunk: its dimension
+// count follows the per-chunk exit.
 template <bool COUNT, int SL = HSL>
 __device__ inline void item_hd(const JobView &J, const JoinItem &w, double *qs, WaveHits &H,
                                uint64_t &dims)
 {
+    static_assert(SL % 8 == 0, "chunks run in pairs");
     const uint32_t lane = H.lane;
     const uint32_t ds = J.ds;
     const uint32_t X = w.x;
@@ -263,17 +271,21 @@ __device__ inline void item_hd(const JobView &J, const JoinItem &w, double *qs, 
             c = ystart + (ci - ybase);
         }
         const bool self_node = cvalid && Y == X;
-        uint32_t alive = 0;  // bit q: pair (query q, this lane's candidate) undecided
-        for (uint32_t q = 0; q < qn; ++q)
-            if (cvalid && (!self_node || q0 + q <= c))
-                alive |= 1u << q;
+        // bit `lane` of amq[q]: pair (query q, this lane's candidate) undecided
+        uint64_t amq[HQ];
         double sq[HQ];
 #pragma unroll
-        for (int q = 0; q < HQ; ++q)
+        for (int q = 0; q < HQ; ++q) {
+            amq[q] = wave_vote(cvalid && (uint32_t)q < qn && (!self_node || q0 + q <= c));
             sq[q] = 0.0;
+        }
         const double *crow = J.xr + (size_t)c * ds;
         for (uint32_t l0 = 0; l0 < ds; l0 += SL) {
-            if (!wave_vote(alive != 0))
+            uint64_t any = 0;
+#pragma unroll
+            for (int q = 0; q < HQ; ++q)
+                any |= amq[q];
+            if (!any)
                 break;
             // query slab -> LDS (qn rows of SL dims), candidate slab -> VGPRs
             wave_sync();
@@ -294,15 +306,13 @@ __device__ inline void item_hd(const JobView &J, const JoinItem &w, double *qs, 
                 }
             }
             wave_sync();
-            // fully unrolled over q (sq[] stays in VGPRs): no break or
-            // continue out of it; a decided query or chunk is skipped by its vote
+            // fully unrolled over q (sq[] in VGPRs, amq[] in SGPRs): no
+            // break or continue out of it; a decided query is skipped by its mask
 #pragma unroll
             for (int q = 0; q < HQ; ++q) {
-                // undecided lanes of query q as a wave mask (0 for q >= qn)
-                uint64_t am = wave_vote((alive >> q) & 1u);
+                uint64_t am = amq[q];
                 if (!am)
                     continue;
-                const uint64_t am0 = am;
                 // the query's slab in one batch of broadcast reads (one LDS
                 // round trip per query and slab, not per chunk)
                 const double *qrow = qs + q * SL;  // wave-uniform LDS address
@@ -311,12 +321,7 @@ __device__ inline void item_hd(const JobView &J, const JoinItem &w, double *qs, 
                 for (int j = 0; j < SL / 2; ++j)
                     qv[j] = *(const double2 *)(qrow + 2 * j);
                 double s = sq[q];
-#pragma unroll
-                for (int k = 0; k < SL; k += 4) {
-                    if (l0 + k >= ds || !am)
-                        continue;
-                    if (COUNT && ((am >> lane) & 1ull) && l0 + k < J.d)
-                        dims += min(4u, J.d - (l0 + k));
+                auto chunk = [&](int k) __attribute__((always_inline)) {
                     const double2 qa = qv[k / 2];
                     const double2 qb = qv[k / 2 + 1];
                     double t0 = qa.x - cc[k];
@@ -330,19 +335,38 @@ __device__ inline void item_hd(const JobView &J, const JoinItem &w, double *qs, 
                     double t3 = qb.y - cc[k + 3];
                     double p3 = t3 * t3;
                     s = s + p3;
-                    // partial sums only grow: testing every second chunk
-                    // (and at the end of the row) decides every pair as the
-                    // per-chunk test does; the counting build tests every
-                    // chunk, its dimension count follows the per-chunk exit
-                    if (COUNT || (k & 4) || l0 + k + 4 >= ds)
+                };
+                if constexpr (COUNT) {
+#pragma unroll
+                    for (int k = 0; k < SL; k += 4) {
+                        if (l0 + k >= ds || !am)
+                            continue;
+                        if (((am >> lane) & 1ull) && l0 + k < J.d)
+                            dims += min(4u, J.d - (l0 + k));
+                        chunk(k);
                         am &= wave_vote(!(s > J.eps2));
+                    }
+                } else {
+#pragma unroll
+                    for (int k = 0; k < SL; k += 8) {
+                        // wave-uniform: the pair of chunks starts inside the
+                        // row and some pair of query q is undecided
+                        if (k == 0 || (am && l0 + k < ds)) {
+                            chunk(k);
+                            chunk(k + 4);
+                            am &= wave_vote(!(s > J.eps2));
+                        }
+                    }
                 }
                 sq[q] = s;
-                if (am != am0 && !((am >> lane) & 1ull))
-                    alive &= ~(1u << q);
+                amq[q] = am;
             }
         }
         // the pairs still undecided after every slab are results
+        uint32_t alive = 0;  // bit q: pair (query q, this lane's candidate)
+#pragma unroll
+        for (int q = 0; q < HQ; ++q)
+            alive |= (uint32_t)((amq[q] >> lane) & 1ull) << q;
         const unsigned long long corig = J.perm[c];
         for (uint32_t q = 0; q < qn; ++q) {
             const bool acc = (alive >> q) & 1u;
