@@ -385,7 +385,7 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
     constexpr uint32_t NW = HALF ? 4 : 8;         // waves per workgroup
     constexpr uint32_t BHITS = 64;                // per-wave band staging
     static_assert(UNITS % NW == 0, "every wave issues the same number of copies");
-    static_assert(!REF || (SPLIT && !HALF), "BF16-first refinement: split full tiles only");
+    static_assert(!REF || SPLIT, "BF16-first refinement: split-BF16 only");
     // two slab buffers: the next slab is copied (global -> LDS, no VGPRs)
     // while the matrix cores work on the current one
     __shared__ uint4 sA[2][NP][SLA];
@@ -679,7 +679,20 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
                     const uint32_t va = ldsA + buf * (NP * SLA * 16) + kk * 1024;
                     const uint32_t vb = ldsB + buf * (NP * SLB * 16) + kk * 1024;
                     uint4 a0, a1, a2, a3, b0, b1, l0, l1, l2, l3, m0, m1;
-                    if (SPLIT && HALF) {  // one k-step per slab: blocks 1 KiB apart
+                    if (SPLIT && HALF && PH == 1) {  // high plane, blocks 1 KiB apart
+                        asm volatile(
+                            "ds_read_b128 %0, %6\n"
+                            "ds_read_b128 %1, %6 offset:1024\n"
+                            "ds_read_b128 %2, %6 offset:2048\n"
+                            "ds_read_b128 %3, %6 offset:3072\n"
+                            "ds_read_b128 %4, %7\n"
+                            "ds_read_b128 %5, %7 offset:1024\n"
+                            "s_waitcnt lgkmcnt(0)"
+                            : "=&v"(a0), "=&v"(a1), "=&v"(a2), "=&v"(a3), "=&v"(b0), "=&v"(b1)
+                            : "v"(va), "v"(vb)
+                            : "memory");
+                        l0 = l1 = l2 = l3 = m0 = m1 = make_uint4(0, 0, 0, 0);
+                    } else if (SPLIT && HALF) {  // one k-step per slab: blocks 1 KiB apart
                         asm volatile(
                             "ds_read_b128 %0, %12\n"
                             "ds_read_b128 %1, %12 offset:1024\n"
@@ -1409,7 +1422,7 @@ void MfmaRunner::prepare(const Index &ix, const JoinRunner &J, const double *xr,
     thr.ensure(Ptot);
     // split-BF16 full tiles score every tile in BF16 first (refine_tiles):
     // they also need the rows' terms under the BF16 bound
-    const bool refine = split && KS > KS_HALF;
+    const bool refine = split;
     if (refine)
         thrb.ensure(Ptot);
     const double Cb = bound_c(d, false);
@@ -1513,7 +1526,7 @@ uint64_t MfmaRunner::run(const Index &ix, JoinRunner &J, const double *xr, uint3
 #define MSJ_MJ(SP, DG)                                                                          \
     do {                                                                                        \
         if (half)                                                                               \
-            MSJ_MJH(SP, DG, true, false, 256);                                                  \
+            MSJ_MJH(SP, DG, true, SP, 256);                                                  \
         else                                                                                    \
             MSJ_MJH(SP, DG, false, SP, 512);                                                    \
     } while (0)
@@ -1583,7 +1596,7 @@ uint64_t MfmaRunner::run(const Index &ix, JoinRunner &J, const double *xr, uint3
             // products on the 128 x 64 wave blocks (8 per tile) that needed
             // them (h[8])
             const double tile_flops = (double)TM * TM * 2.0 * (16.0 * KS);
-            const bool refine = split && !half;
+            const bool refine = split;
             st->mfma_flops = !split ? tiles * tile_flops
                              : refine ? tiles * tile_flops + (double)h[8] * tile_flops / 4.0
                                       : tiles * tile_flops * 3.0;
