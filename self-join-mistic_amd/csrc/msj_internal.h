@@ -407,7 +407,7 @@ struct MfmaRunner {
     DevBuf<uint32_t> porig;      // [P]   padded row -> caller's row id
     DevBuf<uint4> fh, fl;        // fragments of the BF16 high / low parts
     DevBuf<float4> thr;          // [P]   per-row threshold terms
-    DevBuf<float2> thrb;         // [P]   the same under the BF16 bound (split full tiles)
+    DevBuf<float2> thrb;         // [P]   the same under the BF16 bound (split-BF16)
     DevBuf<double> mu, colpart;  // centre (column means) and partial sums
     DevBuf<uint64_t> p64;        // scan scratch
     DevBuf<uint32_t> cnt8;       // [8][E] super-blocks per (class, upper edge)

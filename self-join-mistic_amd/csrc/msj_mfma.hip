@@ -387,9 +387,14 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
     static_assert(UNITS % NW == 0, "every wave issues the same number of copies");
     static_assert(!REF || SPLIT, "BF16-first refinement: split-BF16 only");
     // two slab buffers: the next slab is copied (global -> LDS, no VGPRs)
-    // while the matrix cores work on the current one
-    __shared__ uint4 sA[2][NP][SLA];
-    __shared__ uint4 sB[2][NP][SLB];
+    // while the matrix cores work on the current one.  One region, A blocks
+    // then B blocks; the cross-product pass of split half tiles uses it as
+    // two halves of compacted slabs (CUNITS 1 KB units each).
+    constexpr uint32_t CUNITS = NP * (SLA + SLB) / WAVE;
+    __shared__ uint4 slab[2 * NP * (SLA + SLB)];
+    auto &sA = *reinterpret_cast<uint4(*)[2][NP][SLA]>(&slab[0]);
+    auto &sB = *reinterpret_cast<uint4(*)[2][NP][SLB]>(&slab[2 * NP * SLA]);
+    __shared__ uint32_t s_need[NW];  // split half tiles: per wave, the blocks to refine
     __shared__ float4 tA[TM], tB[TN];
     __shared__ __align__(16) float tAy[TM];  // tA[.].y, four rows per 16-byte read
     // REF: the rows' reject terms under the BF16 bound (first pass)
@@ -776,6 +781,89 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
             }
 
         };
+        // Cross products of split half tiles, block by block: only the 32 x
+        // 32 blocks in the wave's mask wm take them, and the workgroup copies
+        // only the row blocks that some wave's mask needs (an: A blocks, bn:
+        // B blocks), both planes, into compacted slabs of KP k-steps: unit
+        // (k-step kk, r) of a slab holds A block r / 2 of an's list (r <
+        // 2 nA) or B block (r - 2 nA) / 2 of bn's, plane r % 2.  The two
+        // halves of the slab region alternate; the copies of the next
+        // compacted slab run during the current one's MFMAs.
+        auto xpass = [&](uint32_t an, uint32_t bn, uint32_t wm) __attribute__((always_inline)) {
+            const uint32_t nA = __popc(an), nB = __popc(bn);
+            const uint32_t U2 = 2 * (nA + nB);                  // units per k-step
+            const uint32_t KP = std::min<uint32_t>(CUNITS / U2, KS);  // >= 1: U2 <= 24
+            const uint32_t ldsS = (uint32_t)(uintptr_t)(
+                (__attribute__((address_space(3))) uint4 *)&slab[lane]);
+            auto nth = [](uint32_t msk, uint32_t k) {  // index of the k-th set bit
+                for (uint32_t i = 0; i < k; ++i)
+                    msk &= msk - 1u;
+                return (uint32_t)__builtin_ctz(msk);
+            };
+            auto ccopy = [&](uint32_t k0, uint32_t nk, uint32_t hf) {
+                for (uint32_t u = wv; u < nk * U2; u += NW) {
+                    const uint32_t kk = u / U2, r = u % U2;
+                    const bool isb = r >= 2 * nA;
+                    const uint32_t rb = isb ? rbB + nth(bn, (r - 2 * nA) >> 1) : rbA + nth(an, r >> 1);
+                    const uint4 *src = ((r & 1u) ? fl : fh) + ((size_t)rb * KS + k0 + kk) * WAVE + lane;
+                    uint4 *dst = &slab[(hf * CUNITS + u) * WAVE];
+                    __builtin_amdgcn_global_load_lds((const void *)src,
+                                                     (void __attribute__((address_space(3))) *)dst,
+                                                     16, 0, 0);
+                }
+            };
+            // every wave is past its reads of the first pass's slabs (the
+            // barrier that published the masks), and no copy is in flight
+            ccopy(0, KP, 0);
+            for (uint32_t k0 = 0, hf = 0; k0 < KS; k0 += KP, hf ^= 1u) {
+                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+                __syncthreads();
+                if (k0 + KP < KS)
+                    ccopy(k0 + KP, std::min<uint32_t>(KP, KS - k0 - KP), hf ^ 1u);
+                const uint32_t nk = std::min<uint32_t>(KP, KS - k0);
+                for (uint32_t kk = 0; kk < nk; ++kk) {
+                    const uint32_t kb = hf * CUNITS + kk * U2;
+#pragma unroll
+                    for (int bi = 0; bi < 4; ++bi) {
+                        if (((wm >> (2 * bi)) & 3u) == 0)
+                            continue;
+                        const uint32_t ia = __popc(an & ((1u << (4 * wr + bi)) - 1u));
+                        const uint32_t va = ldsS + ((kb + 2 * ia) << 10);
+                        uint4 ah, al;
+                        asm volatile(
+                            "ds_read_b128 %0, %2\n"
+                            "ds_read_b128 %1, %2 offset:1024\n"
+                            "s_waitcnt lgkmcnt(0)"
+                            : "=&v"(ah), "=&v"(al)
+                            : "v"(va)
+                            : "memory");
+#pragma unroll
+                        for (int bj = 0; bj < 2; ++bj) {
+                            if (((wm >> (2 * bi + bj)) & 1u) == 0)
+                                continue;
+                            const uint32_t ib = __popc(bn & ((1u << (2 * wc + bj)) - 1u));
+                            const uint32_t vb = ldsS + ((kb + 2 * nA + 2 * ib) << 10);
+                            uint4 bh, bl;
+                            asm volatile(
+                                "ds_read_b128 %0, %2\n"
+                                "ds_read_b128 %1, %2 offset:1024\n"
+                                "s_waitcnt lgkmcnt(0)"
+                                : "=&v"(bh), "=&v"(bl)
+                                : "v"(vb)
+                                : "memory");
+                            acc[bi][bj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                                as_bf16x8(ah), as_bf16x8(bl), acc[bi][bj], 0, 0, 0);
+                            acc[bi][bj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                                as_bf16x8(al), as_bf16x8(bh), acc[bi][bj], 0, 0, 0);
+                        }
+                    }
+                }
+            }
+            // the compacted slabs span both buffers: every wave is done
+            // reading them before the next tile's first copy
+            __syncthreads();
+        };
+        uint32_t wmask = 0xFFu;  // split half tiles: the blocks that were refined
         bool refined = !REF;
         if constexpr (!REF) {
             kpass(std::integral_constant<int, 0>());
@@ -789,20 +877,44 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
 #pragma unroll
             for (int bj = 0; bj < 2; ++bj)
                 tbb[bj] = tByb[64 * wc + 32 * bj + (lane & 31)];
+            // Per 32 x 32 block (bit 2 bi + bj) the smallest reject score of
+            // the BF16 pass: the lane's column c takes the smallest
+            // RN(alpha'_b,p - 2 G) of its 16 rows, then adds alpha'_b,c (RN
+            // is monotone: no pair of the column scores less; the bound of
+            // these terms allows the rounding of alpha'_b,p - 2 G).  A block
+            // whose smallest score exceeds hi holds certain rejects only
+            // (half tiles refine the other blocks one by one; full tiles the
+            // wave's whole 128 x 64 block).  Scores of padding rows and of
+            // rows outside the item are +inf.
             float m = INF;
+            uint32_t nmask = 0;
+            auto min3 = [](float a, float b, float c) { return fminf(fminf(a, b), c); };
 #pragma unroll
-            for (int bi = 0; bi < 4; ++bi)
+            for (int bi = 0; bi < 4; ++bi) {
+                float s[2][16];
 #pragma unroll
                 for (int eg = 0; eg < 4; ++eg) {
-                    const uint32_t i0 = 128 * wr + 32 * bi + 8 * eg + 4 * h;
-                    const float4 ay4 = *(const float4 *)&tAyb[i0];
+                    const float4 ay4 = *(const float4 *)&tAyb[128 * wr + 32 * bi + 8 * eg + 4 * h];
                     const float ay[4] = {ay4.x, ay4.y, ay4.z, ay4.w};
 #pragma unroll
                     for (int ei = 0; ei < 4; ++ei)
 #pragma unroll
                         for (int bj = 0; bj < 2; ++bj)
-                            m = fminf(m, fmaf(-2.f, acc[bi][bj][4 * eg + ei], ay[ei] + tbb[bj]));
+                            s[bj][4 * eg + ei] = fmaf(-2.f, acc[bi][bj][4 * eg + ei], ay[ei]);
                 }
+#pragma unroll
+                for (int bj = 0; bj < 2; ++bj) {
+                    const float *q = s[bj];
+                    const float x0 = min3(q[0], q[1], q[2]), x1 = min3(q[3], q[4], q[5]);
+                    const float x2 = min3(q[6], q[7], q[8]), x3 = min3(q[9], q[10], q[11]);
+                    const float x4 = min3(q[12], q[13], q[14]);
+                    const float sc = fminf(min3(x0, x1, x2), min3(x3, x4, q[15])) + tbb[bj];
+                    if (HALF)
+                        nmask |= (__builtin_amdgcn_ballot_w64(!(sc > hi)) != 0 ? 1u : 0u)
+                                 << (2 * bi + bj);
+                    m = fminf(m, sc);
+                }
+            }
             if constexpr (DIAG) {
                 // sampled: every pair the BF16 score calls a certain reject
                 // goes to the FP64 check as class 3 before the cross products
@@ -842,9 +954,37 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
             }
             const bool need = sampled || tile_nan || !(m > hi);
             wneed = __builtin_amdgcn_ballot_w64(need) != 0;
-            if (__syncthreads_or(need) != 0) {
-                if (wneed && lane == 0)
-                    atomicAdd(&ctr[8], 1ull);  // wave blocks that take the cross products
+            if constexpr (HALF) {
+                // the 32 x 32 blocks to refine (all of a sampled or NaN-scored
+                // tile); the workgroup copies the row blocks they need
+                const uint32_t wm = (sampled || tile_nan) ? 0xFFu : nmask;
+                wmask = wm;
+                wneed = wm != 0;
+                if (lane == 0)
+                    s_need[wv] = wm;
+                __syncthreads();
+                uint32_t an = 0, bn = 0;  // A row blocks (8), B row blocks (4)
+#pragma unroll
+                for (uint32_t q = 0; q < NW; ++q) {
+                    const uint32_t x = __builtin_amdgcn_readfirstlane(s_need[q]);
+#pragma unroll
+                    for (uint32_t bi = 0; bi < 4; ++bi)
+                        if ((x >> (2 * bi)) & 3u)
+                            an |= 1u << (4 * (q >> 1) + bi);
+#pragma unroll
+                    for (uint32_t bj = 0; bj < 2; ++bj)
+                        if (x & (0x55u << bj))
+                            bn |= 1u << (2 * (q & 1u) + bj);
+                }
+                if (an != 0) {
+                    if (wm != 0 && lane == 0)  // 32 x 32 blocks that take the cross products
+                        atomicAdd(&ctr[8], (unsigned long long)__popc(wm));
+                    xpass(an, bn, wm);
+                    refined = true;
+                }
+            } else if (__syncthreads_or(need) != 0) {
+                if (wneed && lane == 0)  // 32 x 32 blocks that take the cross products
+                    atomicAdd(&ctr[8], 8ull);
                 // both planes again from the first slab (buffer buf is free:
                 // the last slab used the other one)
                 copy_slab(rbA, rbB, 0, buf, NP);
@@ -920,6 +1060,11 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
                 // read out of the matrix registers together.
 #pragma unroll
                 for (int bi = 0; bi < 4; ++bi) {
+                    // split (REF): only the refined blocks; the others hold
+                    // certain rejects of the BF16 pass
+                    const uint32_t bsel = REF ? (wmask >> (2 * bi)) & 3u : 3u;
+                    if (bsel == 0)
+                        continue;
                     // fast pass: the block's smallest reject score (in a diagonal tile,
                     // of pairs p <= c); a block whose smallest score exceeds hi holds
                     // certain rejects only.  Scores of padding rows and of rows
@@ -939,7 +1084,8 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
                                 for (int bj = 0; bj < (int)NB; ++bj) {
                                     const float g = acc[bi][bj][4 * eg + ei];
                                     float sc = fmaf(-2.f, g, ay[ei] + tb[bj].y);
-                                    if (DT && i0 + ei > c0 + 32 * bj + (lane & 31) + doff)
+                                    if ((DT && i0 + ei > c0 + 32 * bj + (lane & 31) + doff) ||
+                                        ((bsel >> bj) & 1u) == 0)
                                         sc = INF;
                                     m = fminf(m, sc);
                                 }
@@ -964,7 +1110,8 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
 #pragma unroll
                         for (int bj = 0; bj < (int)NB; ++bj) {
                             const uint32_t c = cb + c0 + 32 * bj + (lane & 31);
-                            const bool valid = pv & cvalid[bj] & (!diag_tile | (p <= c));
+                            const bool valid = pv & cvalid[bj] & (!diag_tile | (p <= c)) &
+                                               (((bsel >> bj) & 1u) != 0);
                             const float G = acc[bi][bj][e];
                             const bool a_ok = valid & (fmaf(-2.f, G, ta.x + tb[bj].x) <= lo);
                             const bool r_ok = fmaf(-2.f, G, ta.y + tb[bj].y) > hi;
@@ -1339,8 +1486,12 @@ __global__ void k_zero_ctr(unsigned long long *c, uint32_t n)
         MSJ_HIP(hipGetLastError());                                                      \
     } while (0)
 
-// Bound constant C of the threshold terms (see the file comment).
-double bound_c(uint32_t d, bool split)
+// Bound constant C of the threshold terms (see the file comment).  rnd:
+// the rounding of the sum before the final one, in units of 2^-24
+// (|a'|^2 + |b'|^2): 1 for RN(alpha_p + alpha_c); the BF16 pass of the
+// split form rounds alpha'_b,p - 2 G instead, |alpha'_b,p - 2 G| <= 2.01
+// (|a'|^2 + |b'|^2), and takes 3.
+double bound_c(uint32_t d, bool split, double rnd = 1.0)
 {
     const double u = 0x1p-8 * (1.0 + 0x1p-15);
     const double ep = split ? (3 * u * u + 2 * u * u * u + 2 * u * u * u * u) : (2 * u + u * u);
@@ -1348,7 +1499,7 @@ double bound_c(uint32_t d, bool split)
     const double t = 2.0 * m * 0x1p-24;
     const double gam = t / (1.0 - t);
     const double eg = ep + gam * (1 + u) * (1 + u) * (1 + 2 * u);
-    return 1.01 * (eg + 0x1p-24) + 0x1p-40;
+    return 1.01 * (eg + rnd * 0x1p-24) + 0x1p-40;
 }
 
 float round_down_f(double v)
@@ -1420,12 +1571,12 @@ void MfmaRunner::prepare(const Index &ix, const JoinRunner &J, const double *xr,
     if (split)
         fl.ensure(nfrag);
     thr.ensure(Ptot);
-    // split-BF16 full tiles score every tile in BF16 first (refine_tiles):
-    // they also need the rows' terms under the BF16 bound
+    // split-BF16 scores every tile in BF16 first: it also needs the rows'
+    // terms under the BF16 bound
     const bool refine = split;
     if (refine)
         thrb.ensure(Ptot);
-    const double Cb = bound_c(d, false);
+    const double Cb = bound_c(d, false, 3.0);
     if (split)
         LAUNCH(k_frag_prep<true>, grid_for(Ptot, 4), 256, xr, ds, KS, mu.p, pjoin.p, Ptot, C,
                fh.p, fl.p, thr.p, Cb, refine ? thrb.p : nullptr);
@@ -1592,18 +1743,17 @@ uint64_t MfmaRunner::run(const Index &ix, JoinRunner &J, const double *xr, uint3
             st->filter_violations = h[4];
             st->filter_checked_bf16 = h[9];
             st->form_used = split ? MSJ_FORM_MFMA_SPLIT : MSJ_FORM_MFMA_BF16;
-            // split full tiles: BF16 products on every tile, the cross
-            // products on the 128 x 64 wave blocks (8 per tile) that needed
-            // them (h[8])
+            // split: BF16 products on every tile, the two cross products on
+            // the 32 x 32 blocks (64 per tile) that needed them (h[8])
             const double tile_flops = (double)TM * TM * 2.0 * (16.0 * KS);
             const bool refine = split;
             st->mfma_flops = !split ? tiles * tile_flops
-                             : refine ? tiles * tile_flops + (double)h[8] * tile_flops / 4.0
+                             : refine ? tiles * tile_flops + (double)h[8] * tile_flops / 32.0
                                       : tiles * tile_flops * 3.0;
             st->split_share_ppm =
                 !split ? 0u
                 : refine ? (uint32_t)std::min<uint64_t>(
-                               1000000ull, tiles ? h[8] * 1000000ull / (8ull * tiles) : 0)
+                               1000000ull, tiles ? h[8] * 1000000ull / (64ull * tiles) : 0)
                          : 1000000u;
             return R;
         }
