@@ -786,27 +786,25 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
         // only the row blocks that some wave's mask needs (an: A blocks, bn:
         // B blocks), both planes, into compacted slabs of KP k-steps: unit
         // (k-step kk, r) of a slab holds A block r / 2 of an's list (r <
-        // 2 nA) or B block (r - 2 nA) / 2 of bn's, plane r % 2.  The two
-        // halves of the slab region alternate; the copies of the next
-        // compacted slab run during the current one's MFMAs.
+        // 2 nA) or B block (r - 2 nA) / 2 of bn's, plane r % 2.  A slab
+        // fills as much of the region (2 CUNITS units) as it can: each is
+        // one round trip to memory, and the MFMAs between them are few.
         auto xpass = [&](uint32_t an, uint32_t bn, uint32_t wm) __attribute__((always_inline)) {
             const uint32_t nA = __popc(an), nB = __popc(bn);
-            const uint32_t U2 = 2 * (nA + nB);                  // units per k-step
-            const uint32_t KP = std::min<uint32_t>(CUNITS / U2, KS);  // >= 1: U2 <= 24
-            const uint32_t ldsS = (uint32_t)(uintptr_t)(
-                (__attribute__((address_space(3))) uint4 *)&slab[lane]);
+            const uint32_t U2 = 2 * (nA + nB);                      // units per k-step
+            const uint32_t KP = std::min<uint32_t>(2 * CUNITS / U2, KS);  // >= 2: U2 <= 24
             auto nth = [](uint32_t msk, uint32_t k) {  // index of the k-th set bit
                 for (uint32_t i = 0; i < k; ++i)
                     msk &= msk - 1u;
                 return (uint32_t)__builtin_ctz(msk);
             };
-            auto ccopy = [&](uint32_t k0, uint32_t nk, uint32_t hf) {
+            auto ccopy = [&](uint32_t k0, uint32_t nk) {
                 for (uint32_t u = wv; u < nk * U2; u += NW) {
                     const uint32_t kk = u / U2, r = u % U2;
                     const bool isb = r >= 2 * nA;
                     const uint32_t rb = isb ? rbB + nth(bn, (r - 2 * nA) >> 1) : rbA + nth(an, r >> 1);
                     const uint4 *src = ((r & 1u) ? fl : fh) + ((size_t)rb * KS + k0 + kk) * WAVE + lane;
-                    uint4 *dst = &slab[(hf * CUNITS + u) * WAVE];
+                    uint4 *dst = &slab[u * WAVE];
                     __builtin_amdgcn_global_load_lds((const void *)src,
                                                      (void __attribute__((address_space(3))) *)dst,
                                                      16, 0, 0);
@@ -814,48 +812,47 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
             };
             // every wave is past its reads of the first pass's slabs (the
             // barrier that published the masks), and no copy is in flight
-            ccopy(0, KP, 0);
-            for (uint32_t k0 = 0, hf = 0; k0 < KS; k0 += KP, hf ^= 1u) {
+            for (uint32_t k0 = 0; k0 < KS; k0 += KP) {
+                const uint32_t nk = std::min<uint32_t>(KP, KS - k0);
+                if (k0 != 0)
+                    __syncthreads();  // every wave is done with the previous slab
+                ccopy(k0, nk);
                 asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
                 __syncthreads();
-                if (k0 + KP < KS)
-                    ccopy(k0 + KP, std::min<uint32_t>(KP, KS - k0 - KP), hf ^ 1u);
-                const uint32_t nk = std::min<uint32_t>(KP, KS - k0);
                 for (uint32_t kk = 0; kk < nk; ++kk) {
-                    const uint32_t kb = hf * CUNITS + kk * U2;
+                    const uint32_t kb = kk * U2;
+                    // the wave's B fragments (both planes) once per k-step,
+                    // then per 32-row block its A pair and the two cross
+                    // products of its blocks, independent MFMAs adjacent
+                    uint4 bh[2], bl[2];
+#pragma unroll
+                    for (int bj = 0; bj < 2; ++bj) {
+                        bh[bj] = bl[bj] = make_uint4(0, 0, 0, 0);
+                        if ((wm & (0x55u << bj)) == 0)
+                            continue;
+                        const uint32_t ib = __popc(bn & ((1u << (2 * wc + bj)) - 1u));
+                        const uint32_t ub = kb + 2 * nA + 2 * ib;
+                        bh[bj] = slab[ub * WAVE + lane];
+                        bl[bj] = slab[(ub + 1) * WAVE + lane];
+                    }
 #pragma unroll
                     for (int bi = 0; bi < 4; ++bi) {
-                        if (((wm >> (2 * bi)) & 3u) == 0)
+                        const uint32_t sel = (wm >> (2 * bi)) & 3u;
+                        if (sel == 0)
                             continue;
                         const uint32_t ia = __popc(an & ((1u << (4 * wr + bi)) - 1u));
-                        const uint32_t va = ldsS + ((kb + 2 * ia) << 10);
-                        uint4 ah, al;
-                        asm volatile(
-                            "ds_read_b128 %0, %2\n"
-                            "ds_read_b128 %1, %2 offset:1024\n"
-                            "s_waitcnt lgkmcnt(0)"
-                            : "=&v"(ah), "=&v"(al)
-                            : "v"(va)
-                            : "memory");
+                        const uint4 ah = slab[(kb + 2 * ia) * WAVE + lane];
+                        const uint4 al = slab[(kb + 2 * ia + 1) * WAVE + lane];
 #pragma unroll
-                        for (int bj = 0; bj < 2; ++bj) {
-                            if (((wm >> (2 * bi + bj)) & 1u) == 0)
-                                continue;
-                            const uint32_t ib = __popc(bn & ((1u << (2 * wc + bj)) - 1u));
-                            const uint32_t vb = ldsS + ((kb + 2 * nA + 2 * ib) << 10);
-                            uint4 bh, bl;
-                            asm volatile(
-                                "ds_read_b128 %0, %2\n"
-                                "ds_read_b128 %1, %2 offset:1024\n"
-                                "s_waitcnt lgkmcnt(0)"
-                                : "=&v"(bh), "=&v"(bl)
-                                : "v"(vb)
-                                : "memory");
-                            acc[bi][bj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-                                as_bf16x8(ah), as_bf16x8(bl), acc[bi][bj], 0, 0, 0);
-                            acc[bi][bj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-                                as_bf16x8(al), as_bf16x8(bh), acc[bi][bj], 0, 0, 0);
-                        }
+                        for (int bj = 0; bj < 2; ++bj)
+                            if ((sel >> bj) & 1u)
+                                acc[bi][bj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                                    as_bf16x8(ah), as_bf16x8(bl[bj]), acc[bi][bj], 0, 0, 0);
+#pragma unroll
+                        for (int bj = 0; bj < 2; ++bj)
+                            if ((sel >> bj) & 1u)
+                                acc[bi][bj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                                    as_bf16x8(al), as_bf16x8(bh[bj]), acc[bi][bj], 0, 0, 0);
                     }
                 }
             }
