@@ -46,6 +46,12 @@ namespace msj {
 namespace {
 
 constexpr uint64_t EVALS_PER_ITEM = 262144;  // lane evaluations per work item (default)
+// ds > 32: 16 queries x 1,024 candidates.  The query groups of a candidate
+// chunk are consecutive items, dealt out four to a workgroup, so the
+// workgroups on one XCD that share a chunk (1,024 rows, 736 KB at d = 90)
+// run side by side and find it in their L2 (515,345 x 90: 945 -> 808 ms
+// with chunks of 1,024 instead of 16,384 rows).
+constexpr uint64_t EVALS_PER_ITEM_HD = 16384;
 
 __device__ inline double wave_min(double v)
 {
@@ -712,7 +718,8 @@ void JoinRunner::plan(const Index &ix, uint32_t n, uint32_t d, hipStream_t s, bo
     const uint32_t M = ix.M;
     const uint32_t ds = padded_dim(d);
     const uint32_t qmax = item_queries(ds);
-    const uint64_t epi = evals_per_item ? evals_per_item : EVALS_PER_ITEM;
+    const uint64_t epi =
+        evals_per_item ? evals_per_item : (ds > 32 ? EVALS_PER_ITEM_HD : EVALS_PER_ITEM);
     itcnt.ensure(M);
     itoff.ensure((size_t)M + 1);
     ulen.ensure(M);

@@ -54,14 +54,16 @@
 // the next slab's copy is in flight while the current one runs on the
 // matrix cores.
 //
-// Split-BF16 full tiles are scored in BF16 first (high planes only, one
-// third of the products): a tile whose pairs are all certain rejects under
-// the BF16 bound (C of a BF16 accumulation of d products) is done; any
-// other tile (a pair that is not, a sampled item, NaN terms) adds the two
-// cross products over its slabs and is classified under the split-BF16
-// bound, as without this step.  The split bound covers the reordered
-// accumulation (any order of the FP32 additions).  On clustered data most
-// tiles pair rows of different clusters and stop after the BF16 pass.
+// Split-BF16 tiles are scored in BF16 first (high planes only, one third of
+// the products): a 32 x 32 block whose pairs are all certain rejects under
+// the BF16 bound (C of a BF16 accumulation of d products) is done.  Any
+// other block (a pair that is not, a sampled item, NaN terms) adds the two
+// cross products and is classified under the split-BF16 bound, as without
+// this step: half tiles refine just those blocks, from slabs holding only
+// the row blocks they need; full tiles refine the whole 128 x 64 block of
+// a wave that has one.  The split bound covers the reordered accumulation
+// (any order of the FP32 additions).  On this data most blocks pair rows
+// that are far apart and stop after the BF16 pass.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -341,17 +343,18 @@ __global__ __launch_bounds__(256) void k_sblk_fill(const uint32_t *__restrict__ 
 // (count_work): class 1, a certain accept (every one is verified); class 2,
 // a certain reject of a sampled work item (verified); class 3 (REF), a pair
 // of a sampled work item that the first, BF16 pass scores as a certain
-// reject under the BF16 bound -- the test that lets a wave block skip the
-// cross products (verified).
+// reject under the BF16 bound -- the test that lets a block skip the cross
+// products (verified).
 //
 // Tile.  A workgroup (NW waves) computes a TM x TN tile (256 x 256, or a
-// 256 x 128 half tile with HALF; REF: split full tiles, BF16 pass first); wave (wr, wc) the 128 x 64 block of rows
-// 128 wr.. and columns 64 wc.., as 4 x 2 MFMA 32x32 tiles (128 FP32
-// accumulators per lane).  Per slab of KL k-steps the workgroup copies the
-// row blocks of fragments (8 of A, TN / 32 of B; both planes when split)
-// from global memory straight into LDS, and each wave reads its 4 + 2
-// fragments per k-step from there; the next slab's copy runs into the other
-// LDS buffer while the matrix cores work on the current one.
+// 256 x 128 half tile with HALF; REF: split-BF16, BF16 pass first); wave
+// (wr, wc) the 128 x 64 block of rows 128 wr.. and columns 64 wc.., as 4 x 2
+// MFMA 32x32 tiles (128 FP32 accumulators per lane).  Per slab of KL
+// k-steps the workgroup copies the row blocks of fragments (8 of A, TN / 32
+// of B; the high plane) from global memory straight into LDS, and each wave
+// reads its 4 + 2 fragments per k-step from there; the next slab's copy
+// runs into the other LDS buffer while the matrix cores work on the current
+// one.
 //
 // Epilogue.  The A and B rows' terms are staged per tile; rows outside the
 // item (padding, the next node) get +inf terms, so the fast pass -- one add,
