@@ -46,12 +46,13 @@ namespace msj {
 namespace {
 
 constexpr uint64_t EVALS_PER_ITEM = 262144;  // lane evaluations per work item (default)
-// ds > 32: 16 queries x 1,024 candidates.  The query groups of a candidate
-// chunk are consecutive items, dealt out four to a workgroup, so the
-// workgroups on one XCD that share a chunk (1,024 rows, 736 KB at d = 90)
-// run side by side and find it in their L2 (515,345 x 90: 945 -> 808 ms
-// with chunks of 1,024 instead of 16,384 rows).
-constexpr uint64_t EVALS_PER_ITEM_HD = 16384;
+// ds > 32: HQ (24) queries x 1,024 candidates.  The query groups of a
+// candidate chunk are consecutive items, dealt out four to a workgroup, so
+// the workgroups on one XCD that share a chunk (1,024 rows, 736 KB at d =
+// 90) run side by side and find it in their L2 (515,345 x 90: 945 -> 808 ms
+// with chunks of 1,024 instead of 16,384 rows, 788 ms with 24 queries per
+// item instead of 16).
+constexpr uint64_t EVALS_PER_ITEM_HD = 1024ull * HQ;
 
 __device__ inline double wave_min(double v)
 {

@@ -32,7 +32,7 @@ constexpr uint32_t WPB = 4;                // waves per block
 constexpr uint32_t LDS_WAVE_BYTES = 8192;  // staged query rows per wave
 constexpr uint32_t HITS = 256;             // per-wave LDS buffer of result keys
 constexpr int HSL = 16;                    // high-d kernel: dimensions per slab
-constexpr int HQ = 16;                     // high-d kernel: queries per item
+constexpr int HQ = 24;                     // high-d kernel: queries per item
 
 // Queries per work item for a row width DS (in-register dimensions): as many
 // FP64 rows as fit LDS_WAVE_BYTES, at most 64.
