@@ -1057,7 +1057,11 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
                 constexpr Mask HALF_COL = 0x5555u;
 
                 // One 32-row block (bi) at a time, so that only its accumulators are
-                // read out of the matrix registers together.
+                // read out of the matrix registers together: the classification
+                // into lane masks here (unrolled, constant register indices), the
+                // rare band entries and the certain accepts in the loop after it
+                // (not unrolled: one copy of that code, fewer scalar registers)
+                Mask am4[4] = {0u, 0u, 0u, 0u}, bm4[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
                 for (int bi = 0; bi < 4; ++bi) {
                     // split (REF): only the refined blocks; the others hold
@@ -1121,8 +1125,31 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
                             bmask |= (Mask)push << (NB * e + bj);
                         }
                     }
+                    am4[bi] = amask;
+                    bm4[bi] = bmask;
+                }
+#pragma unroll 1
+                for (int bi = 0; bi < 4; ++bi) {
+                    // register arrays need constant indices
+                    const Mask amask = bi == 0 ? am4[0] : bi == 1 ? am4[1] : bi == 2 ? am4[2] : am4[3];
+                    const Mask bmask = bi == 0 ? bm4[0] : bi == 1 ? bm4[1] : bi == 2 ? bm4[2] : bm4[3];
                     // band entries
                     if (__builtin_amdgcn_ballot_w64(bmask != 0)) {
+                        floatx16 cur[NB];
+                        switch (bi) {
+                        case 0:
+                            cur[0] = acc[0][0], cur[1] = acc[0][1];
+                            break;
+                        case 1:
+                            cur[0] = acc[1][0], cur[1] = acc[1][1];
+                            break;
+                        case 2:
+                            cur[0] = acc[2][0], cur[1] = acc[2][1];
+                            break;
+                        default:
+                            cur[0] = acc[3][0], cur[1] = acc[3][1];
+                            break;
+                        }
 #pragma unroll
                         for (int bj = 0; bj < (int)NB; ++bj)
 #pragma unroll
@@ -1131,7 +1158,7 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
                                 if (__builtin_amdgcn_ballot_w64((bmask & col) != 0)) {
 #pragma unroll
                                     for (int j = 0; j < 8; ++j)
-                                        gst[wv][j][lane] = acc[bi][bj][8 * hf + j];
+                                        gst[wv][j][lane] = cur[bj][8 * hf + j];
                                     band_run(bi, bj, hf, bmask, amask);
                                 }
                             }
