@@ -556,6 +556,10 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
     };
     // planes copied for a tile's first pass (REF: the high plane only)
     constexpr uint32_t NP0 = REF ? 1 : NP;
+    // split half tiles: the BF16 pass reads each wave's fragments straight
+    // from global memory into registers (no LDS slab, no barrier per
+    // k-step); only the cross products use the slab region
+    constexpr bool DIRECT = SPLIT && HALF && REF;
 
     // the first tile
     uint64_t v = blockIdx.x;
@@ -575,7 +579,8 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
     uint32_t buf = 0;  // LDS buffer of the current slab
     if (have) {
         fetch_terms(cs, w);
-        copy_slab(a0_of(cs, w) / 32, b0_of(cs, w) / 32, 0, 0, NP0);
+        if (!DIRECT)
+            copy_slab(a0_of(cs, w) / 32, b0_of(cs, w) / 32, 0, 0, NP0);
     }
     while (have) {
         __syncthreads();  // the previous tile is done with tA .. jB
@@ -868,7 +873,40 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
         if constexpr (!REF) {
             kpass(std::integral_constant<int, 0>());
         } else {
-            kpass(std::integral_constant<int, 1>());
+            if constexpr (DIRECT) {
+                // k-step k's 4 + 2 high-plane fragments of this wave, one
+                // 16-byte load per lane each; two k-steps in flight (the
+                // loads of k + 1 are issued before the MFMAs of k)
+                const uint4 *pa = fh + ((size_t)(rbA + 4 * wr) * KS) * WAVE + lane;
+                const uint4 *pb = fh + ((size_t)(rbB + 2 * wc) * KS) * WAVE + lane;
+                auto ld = [&](uint32_t k, uint4 (&a)[4], uint4 (&b)[2]) __attribute__((always_inline)) {
+#pragma unroll
+                    for (int bi = 0; bi < 4; ++bi)
+                        a[bi] = pa[((size_t)bi * KS + k) * WAVE];
+#pragma unroll
+                    for (int bj = 0; bj < 2; ++bj)
+                        b[bj] = pb[((size_t)bj * KS + k) * WAVE];
+                };
+                auto mm = [&](const uint4 (&a)[4], const uint4 (&b)[2]) __attribute__((always_inline)) {
+#pragma unroll
+                    for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+                        for (int bj = 0; bj < 2; ++bj)
+                            acc[bi][bj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                                as_bf16x8(a[bi]), as_bf16x8(b[bj]), acc[bi][bj], 0, 0, 0);
+                };
+                uint4 a0[4], b0[2], a1[4], b1[2];
+                ld(0, a0, b0);
+                for (uint32_t k0 = 0; k0 < KS; k0 += 2) {  // KS is even
+                    ld(k0 + 1, a1, b1);
+                    mm(a0, b0);
+                    if (k0 + 2 < KS)
+                        ld(k0 + 2, a0, b0);
+                    mm(a1, b1);
+                }
+            } else {
+                kpass(std::integral_constant<int, 1>());
+            }
             // BF16 scores of the tile: if every pair is a certain reject
             // under the BF16 bound, the tile is done; otherwise (or when
             // sampled or NaN-scored) the cross products follow and the tile
@@ -1377,7 +1415,8 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
                     nw = items[ns.it];
             }
             if (nhave) {
-                copy_slab(a0_of(ns, nw) / 32, b0_of(ns, nw) / 32, 0, buf, NP0);
+                if (!DIRECT)
+                    copy_slab(a0_of(ns, nw) / 32, b0_of(ns, nw) / 32, 0, buf, NP0);
                 fetch_terms(ns, nw);
             }
         }
