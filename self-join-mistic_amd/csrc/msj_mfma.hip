@@ -389,15 +389,13 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
     constexpr uint32_t BHITS = 64;                // per-wave band staging
     static_assert(UNITS % NW == 0, "every wave issues the same number of copies");
     static_assert(!REF || SPLIT, "BF16-first refinement: split-BF16 only");
+    // split half tiles: both passes read each wave's fragments straight from
+    // global memory into registers (no LDS slab, no barrier per k-step)
+    constexpr bool DIRECT = SPLIT && HALF && REF;
     // two slab buffers: the next slab is copied (global -> LDS, no VGPRs)
-    // while the matrix cores work on the current one.  One region, A blocks
-    // then B blocks; the cross-product pass of split half tiles uses it as
-    // two halves of compacted slabs (CUNITS 1 KB units each).
-    constexpr uint32_t CUNITS = NP * (SLA + SLB) / WAVE;
-    __shared__ uint4 slab[2 * NP * (SLA + SLB)];
-    auto &sA = *reinterpret_cast<uint4(*)[2][NP][SLA]>(&slab[0]);
-    auto &sB = *reinterpret_cast<uint4(*)[2][NP][SLB]>(&slab[2 * NP * SLA]);
-    __shared__ uint32_t s_need[NW];  // split half tiles: per wave, the blocks to refine
+    // while the matrix cores work on the current one
+    __shared__ uint4 sA[2][NP][DIRECT ? 1 : SLA];
+    __shared__ uint4 sB[2][NP][DIRECT ? 1 : SLB];
     __shared__ float4 tA[TM], tB[TN];
     __shared__ __align__(16) float tAy[TM];  // tA[.].y, four rows per 16-byte read
     // REF: the rows' reject terms under the BF16 bound (first pass)
@@ -556,10 +554,6 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
     };
     // planes copied for a tile's first pass (REF: the high plane only)
     constexpr uint32_t NP0 = REF ? 1 : NP;
-    // split half tiles: the BF16 pass reads each wave's fragments straight
-    // from global memory into registers (no LDS slab, no barrier per
-    // k-step); only the cross products use the slab region
-    constexpr bool DIRECT = SPLIT && HALF && REF;
 
     // the first tile
     uint64_t v = blockIdx.x;
@@ -627,6 +621,21 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
                 nw = items[ns.it];
         }
         bool nhave = false;
+        // the next valid slot (a skipped slot costs one more item load)
+        auto find_next = [&]() {
+            while (vn < vtotal) {
+                if (valid(ns, nw)) {
+                    nhave = true;
+                    break;
+                }
+                vn += gridDim.x;
+                if (vn >= vtotal)
+                    break;
+                ns = slot_of(vn);
+                if (ns.in)
+                    nw = items[ns.it];
+            }
+        };
 
         floatx16 acc[4][2];
 #pragma unroll
@@ -789,85 +798,6 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
             }
 
         };
-        // Cross products of split half tiles, block by block: only the 32 x
-        // 32 blocks in the wave's mask wm take them, and the workgroup copies
-        // only the row blocks that some wave's mask needs (an: A blocks, bn:
-        // B blocks), both planes, into compacted slabs of KP k-steps: unit
-        // (k-step kk, r) of a slab holds A block r / 2 of an's list (r <
-        // 2 nA) or B block (r - 2 nA) / 2 of bn's, plane r % 2.  A slab
-        // fills as much of the region (2 CUNITS units) as it can: each is
-        // one round trip to memory, and the MFMAs between them are few.
-        auto xpass = [&](uint32_t an, uint32_t bn, uint32_t wm) __attribute__((always_inline)) {
-            const uint32_t nA = __popc(an), nB = __popc(bn);
-            const uint32_t U2 = 2 * (nA + nB);                      // units per k-step
-            const uint32_t KP = std::min<uint32_t>(2 * CUNITS / U2, KS);  // >= 2: U2 <= 24
-            auto nth = [](uint32_t msk, uint32_t k) {  // index of the k-th set bit
-                for (uint32_t i = 0; i < k; ++i)
-                    msk &= msk - 1u;
-                return (uint32_t)__builtin_ctz(msk);
-            };
-            auto ccopy = [&](uint32_t k0, uint32_t nk) {
-                for (uint32_t u = wv; u < nk * U2; u += NW) {
-                    const uint32_t kk = u / U2, r = u % U2;
-                    const bool isb = r >= 2 * nA;
-                    const uint32_t rb = isb ? rbB + nth(bn, (r - 2 * nA) >> 1) : rbA + nth(an, r >> 1);
-                    const uint4 *src = ((r & 1u) ? fl : fh) + ((size_t)rb * KS + k0 + kk) * WAVE + lane;
-                    uint4 *dst = &slab[u * WAVE];
-                    __builtin_amdgcn_global_load_lds((const void *)src,
-                                                     (void __attribute__((address_space(3))) *)dst,
-                                                     16, 0, 0);
-                }
-            };
-            // every wave is past its reads of the first pass's slabs (the
-            // barrier that published the masks), and no copy is in flight
-            for (uint32_t k0 = 0; k0 < KS; k0 += KP) {
-                const uint32_t nk = std::min<uint32_t>(KP, KS - k0);
-                if (k0 != 0)
-                    __syncthreads();  // every wave is done with the previous slab
-                ccopy(k0, nk);
-                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-                __syncthreads();
-                for (uint32_t kk = 0; kk < nk; ++kk) {
-                    const uint32_t kb = kk * U2;
-                    // the wave's B fragments (both planes) once per k-step,
-                    // then per 32-row block its A pair and the two cross
-                    // products of its blocks, independent MFMAs adjacent
-                    uint4 bh[2], bl[2];
-#pragma unroll
-                    for (int bj = 0; bj < 2; ++bj) {
-                        bh[bj] = bl[bj] = make_uint4(0, 0, 0, 0);
-                        if ((wm & (0x55u << bj)) == 0)
-                            continue;
-                        const uint32_t ib = __popc(bn & ((1u << (2 * wc + bj)) - 1u));
-                        const uint32_t ub = kb + 2 * nA + 2 * ib;
-                        bh[bj] = slab[ub * WAVE + lane];
-                        bl[bj] = slab[(ub + 1) * WAVE + lane];
-                    }
-#pragma unroll
-                    for (int bi = 0; bi < 4; ++bi) {
-                        const uint32_t sel = (wm >> (2 * bi)) & 3u;
-                        if (sel == 0)
-                            continue;
-                        const uint32_t ia = __popc(an & ((1u << (4 * wr + bi)) - 1u));
-                        const uint4 ah = slab[(kb + 2 * ia) * WAVE + lane];
-                        const uint4 al = slab[(kb + 2 * ia + 1) * WAVE + lane];
-#pragma unroll
-                        for (int bj = 0; bj < 2; ++bj)
-                            if ((sel >> bj) & 1u)
-                                acc[bi][bj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-                                    as_bf16x8(ah), as_bf16x8(bl[bj]), acc[bi][bj], 0, 0, 0);
-#pragma unroll
-                        for (int bj = 0; bj < 2; ++bj)
-                            if ((sel >> bj) & 1u)
-                                acc[bi][bj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-                                    as_bf16x8(al), as_bf16x8(bh[bj]), acc[bi][bj], 0, 0, 0);
-                    }
-                }
-            }
-            // the compacted slabs span both buffers: every wave is done
-            // reading them before the next tile's first copy
-            __syncthreads();
-        };
         uint32_t wmask = 0xFFu;  // split half tiles: the blocks that were refined
         bool refined = !REF;
         if constexpr (!REF) {
@@ -904,6 +834,11 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
                         ld(k0 + 2, a0, b0);
                     mm(a1, b1);
                 }
+                // the next tile (its item was loaded at this tile's start):
+                // its row terms arrive during this tile's test and epilogue
+                find_next();
+                if (nhave)
+                    fetch_terms(ns, nw);
             } else {
                 kpass(std::integral_constant<int, 1>());
             }
@@ -992,32 +927,62 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
             }
             const bool need = sampled || tile_nan || !(m > hi);
             wneed = __builtin_amdgcn_ballot_w64(need) != 0;
-            if constexpr (HALF) {
+            if constexpr (DIRECT) {
                 // the 32 x 32 blocks to refine (all of a sampled or NaN-scored
-                // tile); the workgroup copies the row blocks they need
+                // tile): each wave loads both planes of the fragments its
+                // blocks need straight into registers, k-step by k-step, and
+                // adds the two cross products (no LDS, no barrier)
                 const uint32_t wm = (sampled || tile_nan) ? 0xFFu : nmask;
                 wmask = wm;
                 wneed = wm != 0;
-                if (lane == 0)
-                    s_need[wv] = wm;
-                __syncthreads();
-                uint32_t an = 0, bn = 0;  // A row blocks (8), B row blocks (4)
-#pragma unroll
-                for (uint32_t q = 0; q < NW; ++q) {
-                    const uint32_t x = __builtin_amdgcn_readfirstlane(s_need[q]);
+                if (wm != 0) {
+                    if (lane == 0)  // 32 x 32 blocks that take the cross products
+                        atomicAdd(&ctr[8], (unsigned long long)__popc(wm));
+                    uint32_t rows = 0, cols = 0;  // the wave's 32-row blocks and columns in wm
 #pragma unroll
                     for (uint32_t bi = 0; bi < 4; ++bi)
-                        if ((x >> (2 * bi)) & 3u)
-                            an |= 1u << (4 * (q >> 1) + bi);
+                        if ((wm >> (2 * bi)) & 3u)
+                            rows |= 1u << bi;
 #pragma unroll
                     for (uint32_t bj = 0; bj < 2; ++bj)
-                        if (x & (0x55u << bj))
-                            bn |= 1u << (2 * (q & 1u) + bj);
-                }
-                if (an != 0) {
-                    if (wm != 0 && lane == 0)  // 32 x 32 blocks that take the cross products
-                        atomicAdd(&ctr[8], (unsigned long long)__popc(wm));
-                    xpass(an, bn, wm);
+                        if (wm & (0x55u << bj))
+                            cols |= 1u << bj;
+                    const size_t ra = (size_t)(rbA + 4 * wr) * KS, rb = (size_t)(rbB + 2 * wc) * KS;
+                    for (uint32_t k = 0; k < KS; ++k) {
+                        uint4 ah[4], al[4], bh[2], bl[2];
+#pragma unroll
+                        for (int bi = 0; bi < 4; ++bi) {
+                            ah[bi] = al[bi] = make_uint4(0, 0, 0, 0);
+                            if ((rows >> bi) & 1u) {
+                                const size_t o = (ra + (size_t)bi * KS + k) * WAVE + lane;
+                                ah[bi] = fh[o];
+                                al[bi] = fl[o];
+                            }
+                        }
+#pragma unroll
+                        for (int bj = 0; bj < 2; ++bj) {
+                            bh[bj] = bl[bj] = make_uint4(0, 0, 0, 0);
+                            if ((cols >> bj) & 1u) {
+                                const size_t o = (rb + (size_t)bj * KS + k) * WAVE + lane;
+                                bh[bj] = fh[o];
+                                bl[bj] = fl[o];
+                            }
+                        }
+#pragma unroll
+                        for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+                            for (int bj = 0; bj < 2; ++bj)
+                                if ((wm >> (2 * bi + bj)) & 1u)
+                                    acc[bi][bj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                                        as_bf16x8(ah[bi]), as_bf16x8(bl[bj]), acc[bi][bj], 0, 0, 0);
+#pragma unroll
+                        for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+                            for (int bj = 0; bj < 2; ++bj)
+                                if ((wm >> (2 * bi + bj)) & 1u)
+                                    acc[bi][bj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                                        as_bf16x8(al[bi]), as_bf16x8(bh[bj]), acc[bi][bj], 0, 0, 0);
+                    }
                     refined = true;
                 }
             } else if (__syncthreads_or(need) != 0) {
@@ -1399,24 +1364,12 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
                 }
             }
         }
-        if constexpr (REF) {
+        if constexpr (REF && !DIRECT) {
             // the next tile: find it (a skipped slot costs one more item
             // load), start its first slab (high plane) and its row terms
-            while (vn < vtotal) {
-                if (valid(ns, nw)) {
-                    nhave = true;
-                    break;
-                }
-                vn += gridDim.x;
-                if (vn >= vtotal)
-                    break;
-                ns = slot_of(vn);
-                if (ns.in)
-                    nw = items[ns.it];
-            }
+            find_next();
             if (nhave) {
-                if (!DIRECT)
-                    copy_slab(a0_of(ns, nw) / 32, b0_of(ns, nw) / 32, 0, buf, NP0);
+                copy_slab(a0_of(ns, nw) / 32, b0_of(ns, nw) / 32, 0, buf, NP0);
                 fetch_terms(ns, nw);
             }
         }
