@@ -49,19 +49,23 @@
 // d = 128 two workgroups of 4 waves share a CU, each computing a half tile
 // (256 x 128), so one's epilogue overlaps the other's MFMAs.  Tiles are
 // grouped into super-blocks of up to 8 x 8 tiles (the work items);
-// consecutive tile slots run on one XCD.  Slabs of 32 dimensions (16 for
-// split half tiles) are copied global -> LDS directly into a double buffer:
-// the next slab's copy is in flight while the current one runs on the
-// matrix cores.
+// consecutive tile slots run on one XCD.  Slabs of 32 dimensions are
+// copied global -> LDS directly into a double buffer: the next slab's copy
+// is in flight while the current one runs on the matrix cores.  Split half
+// tiles use no slabs: each wave loads its own fragments (the layout is the
+// operand layout) from global memory straight into registers, the next
+// k-step's loads in flight during the current one's MFMAs, with no barrier
+// between waves (measured on 515,345 x 90: 87 -> 80 ms for the BF16 pass,
+// 80 -> 72 ms for the cross products).
 //
 // Split-BF16 tiles are scored in BF16 first (high planes only, one third of
 // the products): a 32 x 32 block whose pairs are all certain rejects under
 // the BF16 bound (C of a BF16 accumulation of d products) is done.  Any
 // other block (a pair that is not, a sampled item, NaN terms) adds the two
 // cross products and is classified under the split-BF16 bound, as without
-// this step: half tiles refine just those blocks, from slabs holding only
-// the row blocks they need; full tiles refine the whole 128 x 64 block of
-// a wave that has one.  The split bound covers the reordered accumulation
+// this step: half tiles refine just those blocks, each wave loading both
+// planes of the fragments they need; full tiles refine the whole 128 x 64
+// block of a wave that has one.  The split bound covers the reordered accumulation
 // (any order of the FP32 additions).  On this data most blocks pair rows
 // that are far apart and stop after the BF16 pass.
 #include <algorithm>
@@ -354,7 +358,8 @@ __global__ __launch_bounds__(256) void k_sblk_fill(const uint32_t *__restrict__ 
 // of B; the high plane) from global memory straight into LDS, and each wave
 // reads its 4 + 2 fragments per k-step from there; the next slab's copy
 // runs into the other LDS buffer while the matrix cores work on the current
-// one.
+// one.  Split half tiles (DIRECT) load the 4 + 2 fragments per k-step into
+// registers instead.
 //
 // Epilogue.  The A and B rows' terms are staged per tile; rows outside the
 // item (padding, the next node) get +inf terms, so the fast pass -- one add,
@@ -362,8 +367,9 @@ __global__ __launch_bounds__(256) void k_sblk_fill(const uint32_t *__restrict__ 
 // Only a 32-row block that holds a pair that is not a certain reject (or
 // NaN-scored) takes the full pass: it tests validity explicitly, collects
 // certain accepts into a lane mask and stages band entries in LDS.  Half
-// tiles take the blocks one at a time through all passes (a band block's
-// accumulators go through a per-lane LDS column); full tiles run the fast
+// tiles classify the blocks one at a time (unrolled), then emit band entries
+// (a band block's accumulators go through a per-lane LDS column) and
+// certain accepts in a loop that is not unrolled; full tiles run the fast
 // pass over all four blocks first.
 template <bool SPLIT, bool DIAG, bool HALF, bool REF>
 __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
