@@ -65,9 +65,9 @@
 // cross products and is classified under the split-BF16 bound, as without
 // this step: half tiles refine just those blocks, each wave loading both
 // planes of the fragments they need; full tiles refine the whole 128 x 64
-// block of a wave that has one.  The split bound covers the reordered accumulation
-// (any order of the FP32 additions).  On this data most blocks pair rows
-// that are far apart and stop after the BF16 pass.
+// block of a wave that has one.  The split bound covers the reordered
+// accumulation (any order of the FP32 additions).  On this data most blocks
+// pair rows that are far apart and stop after the BF16 pass.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
