@@ -372,7 +372,7 @@ __global__ __launch_bounds__(256) void k_sblk_fill(const uint32_t *__restrict__ 
 // certain accepts in a loop that is not unrolled; full tiles run the fast
 // pass over all four blocks first.
 template <bool SPLIT, bool DIAG, bool HALF, bool REF>
-__global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
+__global__ __launch_bounds__(HALF ? 256 : 512, HALF ? (SPLIT && REF ? 3 : 2) : 1) void k_mfma_join(
     const uint4 *__restrict__ fh, const uint4 *__restrict__ fl, uint32_t KS,
     const float4 *__restrict__ thr, const float2 *__restrict__ thrb,
     const uint32_t *__restrict__ porig,
@@ -398,15 +398,21 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
     // split half tiles: both passes read each wave's fragments straight from
     // global memory into registers (no LDS slab, no barrier per k-step)
     constexpr bool DIRECT = SPLIT && HALF && REF;
+    // 32-row blocks per wave (DIRECT: 64 x 64 wave blocks, 128 x 128 quarter
+    // tiles, fewer registers: three workgroups per CU) and the A rows of a
+    // workgroup tile
+    constexpr uint32_t NBI = DIRECT ? 2 : 4;
+    constexpr uint32_t WRS = 32 * NBI;
+    constexpr uint32_t TMA = 2 * WRS;
     // two slab buffers: the next slab is copied (global -> LDS, no VGPRs)
     // while the matrix cores work on the current one
     __shared__ uint4 sA[2][NP][DIRECT ? 1 : SLA];
     __shared__ uint4 sB[2][NP][DIRECT ? 1 : SLB];
-    __shared__ float4 tA[TM], tB[TN];
-    __shared__ __align__(16) float tAy[TM];  // tA[.].y, four rows per 16-byte read
+    __shared__ float4 tA[TMA], tB[TN];
+    __shared__ __align__(16) float tAy[TMA];  // tA[.].y, four rows per 16-byte read
     // REF: the rows' reject terms under the BF16 bound (first pass)
-    __shared__ __align__(16) float tAyb[REF ? TM : 4], tByb[REF ? TN : 4];
-    __shared__ uint32_t oA[TM], oB[TN], jA[TM], jB[TN];
+    __shared__ __align__(16) float tAyb[REF ? TMA : 4], tByb[REF ? TN : 4];
+    __shared__ uint32_t oA[TMA], oB[TN], jA[TMA], jB[TN];
     __shared__ unsigned long long hsh[NW][MHITS];
     __shared__ unsigned long long bsh[NW][BHITS];
     __shared__ float ssh[NW][BHITS];
@@ -474,14 +480,16 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
     const uint64_t vtotal = 8ull * per_xcd;
     struct Slot {
         uint64_t it;
-        uint32_t ta, tbk, half;
+        uint32_t ta, tbk, half, aq;
         bool in;
     };
+    // (DIRECT: quarter slots, B half = u % 2, A half aq = u / 2 % 2)
     auto slot_of = [&](uint64_t v) {
         Slot S;
         const uint64_t u = (v % 8u) * per_xcd + v / 8u;
-        const uint64_t t = HALF ? u >> 1 : u;
+        const uint64_t t = DIRECT ? u >> 2 : HALF ? u >> 1 : u;
         S.half = HALF ? (uint32_t)(u & 1u) : 0u;
+        S.aq = DIRECT ? (uint32_t)((u >> 1) & 1u) : 0u;
         // the slot map stays in memory (scalar loads once per tile): held
         // in SGPRs for the whole kernel it pushed other values into spills
         const SlotMap &sm = *smp;
@@ -497,12 +505,15 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
         S.tbk = r % sz;
         return S;
     };
-    // first A row and first B row of a half tile
-    auto a0_of = [&](const Slot &S, const MItem &w) { return w.qbeg + S.ta * TM; };
+    // first A row and first B row of a half (quarter) tile
+    auto a0_of = [&](const Slot &S, const MItem &w) { return w.qbeg + S.ta * TM + S.aq * TMA; };
     auto b0_of = [&](const Slot &S, const MItem &w) { return w.cbeg + S.tbk * TM + S.half * TN; };
+    // (a diagonal tile's quarter of A rows 128.. against B rows ..127 lies
+    // below the diagonal)
     auto valid = [&](const Slot &S, const MItem &w) {
         return S.in && a0_of(S, w) < w.qend && b0_of(S, w) < w.cend &&
-               !((w.self & 0x200u) && S.tbk < S.ta);
+               !((w.self & 0x200u) && S.tbk < S.ta) &&
+               !(DIRECT && (w.self & 0x200u) && S.tbk == S.ta && S.aq > S.half);
     };
     // slab copy into buffer `buf`: unit u = ((plane * (RBA + RBB)) + block)
     // * KL + kstep, blocks 0..RBA-1 of A then RBB of B, is one wave-wide
@@ -533,7 +544,7 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
     float tyb = 0.f, uyb = 0.f;  // REF: reject terms under the BF16 bound
     uint32_t o4 = 0, j4 = 0, ou = 0, ju = 0;
     auto fetch_terms = [&](const Slot &S, const MItem &w) {
-        if (tid < TM) {
+        if (tid < TMA) {
             const uint32_t ra = a0_of(S, w) + tid;
             t4 = thr[ra];
             if (REF)
@@ -563,7 +574,7 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
 
     // the first tile
     uint64_t v = blockIdx.x;
-    Slot cs = {0, 0, 0, 0, false};
+    Slot cs = {0, 0, 0, 0, 0, false};
     MItem w = {0, 0, 0, 0, 0};
     bool have = false;
     for (; v < vtotal; v += gridDim.x) {
@@ -584,7 +595,7 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
     }
     while (have) {
         __syncthreads();  // the previous tile is done with tA .. jB
-        if (tid < TM) {
+        if (tid < TMA) {
             tA[tid] = t4;
             oA[tid] = o4;
             jA[tid] = j4;
@@ -607,7 +618,7 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
                      band_budget;
         // a row of the tile with NaN terms (non-finite or huge values) sends
         // every 32-row block to the full pass
-        const bool tile_nan = __syncthreads_or((tid < TM && (t4.x != t4.x || t4.y != t4.y)) ||
+        const bool tile_nan = __syncthreads_or((tid < TMA && (t4.x != t4.x || t4.y != t4.y)) ||
                                                ((HALF ? tid < TN : tid >= TM) &&
                                                 (u4.x != u4.x || u4.y != u4.y))) != 0;
         if (s_stop)
@@ -619,7 +630,7 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
         const uint32_t rbA = qbeg / 32, rbB = cb / 32;
         // the next slot's item, loaded now, tested before the last slab
         uint64_t vn = v + gridDim.x;
-        Slot ns = {0, 0, 0, 0, false};
+        Slot ns = {0, 0, 0, 0, 0, false};
         MItem nw = {0, 0, 0, 0, 0};
         if (vn < vtotal) {
             ns = slot_of(vn);
@@ -643,9 +654,9 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
             }
         };
 
-        floatx16 acc[4][2];
+        floatx16 acc[NBI][2];
 #pragma unroll
-        for (int bi = 0; bi < 4; ++bi)
+        for (int bi = 0; bi < (int)NBI; ++bi)
 #pragma unroll
             for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
@@ -813,25 +824,25 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
                 // k-step k's 4 + 2 high-plane fragments of this wave, one
                 // 16-byte load per lane each; two k-steps in flight (the
                 // loads of k + 1 are issued before the MFMAs of k)
-                const uint4 *pa = fh + ((size_t)(rbA + 4 * wr) * KS) * WAVE + lane;
+                const uint4 *pa = fh + ((size_t)(rbA + NBI * wr) * KS) * WAVE + lane;
                 const uint4 *pb = fh + ((size_t)(rbB + 2 * wc) * KS) * WAVE + lane;
-                auto ld = [&](uint32_t k, uint4 (&a)[4], uint4 (&b)[2]) __attribute__((always_inline)) {
+                auto ld = [&](uint32_t k, uint4 (&a)[NBI], uint4 (&b)[2]) __attribute__((always_inline)) {
 #pragma unroll
-                    for (int bi = 0; bi < 4; ++bi)
+                    for (int bi = 0; bi < (int)NBI; ++bi)
                         a[bi] = pa[((size_t)bi * KS + k) * WAVE];
 #pragma unroll
                     for (int bj = 0; bj < 2; ++bj)
                         b[bj] = pb[((size_t)bj * KS + k) * WAVE];
                 };
-                auto mm = [&](const uint4 (&a)[4], const uint4 (&b)[2]) __attribute__((always_inline)) {
+                auto mm = [&](const uint4 (&a)[NBI], const uint4 (&b)[2]) __attribute__((always_inline)) {
 #pragma unroll
-                    for (int bi = 0; bi < 4; ++bi)
+                    for (int bi = 0; bi < (int)NBI; ++bi)
 #pragma unroll
                         for (int bj = 0; bj < 2; ++bj)
                             acc[bi][bj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
                                 as_bf16x8(a[bi]), as_bf16x8(b[bj]), acc[bi][bj], 0, 0, 0);
                 };
-                uint4 a0[4], b0[2], a1[4], b1[2];
+                uint4 a0[NBI], b0[2], a1[NBI], b1[2];
                 ld(0, a0, b0);
                 for (uint32_t k0 = 0; k0 < KS; k0 += 2) {  // KS is even
                     ld(k0 + 1, a1, b1);
@@ -869,11 +880,11 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
             uint32_t nmask = 0;
             auto min3 = [](float a, float b, float c) { return fminf(fminf(a, b), c); };
 #pragma unroll
-            for (int bi = 0; bi < 4; ++bi) {
+            for (int bi = 0; bi < (int)NBI; ++bi) {
                 float s[2][16];
 #pragma unroll
                 for (int eg = 0; eg < 4; ++eg) {
-                    const float4 ay4 = *(const float4 *)&tAyb[128 * wr + 32 * bi + 8 * eg + 4 * h];
+                    const float4 ay4 = *(const float4 *)&tAyb[WRS * wr + 32 * bi + 8 * eg + 4 * h];
                     const float ay[4] = {ay4.x, ay4.y, ay4.z, ay4.w};
 #pragma unroll
                     for (int ei = 0; ei < 4; ++ei)
@@ -899,12 +910,12 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
                 // goes to the FP64 check as class 3 before the cross products
                 // overwrite the accumulators
                 if (sampled) {
-                    const bool dg = self && cb < qbeg + TM && qbeg < cb + TN;
+                    const bool dg = self && cb < qbeg + TMA && qbeg < cb + TN;
 #pragma unroll
-                    for (int bi = 0; bi < 4; ++bi)  // constant register indices
+                    for (int bi = 0; bi < (int)NBI; ++bi)  // constant register indices
 #pragma unroll
                         for (int e = 0; e < 16; ++e) {
-                            const uint32_t il = 128 * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
+                            const uint32_t il = WRS * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
 #pragma unroll
                             for (int bj = 0; bj < 2; ++bj) {
                                 const uint32_t jl = 64 * wc + 32 * bj + (lane & 31);
@@ -938,7 +949,7 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
                 // tile): each wave loads both planes of the fragments its
                 // blocks need straight into registers, k-step by k-step, and
                 // adds the two cross products (no LDS, no barrier)
-                const uint32_t wm = (sampled || tile_nan) ? 0xFFu : nmask;
+                const uint32_t wm = (sampled || tile_nan) ? (1u << (2 * NBI)) - 1u : nmask;
                 wmask = wm;
                 wneed = wm != 0;
                 if (wm != 0) {
@@ -946,18 +957,18 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
                         atomicAdd(&ctr[8], (unsigned long long)__popc(wm));
                     uint32_t rows = 0, cols = 0;  // the wave's 32-row blocks and columns in wm
 #pragma unroll
-                    for (uint32_t bi = 0; bi < 4; ++bi)
+                    for (uint32_t bi = 0; bi < NBI; ++bi)
                         if ((wm >> (2 * bi)) & 3u)
                             rows |= 1u << bi;
 #pragma unroll
                     for (uint32_t bj = 0; bj < 2; ++bj)
                         if (wm & (0x55u << bj))
                             cols |= 1u << bj;
-                    const size_t ra = (size_t)(rbA + 4 * wr) * KS, rb = (size_t)(rbB + 2 * wc) * KS;
+                    const size_t ra = (size_t)(rbA + NBI * wr) * KS, rb = (size_t)(rbB + 2 * wc) * KS;
                     for (uint32_t k = 0; k < KS; ++k) {
-                        uint4 ah[4], al[4], bh[2], bl[2];
+                        uint4 ah[NBI], al[NBI], bh[2], bl[2];
 #pragma unroll
-                        for (int bi = 0; bi < 4; ++bi) {
+                        for (int bi = 0; bi < (int)NBI; ++bi) {
                             ah[bi] = al[bi] = make_uint4(0, 0, 0, 0);
                             if ((rows >> bi) & 1u) {
                                 const size_t o = (ra + (size_t)bi * KS + k) * WAVE + lane;
@@ -975,14 +986,14 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
                             }
                         }
 #pragma unroll
-                        for (int bi = 0; bi < 4; ++bi)
+                        for (int bi = 0; bi < (int)NBI; ++bi)
 #pragma unroll
                             for (int bj = 0; bj < 2; ++bj)
                                 if ((wm >> (2 * bi + bj)) & 1u)
                                     acc[bi][bj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
                                         as_bf16x8(ah[bi]), as_bf16x8(bl[bj]), acc[bi][bj], 0, 0, 0);
 #pragma unroll
-                        for (int bi = 0; bi < 4; ++bi)
+                        for (int bi = 0; bi < (int)NBI; ++bi)
 #pragma unroll
                             for (int bj = 0; bj < 2; ++bj)
                                 if ((wm >> (2 * bi + bj)) & 1u)
@@ -1008,7 +1019,7 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
             // j + 32h, element e, with i = 8 (e / 4) + 4 h + e % 4.
             // a half tile crossing the diagonal of a node with itself: pairs
             // p <= c only (there cb >= qbeg: doff = cb - qbeg is 0 or TN)
-            const bool diag_tile = self && cb < qbeg + TM && qbeg < cb + TN;
+            const bool diag_tile = self && cb < qbeg + TMA && qbeg < cb + TN;
             const uint32_t doff = cb - qbeg;
             float4 tb[2];
 #pragma unroll
@@ -1034,7 +1045,7 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
 #pragma unroll 1
                     for (uint32_t j = 0; j < 8; ++j) {
                         const uint32_t e = 8 * hf + j;
-                        const uint32_t il = 128 * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
+                        const uint32_t il = WRS * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
                         const bool push = (bm >> (NB * e + bj)) & 1u;
                         const uint64_t mb = __builtin_amdgcn_ballot_w64(push);
                         if (!mb)
@@ -1072,7 +1083,7 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
                 // (not unrolled: one copy of that code, fewer scalar registers)
                 Mask am4[4] = {0u, 0u, 0u, 0u}, bm4[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
-                for (int bi = 0; bi < 4; ++bi) {
+                for (int bi = 0; bi < (int)NBI; ++bi) {
                     // split (REF): only the refined blocks; the others hold
                     // certain rejects of the BF16 pass
                     const uint32_t bsel = REF ? (wmask >> (2 * bi)) & 3u : 3u;
@@ -1088,7 +1099,7 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
                         float m = INF;
 #pragma unroll
                         for (int eg = 0; eg < 4; ++eg) {
-                            const uint32_t i0 = 128 * wr + 32 * bi + 8 * eg + 4 * h;
+                            const uint32_t i0 = WRS * wr + 32 * bi + 8 * eg + 4 * h;
                             const float4 ay4 = *(const float4 *)&tAy[i0];
                             const float ay[4] = {ay4.x, ay4.y, ay4.z, ay4.w};
 #pragma unroll
@@ -1116,7 +1127,7 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
                     Mask amask = 0, bmask = 0;
 #pragma unroll
                     for (int e = 0; e < 16; ++e) {
-                        const uint32_t il = 128 * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
+                        const uint32_t il = WRS * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
                         const float4 ta = tA[il];
                         const uint32_t p = qbeg + il;
                         const bool pv = p < w.qend;
@@ -1138,7 +1149,7 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
                     bm4[bi] = bmask;
                 }
 #pragma unroll 1
-                for (int bi = 0; bi < 4; ++bi) {
+                for (int bi = 0; bi < (int)NBI; ++bi) {
                     // register arrays need constant indices
                     const Mask amask = bi == 0 ? am4[0] : bi == 1 ? am4[1] : bi == 2 ? am4[2] : am4[3];
                     const Mask bmask = bi == 0 ? bm4[0] : bi == 1 ? bm4[1] : bi == 2 ? bm4[2] : bm4[3];
@@ -1153,10 +1164,10 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
                             cur[0] = acc[1][0], cur[1] = acc[1][1];
                             break;
                         case 2:
-                            cur[0] = acc[2][0], cur[1] = acc[2][1];
+                            cur[0] = acc[NBI > 2 ? 2 : 0][0], cur[1] = acc[NBI > 2 ? 2 : 0][1];
                             break;
                         default:
-                            cur[0] = acc[3][0], cur[1] = acc[3][1];
+                            cur[0] = acc[NBI > 3 ? 3 : 0][0], cur[1] = acc[NBI > 3 ? 3 : 0][1];
                             break;
                         }
 #pragma unroll
@@ -1184,7 +1195,7 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
                             const uint32_t k = (uint32_t)__builtin_ctzll((uint64_t)am);
                             am &= am - 1;
                             const uint32_t e = k / NB, bj = k % NB;
-                            il = 128 * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
+                            il = WRS * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
                             jl = c0 + 32 * bj + (lane & 31);
                         }
                         const bool two = has && (qbeg + il) != (cb + jl);
@@ -1213,11 +1224,11 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
                 auto fast = [&](auto dg) {
                     constexpr bool DT = decltype(dg)::value;
 #pragma unroll
-                    for (int bi = 0; bi < 4; ++bi) {
+                    for (int bi = 0; bi < (int)NBI; ++bi) {
                         float m = INF;
 #pragma unroll
                         for (int eg = 0; eg < 4; ++eg) {
-                            const uint32_t i0 = 128 * wr + 32 * bi + 8 * eg + 4 * h;
+                            const uint32_t i0 = WRS * wr + 32 * bi + 8 * eg + 4 * h;
                             const float4 ay4 = *(const float4 *)&tAy[i0];
                             const float ay[4] = {ay4.x, ay4.y, ay4.z, ay4.w};
 #pragma unroll
@@ -1249,13 +1260,13 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
                     cvalid[bj] = cb + 64 * wc + 32 * bj + (lane & 31) < w.cend;
                 uint32_t amask[4] = {0u, 0u, 0u, 0u}, bmask[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
-                for (int bi = 0; bi < 4; ++bi) {
+                for (int bi = 0; bi < (int)NBI; ++bi) {
                     const bool need = sampled || tile_nan || !(mn[bi] > hi);
                     if (!__builtin_amdgcn_ballot_w64(need))
                         continue;
 #pragma unroll
                     for (int e = 0; e < 16; ++e) {
-                        const uint32_t il = 128 * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
+                        const uint32_t il = WRS * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
                         const float4 ta = tA[il];
                         const uint32_t p = qbeg + il;
                         const bool pv = p < w.qend;
@@ -1277,7 +1288,7 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
                 // band entries (rare): staged slot by slot, with their scores; one
                 // 32-row block at a time (not unrolled: a copy of its accumulators)
 #pragma unroll 1
-                for (int bi = 0; bi < 4; ++bi) {
+                for (int bi = 0; bi < (int)NBI; ++bi) {
                     // register arrays need constant indices
                     const uint32_t bm =
                         bi == 0 ? bmask[0] : bi == 1 ? bmask[1] : bi == 2 ? bmask[2] : bmask[3];
@@ -1294,15 +1305,15 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
                         cur[0] = acc[1][0], cur[1] = acc[1][1];
                         break;
                     case 2:
-                        cur[0] = acc[2][0], cur[1] = acc[2][1];
+                        cur[0] = acc[NBI > 2 ? 2 : 0][0], cur[1] = acc[NBI > 2 ? 2 : 0][1];
                         break;
                     default:
-                        cur[0] = acc[3][0], cur[1] = acc[3][1];
+                        cur[0] = acc[NBI > 3 ? 3 : 0][0], cur[1] = acc[NBI > 3 ? 3 : 0][1];
                         break;
                     }
 #pragma unroll
                     for (int e = 0; e < 16; ++e) {
-                        const uint32_t il = 128 * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
+                        const uint32_t il = WRS * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
 #pragma unroll
                         for (int bj = 0; bj < 2; ++bj) {
                             const bool push = (bm >> (2 * e + bj)) & 1u;
@@ -1337,7 +1348,7 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
                 }
                 // certain accepts: both orders (the self pair once)
 #pragma unroll
-                for (int bi = 0; bi < 4; ++bi) {
+                for (int bi = 0; bi < (int)NBI; ++bi) {
                     uint32_t am = amask[bi];
                     while (true) {
                         const bool has = am != 0;
@@ -1349,7 +1360,7 @@ __global__ __launch_bounds__(HALF ? 256 : 512, HALF ? 2 : 1) void k_mfma_join(
                             const uint32_t k = (uint32_t)__builtin_ctz(am);
                             am &= am - 1;
                             const uint32_t e = k >> 1, bj = k & 1u;
-                            il = 128 * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
+                            il = WRS * wr + 32 * bi + 8 * (e / 4) + 4 * h + (e % 4);
                             jl = 64 * wc + 32 * bj + (lane & 31);
                         }
                         const bool two = has && (qbeg + il) != (cb + jl);
@@ -1692,7 +1703,8 @@ uint64_t MfmaRunner::run(const Index &ix, JoinRunner &J, const double *xr, uint3
         // epilogue of one overlaps the other's MFMAs); above: 256 x 256 tiles
         // (twice the operand reuse per slab copy)
         const bool half = KS <= KS_HALF;
-        const uint64_t slots = (half ? 2 : 1) * smap.slot_base[SBS];
+        // slots: half tiles (split-BF16: quarter tiles, 128 x 128) or tiles
+        const uint64_t slots = (half ? (split ? 4 : 2) : 1) * smap.slot_base[SBS];
         const uint64_t per_xcd = (slots + 7) / 8;
         const uint32_t grid =
             (uint32_t)std::max<uint64_t>(8, std::min<uint64_t>(per_xcd * 8, MAX_GRID));
