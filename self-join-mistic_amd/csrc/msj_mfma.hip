@@ -47,7 +47,9 @@
 // pairs p <= c).  Above d = 128 a workgroup of 8 waves computes one tile at a
 // time, wave (wr, wc) a 128 x 64 block as 4 x 2 MFMA 32x32 tiles; up to
 // d = 128 two workgroups of 4 waves share a CU, each computing a half tile
-// (256 x 128), so one's epilogue overlaps the other's MFMAs.  Tiles are
+// (256 x 128), so one's epilogue overlaps the other's MFMAs; split-BF16 up
+// to d = 128 runs quarter tiles (128 x 128, 64 x 64 per wave, 168 VGPRs),
+// three workgroups per CU.  Tiles are
 // grouped into super-blocks of up to 8 x 8 tiles (the work items);
 // consecutive tile slots run on one XCD.  Slabs of 32 dimensions are
 // copied global -> LDS directly into a double buffer: the next slab's copy
@@ -350,10 +352,11 @@ __global__ __launch_bounds__(256) void k_sblk_fill(const uint32_t *__restrict__ 
 // reject under the BF16 bound -- the test that lets a block skip the cross
 // products (verified).
 //
-// Tile.  A workgroup (NW waves) computes a TM x TN tile (256 x 256, or a
-// 256 x 128 half tile with HALF; REF: split-BF16, BF16 pass first); wave
-// (wr, wc) the 128 x 64 block of rows 128 wr.. and columns 64 wc.., as 4 x 2
-// MFMA 32x32 tiles (128 FP32 accumulators per lane).  Per slab of KL
+// Tile.  A workgroup (NW waves) computes a TMA x TN tile (256 x 256, or a
+// 256 x 128 half tile with HALF, or a 128 x 128 quarter tile with DIRECT;
+// REF: split-BF16, BF16 pass first); wave (wr, wc) the WRS x 64 block of
+// rows WRS wr.. and columns 64 wc.., as NBI x 2 MFMA 32x32 tiles (128 or,
+// DIRECT, 64 FP32 accumulators per lane).  Per slab of KL
 // k-steps the workgroup copies the row blocks of fragments (8 of A, TN / 32
 // of B; the high plane) from global memory straight into LDS, and each wave
 // reads its 4 + 2 fragments per k-step from there; the next slab's copy
