@@ -25,6 +25,7 @@
 //      write pass; the neighbour lists come out in the reference's order.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -173,12 +174,18 @@ __global__ __launch_bounds__(SIG_BLOCK) void k_col_sigma(const double *__restric
             if (tid == 0) {
                 const double *T = tile[t & 1];
                 const uint32_t m = min(SIG_TILE, rows - t * SIG_TILE);
-                uint32_t r = 0;
-                for (; r + 16 <= m; r += 16) {
-                    double w[16];
+                const uint32_t m16 = m & ~15u;
+                // the next 16 values are read while the current 16 are added
+                double w[16];
+#pragma unroll
+                for (uint32_t k = 0; k < 16; ++k)
+                    w[k] = m16 ? T[k] : 0.0;
+                for (uint32_t r = 0; r < m16; r += 16) {
+                    const uint32_t rn = r + 16 < m16 ? r + 16 : r;
+                    double nx[16];
 #pragma unroll
                     for (uint32_t k = 0; k < 16; ++k)
-                        w[k] = T[r + k];
+                        nx[k] = T[rn + k];
 #pragma unroll
                     for (uint32_t k = 0; k < 16; ++k) {
                         if (pass == 0) {
@@ -189,8 +196,11 @@ __global__ __launch_bounds__(SIG_BLOCK) void k_col_sigma(const double *__restric
                             acc = acc + sq;
                         }
                     }
+#pragma unroll
+                    for (uint32_t k = 0; k < 16; ++k)
+                        w[k] = nx[k];
                 }
-                for (; r < m; ++r) {
+                for (uint32_t r = m16; r < m; ++r) {
                     if (pass == 0) {
                         acc = acc + T[r];
                     } else {
@@ -359,6 +369,109 @@ __global__ void k_cand_sse(const uint32_t *__restrict__ runfirst,
     }
     sse[t] = s;
     cnt[t] = r1 - r0;
+}
+
+// Dense form of the candidate node sizes, for layers whose (parent node,
+// key) bins fit a workgroup's LDS: bin node * K + key (K = max key + 1; key
+// 0 for a parent that stopped splitting) of candidate t counts the points
+// with that parent and key.  The non-empty bins in bin order are exactly the
+// runs of the composite-key sort, in the same order, so the SSE below adds
+// the same terms in the same sequence.
+constexpr uint32_t DENSE_BINS = 8192;   // bins per candidate: 32 KB of LDS
+constexpr uint32_t DENSE_CHUNK = 8192;  // points per histogram workgroup
+constexpr uint32_t STOPPED = 0x80000000u;
+
+// code[p] = first bin of point p's parent, STOPPED set when it stopped splitting.
+__global__ __launch_bounds__(256) void k_point_bins(const uint32_t *__restrict__ perm,
+                                                    const uint32_t *__restrict__ pnode,
+                                                    const uint8_t *__restrict__ nsplit,
+                                                    uint32_t n, uint32_t K,
+                                                    uint32_t *__restrict__ code)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const uint32_t node = pnode[i];
+    code[perm[i]] = node * K | (nsplit[node] ? 0u : STOPPED);
+}
+
+// Workgroup (x, t): points [x * DENSE_CHUNK, ...) of candidate t counted in
+// LDS (T <= DENSE_BINS bins), then added to tab[t * T + bin].
+__global__ __launch_bounds__(256) void k_dense_hist(const uint32_t *__restrict__ keys,
+                                                    const uint32_t *__restrict__ code, uint32_t n,
+                                                    uint32_t T, uint32_t *__restrict__ tab)
+{
+    __shared__ uint32_t h[DENSE_BINS];
+    const uint32_t t = blockIdx.y, tid = threadIdx.x;
+    for (uint32_t b = tid; b < T; b += blockDim.x)
+        h[b] = 0;
+    __syncthreads();
+    const uint32_t p0 = blockIdx.x * DENSE_CHUNK, p1 = min(n, p0 + DENSE_CHUNK);
+    const uint32_t *kt = keys + (size_t)t * n;
+    for (uint32_t p = p0 + tid; p < p1; p += blockDim.x) {
+        const uint32_t c = code[p];
+        const uint32_t bin = (c & STOPPED) ? (c & ~STOPPED) : c + kt[p];
+        atomicAdd(&h[bin], 1u);
+    }
+    __syncthreads();
+    uint32_t *tt = tab + (size_t)t * T;
+    for (uint32_t b = tid; b < T; b += blockDim.x)
+        if (h[b])
+            atomicAdd(&tt[b], h[b]);
+}
+
+// nodes.cu:194-199 over the dense bins of candidate t = blockIdx.x: the
+// non-empty bins are counted in parallel, then one thread adds
+// (size - N/#nodes)^2 bin by bin in bin order (an empty bin adds +0.0, which
+// leaves the non-negative sum unchanged).
+__global__ __launch_bounds__(256) void k_dense_sse(const uint32_t *__restrict__ tab, uint32_t T,
+                                                   uint32_t n, double *__restrict__ sse,
+                                                   uint32_t *__restrict__ cnt)
+{
+    __shared__ uint32_t h[DENSE_BINS + 32];
+    __shared__ uint32_t part[256 / WAVE];
+    const uint32_t t = blockIdx.x, tid = threadIdx.x;
+    const uint32_t *tt = tab + (size_t)t * T;
+    uint32_t c = 0;
+    for (uint32_t b = tid; b < T + 32; b += blockDim.x) {
+        const uint32_t v = b < T ? tt[b] : 0u;
+        h[b] = v;
+        c += v ? 1u : 0u;
+    }
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1)
+        c += (uint32_t)__shfl_down(c, o, WAVE);
+    if ((tid & (WAVE - 1)) == 0)
+        part[tid / WAVE] = c;
+    __syncthreads();
+    if (tid != 0)
+        return;
+    uint32_t nodes = 0;
+    for (uint32_t w = 0; w < blockDim.x / WAVE; ++w)
+        nodes += part[w];
+    const double avg = (double)n * 1.0 / (double)nodes;
+    double s = 0;
+    uint32_t v[16];
+#pragma unroll
+    for (uint32_t k = 0; k < 16; ++k)
+        v[k] = h[k];
+    for (uint32_t b = 0; b < T; b += 16) {
+        uint32_t nx[16];
+#pragma unroll
+        for (uint32_t k = 0; k < 16; ++k)
+            nx[k] = h[b + 16 + k];  // b < T, so b + 16 + k < T + 32: zeros past T
+#pragma unroll
+        for (uint32_t k = 0; k < 16; ++k) {
+            double df = (double)v[k] - avg;
+            double sq = df * df;
+            s = s + (v[k] ? sq : 0.0);
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < 16; ++k)
+            v[k] = nx[k];
+    }
+    sse[t] = s;
+    cnt[t] = nodes;
 }
 
 __global__ __launch_bounds__(256) void k_apply_keys(const uint32_t *__restrict__ key_t,
@@ -553,68 +666,112 @@ T d2h_scalar(const T *dptr, hipStream_t s)
         MSJ_HIP(hipGetLastError());                                                      \
     } while (0)
 
-// Host state of the reference-point generation of one join.
+// Host state of the reference-point generation of one build: the test
+// candidates and sigmas of a group of layers, each layer's reference points.
 struct RpHost {
-    std::vector<double> trp, sig, maxc, rp, rpT;
+    uint32_t nrp, d, l0 = 0, lc = 0;  // layers [l0, l0 + lc) are in trp / sig / rpT
+    std::vector<double> trp, sig, maxc, box, rp, rpT;
     std::vector<uint32_t> order;
     RpHost(uint32_t nrp, uint32_t d)
-        : trp((size_t)2 * nrp * d), sig(2 * nrp), maxc(d), rp((size_t)nrp * d),
-          rpT((size_t)d * KEY_RP), order(2 * nrp)
+        : nrp(nrp), d(d), maxc(d), box((size_t)2 * nrp * d), rp((size_t)nrp * d),
+          order(2 * nrp)
     {
     }
 };
 
-// utils.cu:66-137 createRPArray for one layer: 2*nrp random candidates from
-// the glibc stream, their sample-distance sigma and the sample's max corner
-// on the device, the host's stable sigma order and the box corners; the
-// layer's reference points end up transposed in B.rpT (d x KEY_RP).
-template <class T>
-void layer_rps(IndexBuilder &B, const T *xs, uint32_t n, uint32_t d, uint32_t S, uint32_t nrp,
-               GlibcRand &rng, RpHost &h, hipStream_t s)
+// Layers whose test candidates share one distance matrix (S x lc*2*nrp
+// doubles, at most about 256 MiB).
+inline uint32_t rp_group_layers(uint32_t S, uint32_t ntest, uint32_t left)
 {
-    const uint32_t ntest = nrp * 2;
+    const uint64_t per = (uint64_t)std::max<uint32_t>(S, 1) * ntest * sizeof(double);
+    const uint64_t fit = std::max<uint64_t>(1, (256ull << 20) / per);
+    return (uint32_t)std::min<uint64_t>(left, fit);
+}
+
+// utils.cu:66-137 createRPArray for layers [l0, l0 + lc).  The glibc stream's
+// draws do not depend on any layer's result, so the 2*nrp test candidates of
+// every layer in the group are drawn in stream order up front, their
+// sample-distance sigmas come from one distance matrix and one sigma launch
+// (one column per candidate, each summed sequentially as before), and the
+// sample's max corner, the same for every layer, from one reduction; one
+// round trip to the host then gives every layer's stable sigma order and box
+// corners.  Layer l's reference points end up transposed in
+// B.rpT + (l - l0) * d * KEY_RP.
+template <class T>
+void group_rps(IndexBuilder &B, const T *xs, uint32_t n, uint32_t d, uint32_t S, uint32_t nrp,
+               uint32_t l0, uint32_t lc, GlibcRand &rng, RpHost &h, hipStream_t s)
+{
+    const uint32_t ntest = nrp * 2, nt = lc * ntest;
+    h.l0 = l0;
+    h.lc = lc;
+    h.trp.resize((size_t)nt * d);
+    h.sig.resize(nt);
+    h.rpT.assign((size_t)lc * d * KEY_RP, 0.0);
     for (size_t i = 0; i < h.trp.size(); ++i)
         h.trp[i] = (double)rng.next() / (double)2147483647;
+    B.test_rp.ensure((size_t)nt * d);
+    B.distmat.ensure((size_t)std::max<uint32_t>(S, 1) * nt);
+    B.sigma.ensure(nt);
+    B.rpT.ensure((size_t)lc * d * KEY_RP);
     MSJ_HIP(hipMemcpyAsync(B.test_rp.p, h.trp.data(), h.trp.size() * sizeof(double),
                            hipMemcpyHostToDevice, s));
     if (S > 0)
-        LAUNCH(k_rp_distmat<T>, grid_for((size_t)S * ntest, 256), 256, xs, n, d, S, B.test_rp.p,
-               ntest, B.distmat.p);
-    LAUNCH(k_col_sigma, ntest, SIG_BLOCK, B.distmat.p, S, (size_t)ntest, (size_t)1, B.sigma.p);
-    LAUNCH(k_col_max<T>, d, 256, xs, n, d, S, B.maxc.p);
-    MSJ_HIP(hipMemcpyAsync(h.sig.data(), B.sigma.p, ntest * sizeof(double), hipMemcpyDeviceToHost,
+        LAUNCH(k_rp_distmat<T>, grid_for((size_t)S * nt, 256), 256, xs, n, d, S, B.test_rp.p, nt,
+               B.distmat.p);
+    LAUNCH(k_col_sigma, nt, SIG_BLOCK, B.distmat.p, S, (size_t)nt, (size_t)1, B.sigma.p);
+    if (l0 == 0) {
+        LAUNCH(k_col_max<T>, d, 256, xs, n, d, S, B.maxc.p);
+        MSJ_HIP(hipMemcpyAsync(h.maxc.data(), B.maxc.p, d * sizeof(double),
+                               hipMemcpyDeviceToHost, s));
+    }
+    MSJ_HIP(hipMemcpyAsync(h.sig.data(), B.sigma.p, nt * sizeof(double), hipMemcpyDeviceToHost,
                            s));
-    MSJ_HIP(hipMemcpyAsync(h.maxc.data(), B.maxc.p, d * sizeof(double), hipMemcpyDeviceToHost, s));
     MSJ_HIP(hipStreamSynchronize(s));
-    // thrust::sort_by_key (stable, IEEE radix order) ascending, reversed
-    for (uint32_t i = 0; i < ntest; ++i)
-        h.order[i] = i;
-    std::stable_sort(h.order.begin(), h.order.end(), [&](uint32_t a, uint32_t b) {
-        return radix_key_f64(h.sig[a]) < radix_key_f64(h.sig[b]);
-    });
-    std::reverse(h.order.begin(), h.order.end());
-    for (uint32_t i = 0; i < nrp / 2; ++i)
-        for (uint32_t j = 0; j < d; ++j)
-            h.rp[(size_t)i * d + j] = h.trp[(size_t)h.order[i] * d + j];
     // box points: max corner, then corner patterns (utils.cu:104-136)
     for (uint32_t j = 0; j < d; ++j)
-        h.trp[j] = h.maxc[j];
+        h.box[j] = h.maxc[j];
     const uint32_t step = host_cvt_u32(std::pow(2.0, (double)d) / ntest);
     for (uint32_t i = 1; i < ntest; ++i)
         for (uint32_t j = 0; j < d; ++j) {
             uint32_t div = (uint32_t)host_cvt_i32(std::pow(2.0, (double)j));
             uint32_t bit = ((i * step) / div) % 2u;
-            h.trp[(size_t)i * d + j] = (double)bit * h.trp[j];
+            h.box[(size_t)i * d + j] = (double)bit * h.box[j];
         }
-    for (uint32_t i = 0; i < nrp / 2 + nrp % 2; ++i)
-        for (uint32_t j = 0; j < d; ++j)
-            h.rp[(size_t)(nrp / 2 + i) * d + j] = h.trp[(size_t)i * d + j];
-    std::fill(h.rpT.begin(), h.rpT.end(), 0.0);
-    for (uint32_t t = 0; t < nrp; ++t)
-        for (uint32_t j = 0; j < d; ++j)
-            h.rpT[(size_t)j * KEY_RP + t] = h.rp[(size_t)t * d + j];
+    for (uint32_t g = 0; g < lc; ++g) {
+        const double *trp = h.trp.data() + (size_t)g * ntest * d;
+        const double *sig = h.sig.data() + (size_t)g * ntest;
+        // thrust::sort_by_key (stable, IEEE radix order) ascending, reversed
+        for (uint32_t i = 0; i < ntest; ++i)
+            h.order[i] = i;
+        std::stable_sort(h.order.begin(), h.order.end(), [&](uint32_t a, uint32_t b) {
+            return radix_key_f64(sig[a]) < radix_key_f64(sig[b]);
+        });
+        std::reverse(h.order.begin(), h.order.end());
+        for (uint32_t i = 0; i < nrp / 2; ++i)
+            for (uint32_t j = 0; j < d; ++j)
+                h.rp[(size_t)i * d + j] = trp[(size_t)h.order[i] * d + j];
+        for (uint32_t i = 0; i < nrp / 2 + nrp % 2; ++i)
+            for (uint32_t j = 0; j < d; ++j)
+                h.rp[(size_t)(nrp / 2 + i) * d + j] = h.box[(size_t)i * d + j];
+        double *rpT = h.rpT.data() + (size_t)g * d * KEY_RP;
+        for (uint32_t t = 0; t < nrp; ++t)
+            for (uint32_t j = 0; j < d; ++j)
+                rpT[(size_t)j * KEY_RP + t] = h.rp[(size_t)t * d + j];
+    }
     MSJ_HIP(hipMemcpyAsync(B.rpT.p, h.rpT.data(), h.rpT.size() * sizeof(double),
                            hipMemcpyHostToDevice, s));
+}
+
+// Layer l's transposed reference points (d x KEY_RP), drawing the next group
+// of layers first when l is past the current one.
+template <class T>
+const double *layer_rps(IndexBuilder &B, const T *xs, uint32_t n, uint32_t d, uint32_t S,
+                        uint32_t nrp, uint32_t l, uint32_t layers, GlibcRand &rng, RpHost &h,
+                        hipStream_t s)
+{
+    if (h.lc == 0 || l >= h.l0 + h.lc)
+        group_rps(B, xs, n, d, S, nrp, l, rp_group_layers(S, 2 * nrp, layers - l), rng, h, s);
+    return B.rpT.p + (size_t)(l - h.l0) * d * KEY_RP;
 }
 
 template <class T>
@@ -625,7 +782,6 @@ void build_impl(IndexBuilder &B, const T *xs, uint32_t n, uint32_t d, double eps
     const uint32_t naxis = std::min(p.numidim, d);
     const uint32_t ncand = nrp + naxis;
     const uint32_t layers = std::min<uint32_t>(p.maxrp, MSJ_MAX_LAYERS);
-    const uint32_t ntest = nrp * 2;
     // (unsigned)(numPoints * SAMPLE_PER), utils.cu:68
     const uint32_t S = host_cvt_u32((double)(unsigned long long)n * p.sample_per);
     st.ncand = ncand;
@@ -635,11 +791,7 @@ void build_impl(IndexBuilder &B, const T *xs, uint32_t n, uint32_t d, double eps
     rng.seed(p.seed);
 
     B.keys.ensure((size_t)ncand * n);
-    B.test_rp.ensure((size_t)ntest * d);
-    B.distmat.ensure((size_t)std::max<uint32_t>(S, 1) * ntest);
-    B.sigma.ensure(ntest);
     B.maxc.ensure(d);
-    B.rpT.ensure((size_t)d * KEY_RP);
     B.u32scalar.ensure(1);
     B.u64scalar.ensure(1);
     out.perm.ensure(n);
@@ -657,13 +809,17 @@ void build_impl(IndexBuilder &B, const T *xs, uint32_t n, uint32_t d, double eps
 
     RpHost rh(nrp, d);
     std::vector<double> h_sse;
+    // MSJ_NODE_SSE=sort keeps every layer on the composite-key sort (tests
+    // compare the two forms); otherwise layers whose bins fit use the dense one.
+    const char *sse_env = std::getenv("MSJ_NODE_SSE");
+    const bool sse_by_sort = sse_env && std::strcmp(sse_env, "sort") == 0;
 
     for (uint32_t l = 0; l < layers; ++l) {
         // ---- (1) reference points, utils.cu:66-137
-        layer_rps(B, xs, n, d, S, nrp, rng, rh, s);
+        const double *rpT = layer_rps(B, xs, n, d, S, nrp, l, layers, rng, rh, s);
 
         // ---- (2) keys of every candidate partition
-        LAUNCH(k_keys<T>, grid_for(n, 256), 256, xs, n, d, B.rpT.p, nrp, naxis, eps, B.keys.p);
+        LAUNCH(k_keys<T>, grid_for(n, 256), 256, xs, n, d, rpT, nrp, naxis, eps, B.keys.p);
         if (B.probe && B.probe->keys && B.probe->layer == l) {
             MSJ_HIP(hipMemcpyAsync(B.probe->keys, B.keys.p, sizeof(uint32_t) * ncand * n,
                                    hipMemcpyDeviceToHost, s));
@@ -685,7 +841,22 @@ void build_impl(IndexBuilder &B, const T *xs, uint32_t n, uint32_t d, double eps
         B.sse.ensure(ncand);
         B.cnt_nodes.ensure(ncand);
         B.runfirst.ensure((size_t)ncand + 1);
-        for (uint32_t t0 = 0; t0 < ncand; t0 += group) {
+        const uint64_t K = (uint64_t)kmax + 1, nbins = (uint64_t)mold * K;
+        const bool dense = nbins <= DENSE_BINS && !sse_by_sort;
+        if (dense) {
+            const size_t tabn = (size_t)ncand * nbins;
+            B.code.ensure(n);
+            B.tab.ensure(tabn);
+            LAUNCH(k_point_bins, grid_for(n, 256), 256, out.perm.p, B.pnode.p, B.nsplit.p, n,
+                   (uint32_t)K, B.code.p);
+            LAUNCH(k_fill<uint32_t>, grid_for(tabn, 256), 256, B.tab.p, tabn, 0u);
+            const dim3 hg((n + DENSE_CHUNK - 1) / DENSE_CHUNK, ncand);
+            hipLaunchKernelGGL(k_dense_hist, hg, dim3(256), 0, s, B.keys.p, B.code.p, n,
+                               (uint32_t)nbins, B.tab.p);
+            MSJ_HIP(hipGetLastError());
+            LAUNCH(k_dense_sse, ncand, 256, B.tab.p, (uint32_t)nbins, n, B.sse.p, B.cnt_nodes.p);
+        }
+        for (uint32_t t0 = 0; !dense && t0 < ncand; t0 += group) {
             const uint32_t tc = std::min(group, ncand - t0);
             const size_t m = (size_t)tc * n;
             const int tb = bits_for(tc - 1);
@@ -1073,7 +1244,6 @@ void build_coss_impl(IndexBuilder &B, const T *xs, uint32_t n, uint32_t d, doubl
     const uint32_t L = std::min<uint32_t>(p.maxrp, MSJ_MAX_LAYERS);
     const uint32_t naxis = std::min(L, d);  // maxRP axes (tree.cu:278), j < d
     const uint32_t ncand = nrp + naxis;
-    const uint32_t ntest = nrp * 2;
     const uint32_t S = host_cvt_u32((double)(unsigned long long)n * p.sample_per);
     st = LayerStats{};
     st.ncand = ncand;
@@ -1082,11 +1252,7 @@ void build_coss_impl(IndexBuilder &B, const T *xs, uint32_t n, uint32_t d, doubl
     GlibcRand rng;
     rng.seed(p.seed);
     B.keys.ensure((size_t)ncand * n);
-    B.test_rp.ensure((size_t)ntest * d);
-    B.distmat.ensure((size_t)std::max<uint32_t>(S, 1) * ntest);
-    B.sigma.ensure(ntest);
     B.maxc.ensure(d);
-    B.rpT.ensure((size_t)d * KEY_RP);
     B.cmm.ensure(2 * (size_t)ncand);
     B.cslots.ensure(4 * (size_t)ncand);
     B.cacc.ensure(ncand);
@@ -1122,8 +1288,8 @@ void build_coss_impl(IndexBuilder &B, const T *xs, uint32_t n, uint32_t d, doubl
 
     for (uint32_t cl = 0; cl < L; ++cl) {
         // ---- candidates and their keys (createRPArray, binningKernel, tree.cu:284)
-        layer_rps(B, xs, n, d, S, nrp, rng, rh, s);
-        LAUNCH((k_keys<T, true>), grid_for(n, 256), 256, xs, n, d, B.rpT.p, nrp, naxis, eps,
+        const double *rpT = layer_rps(B, xs, n, d, S, nrp, cl, L, rng, rh, s);
+        LAUNCH((k_keys<T, true>), grid_for(n, 256), 256, xs, n, d, rpT, nrp, naxis, eps,
                B.keys.p);
         if (B.probe && B.probe->keys && B.probe->layer == cl) {
             MSJ_HIP(hipMemcpyAsync(B.probe->keys, B.keys.p, sizeof(uint32_t) * ncand * n,

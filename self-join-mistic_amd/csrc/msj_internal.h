@@ -285,6 +285,7 @@ struct IndexBuilder {
     DevBuf<double> test_rp, distmat, sigma, maxc, rpT;
     DevBuf<uint64_t> ck;         // composite keys
     DevBuf<uint32_t> flags, runidx, starts, runfirst, cnt_nodes;
+    DevBuf<uint32_t> code, tab;  // dense candidate bins: point -> parent's first bin, counts
     DevBuf<double> sse;
     DevBuf<uint32_t> vals, pnode, pnode2, perm2;
     DevBuf<uint8_t> nsplit, nsplit2;

@@ -107,6 +107,29 @@ def test_context_reuse_two_eps(msj_mod, oracle_mod):
             _assert_same(got, ref)
 
 
+@pytest.mark.parametrize("gen,n,d,eps,seed", [
+    ("uniform", 65536, 2, 0.01, 1001),
+    ("mixture", 20000, 18, 0.12, 13),
+    ("correlated", 5000, 90, 1.14, 22),
+])
+def test_node_sizes_dense_and_sort(gen, n, d, eps, seed, msj_mod, monkeypatch):
+    """Candidate node sizes from the dense (parent, key) bins and from the
+    composite-key sort (MSJ_NODE_SSE=sort) give the same choices, SSE bits,
+    node counts and node order in every layer."""
+    x = _prep(getattr(datagen, gen)(n, d, seed), msj_mod)
+    _announce("dense vs sort node sizes, " + gen, x, eps)
+    a = msj_mod.self_join(x, eps)
+    monkeypatch.setenv("MSJ_NODE_SSE", "sort")
+    b = msj_mod.self_join(x, eps)
+    assert a.total_pairs == b.total_pairs
+    assert np.array_equal(a.node_order, b.node_order)
+    assert np.array_equal(a.offsets, b.offsets)
+    assert np.array_equal(a.nbrs, b.nbrs)
+    for k in ("num_nodes", "num_calcs", "num_layers", "chosen", "nodes_per_layer",
+              "calcs_per_layer", "sse"):
+        assert a.stats[k] == b.stats[k], k
+
+
 def test_empty_input(msj_mod):
     x = np.zeros((0, 5))
     _announce("empty", x, 0.1)
