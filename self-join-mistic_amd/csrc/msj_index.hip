@@ -111,10 +111,12 @@ __global__ void k_fill(T *p, size_t n, T v)
 }
 
 // utils.cu:84-89: distmat[i*test_rp + j] = sqrt(sum (rp - x)^2), host order.
+// The candidates come transposed (trpT[k * ntest + j]), so the lanes of a
+// wave, consecutive candidates j of one sample row i, read consecutive words.
 template <class T>
 __global__ __launch_bounds__(256) void k_rp_distmat(const T *__restrict__ xs, uint32_t n,
                                                     uint32_t d, uint32_t S,
-                                                    const double *__restrict__ trp,
+                                                    const double *__restrict__ trpT,
                                                     uint32_t ntest, double *__restrict__ distmat)
 {
     size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -123,7 +125,7 @@ __global__ __launch_bounds__(256) void k_rp_distmat(const T *__restrict__ xs, ui
     uint32_t i = (uint32_t)(idx / ntest), j = (uint32_t)(idx % ntest);
     double dist = 0;
     for (uint32_t k = 0; k < d; ++k) {
-        double diff = trp[(size_t)j * d + k] - (double)xs[(size_t)k * n + i];
+        double diff = trpT[(size_t)k * ntest + j] - (double)xs[(size_t)k * n + i];
         double sq = diff * diff;
         dist = dist + sq;
     }
@@ -258,37 +260,62 @@ __global__ __launch_bounds__(256) void k_col_max(const T *__restrict__ xs, uint3
 
 // kernel.cu:880-909 (32 RP keys) + nodes.cu:128-135 (axis keys).  One thread
 // per point reads its column once; RP coordinates are wave-uniform (scalar
-// loads of rpT[j][0..31]).
+// loads of rpT[j][0..31]).  With kmax given (zeroed before), the largest key
+// is folded into it: one atomic per workgroup at most, none when the
+// workgroup's maximum does not exceed what kmax already holds.
 // COSS: the tree's axis keys instead (tree.cu:284).
 template <class T, bool COSS = false>
 __global__ __launch_bounds__(256) void k_keys(const T *__restrict__ xs, uint32_t n, uint32_t d,
                                               const double *__restrict__ rpT, uint32_t nrp,
                                               uint32_t naxis, double eps,
-                                              uint32_t *__restrict__ keys)
+                                              uint32_t *__restrict__ keys,
+                                              uint32_t *__restrict__ kmax)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
-        return;
-    double acc[KEY_RP];
+    __shared__ uint32_t part[256 / WAVE];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t m = 0;
+    if (i < n) {
+        double acc[KEY_RP];
 #pragma unroll
-    for (int t = 0; t < KEY_RP; ++t)
-        acc[t] = 0.0;
-    for (uint32_t j = 0; j < d; ++j) {
-        const double x = (double)xs[(size_t)j * n + i];
-        const double *r = rpT + (size_t)j * KEY_RP;
+        for (int t = 0; t < KEY_RP; ++t)
+            acc[t] = 0.0;
+        for (uint32_t j = 0; j < d; ++j) {
+            const double x = (double)xs[(size_t)j * n + i];
+            const double *r = rpT + (size_t)j * KEY_RP;
 #pragma unroll
-        for (int t = 0; t < KEY_RP; ++t) {
-            double df = x - r[t];
-            acc[t] = fma(df, df, acc[t]);
+            for (int t = 0; t < KEY_RP; ++t) {
+                double df = x - r[t];
+                acc[t] = fma(df, df, acc[t]);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < KEY_RP; ++t)
+            if ((uint32_t)t < nrp) {
+                const uint32_t k = rp_key(acc[t], eps);
+                keys[(size_t)t * n + i] = k;
+                m = max(m, k);
+            }
+        for (uint32_t a = 0; a < naxis; ++a) {
+            const uint32_t k = COSS ? coss_axis_key((double)xs[(size_t)a * n + i], eps)
+                                    : axis_key((double)xs[(size_t)a * n + i], eps);
+            keys[(size_t)(nrp + a) * n + i] = k;
+            m = max(m, k);
         }
     }
+    if (!kmax)
+        return;
 #pragma unroll
-    for (int t = 0; t < KEY_RP; ++t)
-        if ((uint32_t)t < nrp)
-            keys[(size_t)t * n + i] = rp_key(acc[t], eps);
-    for (uint32_t a = 0; a < naxis; ++a)
-        keys[(size_t)(nrp + a) * n + i] = COSS ? coss_axis_key((double)xs[(size_t)a * n + i], eps)
-                                               : axis_key((double)xs[(size_t)a * n + i], eps);
+    for (int o = WAVE / 2; o > 0; o >>= 1)
+        m = max(m, (uint32_t)__shfl_down(m, o, WAVE));
+    if ((threadIdx.x & (WAVE - 1)) == 0)
+        part[threadIdx.x / WAVE] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (uint32_t w = 1; w < blockDim.x / WAVE; ++w)
+            m = max(m, part[w]);
+        if (m > __hip_atomic_load(kmax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(kmax, m);
+    }
 }
 
 // Composite (candidate | parent node | key) for candidates [t0, t0+tcount).
@@ -474,7 +501,25 @@ __global__ __launch_bounds__(256) void k_dense_sse(const uint32_t *__restrict__ 
     cnt[t] = nodes;
 }
 
-__global__ __launch_bounds__(256) void k_apply_keys(const uint32_t *__restrict__ key_t,
+// nodes.cu:198,207: the candidate with the smallest SSE, the lowest index on
+// a tie (strict '<'); kept per layer for the statistics read after the last
+// layer, so the host never waits for it.
+__global__ void k_pick_best(const double *__restrict__ sse, uint32_t ncand,
+                            uint32_t *__restrict__ best, double *__restrict__ best_sse)
+{
+    if (threadIdx.x != 0)
+        return;
+    uint32_t b = 0;
+    for (uint32_t t = 1; t < ncand; ++t)
+        if (sse[t] < sse[b])
+            b = t;
+    *best = b;
+    *best_sse = sse[b];
+}
+
+// Keys of the winner *best (candidate rows of `keys`, n each).
+__global__ __launch_bounds__(256) void k_apply_keys(const uint32_t *__restrict__ keys,
+                                                    const uint32_t *__restrict__ best,
                                                     const uint32_t *__restrict__ perm,
                                                     const uint32_t *__restrict__ pnode,
                                                     const uint8_t *__restrict__ nsplit, uint32_t n,
@@ -484,6 +529,7 @@ __global__ __launch_bounds__(256) void k_apply_keys(const uint32_t *__restrict__
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
+    const uint32_t *key_t = keys + (size_t)(*best) * n;
     uint32_t node = pnode[i];
     uint32_t row = perm[i];
     uint64_t k = nsplit[node] ? key_t[row] : 0u;
@@ -670,7 +716,7 @@ T d2h_scalar(const T *dptr, hipStream_t s)
 // candidates and sigmas of a group of layers, each layer's reference points.
 struct RpHost {
     uint32_t nrp, d, l0 = 0, lc = 0;  // layers [l0, l0 + lc) are in trp / sig / rpT
-    std::vector<double> trp, sig, maxc, box, rp, rpT;
+    std::vector<double> trp, trpT, sig, maxc, box, rp, rpT;
     std::vector<uint32_t> order;
     RpHost(uint32_t nrp, uint32_t d)
         : nrp(nrp), d(d), maxc(d), box((size_t)2 * nrp * d), rp((size_t)nrp * d),
@@ -709,11 +755,15 @@ void group_rps(IndexBuilder &B, const T *xs, uint32_t n, uint32_t d, uint32_t S,
     h.rpT.assign((size_t)lc * d * KEY_RP, 0.0);
     for (size_t i = 0; i < h.trp.size(); ++i)
         h.trp[i] = (double)rng.next() / (double)2147483647;
+    h.trpT.resize(h.trp.size());
+    for (uint32_t j = 0; j < nt; ++j)
+        for (uint32_t k = 0; k < d; ++k)
+            h.trpT[(size_t)k * nt + j] = h.trp[(size_t)j * d + k];
     B.test_rp.ensure((size_t)nt * d);
     B.distmat.ensure((size_t)std::max<uint32_t>(S, 1) * nt);
     B.sigma.ensure(nt);
     B.rpT.ensure((size_t)lc * d * KEY_RP);
-    MSJ_HIP(hipMemcpyAsync(B.test_rp.p, h.trp.data(), h.trp.size() * sizeof(double),
+    MSJ_HIP(hipMemcpyAsync(B.test_rp.p, h.trpT.data(), h.trpT.size() * sizeof(double),
                            hipMemcpyHostToDevice, s));
     if (S > 0)
         LAUNCH(k_rp_distmat<T>, grid_for((size_t)S * nt, 256), 256, xs, n, d, S, B.test_rp.p, nt,
@@ -791,9 +841,13 @@ void build_impl(IndexBuilder &B, const T *xs, uint32_t n, uint32_t d, double eps
     rng.seed(p.seed);
 
     B.keys.ensure((size_t)ncand * n);
+    if (layers > 1)
+        B.keys2.ensure((size_t)ncand * n);
     B.maxc.ensure(d);
-    B.u32scalar.ensure(1);
-    B.u64scalar.ensure(1);
+    B.u32scalar.ensure(std::max<uint32_t>(layers, 1));
+    B.u64scalar.ensure(std::max<uint32_t>(layers, 1));
+    B.best.ensure(std::max<uint32_t>(layers, 1));
+    B.best_sse.ensure(std::max<uint32_t>(layers, 1));
     out.perm.ensure(n);
     B.vals.ensure(n);
     B.pnode.ensure(n);
@@ -808,25 +862,35 @@ void build_impl(IndexBuilder &B, const T *xs, uint32_t n, uint32_t d, double eps
     uint32_t mold = 1;
 
     RpHost rh(nrp, d);
-    std::vector<double> h_sse;
     // MSJ_NODE_SSE=sort keeps every layer on the composite-key sort (tests
     // compare the two forms); otherwise layers whose bins fit use the dense one.
     const char *sse_env = std::getenv("MSJ_NODE_SSE");
     const bool sse_by_sort = sse_env && std::strcmp(sse_env, "sort") == 0;
 
-    for (uint32_t l = 0; l < layers; ++l) {
-        // ---- (1) reference points, utils.cu:66-137
+    // ---- (1) reference points, utils.cu:66-137, and (2) the keys of every
+    // candidate partition of layer l into `dst`, the largest key into
+    // u32scalar[l].  Keys do not depend on the node net: layer l + 1's are
+    // computed during layer l, and its largest key comes back with layer l's
+    // adjacency size.
+    LAUNCH(k_fill<uint32_t>, 1, 64, B.u32scalar.p, (size_t)std::max<uint32_t>(layers, 1), 0u);
+    auto layer_keys = [&](uint32_t l, uint32_t *dst) {
         const double *rpT = layer_rps(B, xs, n, d, S, nrp, l, layers, rng, rh, s);
-
-        // ---- (2) keys of every candidate partition
-        LAUNCH(k_keys<T>, grid_for(n, 256), 256, xs, n, d, rpT, nrp, naxis, eps, B.keys.p);
+        LAUNCH(k_keys<T>, grid_for(n, 256), 256, xs, n, d, rpT, nrp, naxis, eps, dst,
+               B.u32scalar.p + l);
         if (B.probe && B.probe->keys && B.probe->layer == l) {
-            MSJ_HIP(hipMemcpyAsync(B.probe->keys, B.keys.p, sizeof(uint32_t) * ncand * n,
+            MSJ_HIP(hipMemcpyAsync(B.probe->keys, dst, sizeof(uint32_t) * ncand * n,
                                    hipMemcpyDeviceToHost, s));
             MSJ_HIP(hipStreamSynchronize(s));
         }
-        reduce_max_u32(B.keys.p, (size_t)ncand * n, B.u32scalar.p, s);
-        const uint32_t kmax = d2h_scalar(B.u32scalar.p, s);
+    };
+    uint32_t kmax_next = 0;
+    if (layers) {
+        layer_keys(0, B.keys.p);
+        kmax_next = d2h_scalar(B.u32scalar.p, s);
+    }
+
+    for (uint32_t l = 0; l < layers; ++l) {
+        const uint32_t kmax = kmax_next;
         const int kb = bits_for(kmax);
         const int nb = bits_for(mold - 1);
 
@@ -837,7 +901,6 @@ void build_impl(IndexBuilder &B, const T *xs, uint32_t n, uint32_t d, double eps
             int room = 64 - nb - kb;  // >= 0 since nb, kb <= 32
             group = room >= 31 ? ncand : std::max<uint32_t>(1u, std::min<uint32_t>(ncand, 1u << room));
         }
-        h_sse.assign(ncand, 0.0);
         B.sse.ensure(ncand);
         B.cnt_nodes.ensure(ncand);
         B.runfirst.ensure((size_t)ncand + 1);
@@ -874,19 +937,13 @@ void build_impl(IndexBuilder &B, const T *xs, uint32_t n, uint32_t d, double eps
             LAUNCH(k_cand_sse, 1, 64 * ((tc + 63) / 64), B.runfirst.p, B.starts.p, tc, n,
                    B.sse.p + t0, B.cnt_nodes.p + t0);
         }
-        MSJ_HIP(hipMemcpyAsync(h_sse.data(), B.sse.p, ncand * sizeof(double),
-                               hipMemcpyDeviceToHost, s));
-        MSJ_HIP(hipStreamSynchronize(s));
-        uint32_t best = 0;  // strict '<': the lowest index wins a tie
-        for (uint32_t t = 1; t < ncand; ++t)
-            if (h_sse[t] < h_sse[best])
-                best = t;
+        LAUNCH(k_pick_best, 1, 64, B.sse.p, ncand, B.best.p + l, B.best_sse.p + l);
 
         // ---- (4) apply the winner
         B.ck.ensure(n);
         B.flags.ensure(n);
         B.runidx.ensure((size_t)n + 1);
-        LAUNCH(k_apply_keys, grid_for(n, 256), 256, B.keys.p + (size_t)best * n, out.perm.p,
+        LAUNCH(k_apply_keys, grid_for(n, 256), 256, B.keys.p, B.best.p + l, out.perm.p,
                B.pnode.p, B.nsplit.p, n, kb, B.ck.p, B.vals.p);
         B.sorter.sort(B.ck.p, B.vals.p, n, nb + kb, s);
         LAUNCH(k_run_flags, grid_for(n, 256), 256, B.ck.p, (size_t)n, B.flags.p);
@@ -920,7 +977,15 @@ void build_impl(IndexBuilder &B, const T *xs, uint32_t n, uint32_t d, double eps
             // degree prefix in 64 bits first, to detect > 2^32 adjacency entries
             B.tmp64.ensure((size_t)mnew + 1);
             B.scan.excl_u32_to_u64(B.deg.p, B.tmp64.p, mnew, s);
-            const uint64_t edges = d2h_scalar(B.tmp64.p + mnew, s);
+            if (l + 1 < layers)
+                layer_keys(l + 1, B.keys2.p);
+            uint64_t edges;
+            MSJ_HIP(hipMemcpyAsync(&edges, B.tmp64.p + mnew, sizeof(edges),
+                                   hipMemcpyDeviceToHost, s));
+            if (l + 1 < layers)
+                MSJ_HIP(hipMemcpyAsync(&kmax_next, B.u32scalar.p + l + 1, sizeof(kmax_next),
+                                       hipMemcpyDeviceToHost, s));
+            MSJ_HIP(hipStreamSynchronize(s));
             if (edges >= 0xFFFFFFFFull)
                 throw Error(MSJ_E_OVERFLOW, "node adjacency exceeds 2^32 entries");
             B.scan.excl_u32_to_u32(B.deg.p, B.aoff2.p, mnew, s);
@@ -931,23 +996,38 @@ void build_impl(IndexBuilder &B, const T *xs, uint32_t n, uint32_t d, double eps
         B.nsplit2.ensure(mnew);
         LAUNCH(k_nsplit, grid_for(mnew, 256), 256, B.ncalcs2.p, mnew, p.min_calcs_per_node,
                B.nsplit2.p);
-        reduce_sum_u64(B.ncalcs2.p, mnew, B.u64scalar.p, s);
-        const uint64_t calcs = d2h_scalar(B.u64scalar.p, s);
+        // the layer's candidate evaluations, read back once after the last layer
+        reduce_sum_u64(B.ncalcs2.p, mnew, B.u64scalar.p + l, s);
 
         swap_buf(out.perm, B.vals);
         swap_buf(B.pnode, B.pnode2);
         swap_buf(B.nsplit, B.nsplit2);
         swap_buf(B.aoff1, B.aoff2);
         swap_buf(B.adj1, B.adj2);
+        swap_buf(B.keys, B.keys2);
         mold = mnew;
-        st.chosen[l] = best;
-        st.sse[l] = h_sse[best];
         st.nodes[l] = mnew;
-        st.calcs[l] = calcs;
-        out.num_calcs = calcs;
-        if (p.verbose)
-            printf("Layer %u Selecting RP %u with Nodes: %u and calcs: %llu and stdev %f\n", l,
-                   best, mnew, (unsigned long long)calcs, h_sse[best]);
+    }
+    if (layers) {
+        std::vector<unsigned long long> calcs(layers);
+        std::vector<uint32_t> best(layers);
+        std::vector<double> best_sse(layers);
+        MSJ_HIP(hipMemcpyAsync(calcs.data(), B.u64scalar.p, sizeof(unsigned long long) * layers,
+                               hipMemcpyDeviceToHost, s));
+        MSJ_HIP(hipMemcpyAsync(best.data(), B.best.p, sizeof(uint32_t) * layers,
+                               hipMemcpyDeviceToHost, s));
+        MSJ_HIP(hipMemcpyAsync(best_sse.data(), B.best_sse.p, sizeof(double) * layers,
+                               hipMemcpyDeviceToHost, s));
+        MSJ_HIP(hipStreamSynchronize(s));
+        for (uint32_t l = 0; l < layers; ++l) {
+            st.chosen[l] = best[l];
+            st.sse[l] = best_sse[l];
+            st.calcs[l] = calcs[l];
+            if (p.verbose)
+                printf("Layer %u Selecting RP %u with Nodes: %u and calcs: %llu and stdev %f\n",
+                       l, st.chosen[l], st.nodes[l], calcs[l], st.sse[l]);
+        }
+        out.num_calcs = calcs[layers - 1];
     }
     out.M = mold;
     swap_buf(out.nstart, B.cstart);
@@ -1290,7 +1370,7 @@ void build_coss_impl(IndexBuilder &B, const T *xs, uint32_t n, uint32_t d, doubl
         // ---- candidates and their keys (createRPArray, binningKernel, tree.cu:284)
         const double *rpT = layer_rps(B, xs, n, d, S, nrp, cl, L, rng, rh, s);
         LAUNCH((k_keys<T, true>), grid_for(n, 256), 256, xs, n, d, rpT, nrp, naxis, eps,
-               B.keys.p);
+               B.keys.p, (uint32_t *)nullptr);
         if (B.probe && B.probe->keys && B.probe->layer == cl) {
             MSJ_HIP(hipMemcpyAsync(B.probe->keys, B.keys.p, sizeof(uint32_t) * ncand * n,
                                    hipMemcpyDeviceToHost, s));

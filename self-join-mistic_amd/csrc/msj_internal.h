@@ -125,8 +125,6 @@ struct RadixSorter {
 
 // Sum of a u64 array into *dst (device), *dst cleared first.
 void reduce_sum_u64(const uint64_t *in, size_t n, unsigned long long *dst, hipStream_t s);
-// Max of a u32 array into *dst (device), *dst cleared first.
-void reduce_max_u32(const uint32_t *in, size_t n, uint32_t *dst, hipStream_t s);
 
 inline unsigned grid_for(size_t n, unsigned block)
 {
@@ -281,12 +279,13 @@ __host__ __device__ inline double repeat_add(double s, double c, uint64_t g)
 struct IndexBuilder {
     IndexProbe *probe = nullptr;
     // per-layer scratch
-    DevBuf<uint32_t> keys;       // [ncand*n] candidate keys, row order
+    DevBuf<uint32_t> keys, keys2;  // [ncand*n] candidate keys, row order (this, next layer)
     DevBuf<double> test_rp, distmat, sigma, maxc, rpT;
     DevBuf<uint64_t> ck;         // composite keys
     DevBuf<uint32_t> flags, runidx, starts, runfirst, cnt_nodes;
     DevBuf<uint32_t> code, tab;  // dense candidate bins: point -> parent's first bin, counts
-    DevBuf<double> sse;
+    DevBuf<double> sse, best_sse;
+    DevBuf<uint32_t> best;       // per layer: the winning candidate
     DevBuf<uint32_t> vals, pnode, pnode2, perm2;
     DevBuf<uint8_t> nsplit, nsplit2;
     DevBuf<uint32_t> cparent, ckey, cstart, cfirst, deg, aoff2, adj2, aoff1, adj1;

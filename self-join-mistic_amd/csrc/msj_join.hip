@@ -456,25 +456,36 @@ __global__ __launch_bounds__(256) void k_pca_proj(const T *__restrict__ xs, uint
         P[(size_t)k * n + row] = p[k];
 }
 
-// Top PCA_K eigenvectors of the symmetric dp x dp matrix C (upper triangle
-// filled) by power iteration with deflation; false if C is degenerate.
-bool top_directions(std::vector<double> &C, uint32_t dp, std::vector<double> &V)
+// Top K (<= PCA_K) eigenvectors of the symmetric dp x dp matrix C (upper
+// triangle filled) by power iteration with deflation, a fixed 32 steps per
+// direction: the directions only order the rows, so they need not be exact,
+// only the same from run to run.  False if C is degenerate.
+bool top_directions(std::vector<double> &C, uint32_t dp, uint32_t K, std::vector<double> &V)
 {
     for (uint32_t a = 0; a < dp; ++a)
         for (uint32_t b = 0; b < a; ++b)
             C[(size_t)a * dp + b] = C[(size_t)b * dp + a];
     V.assign((size_t)PCA_K * dp, 0.0);
     std::vector<double> v(dp), w(dp);
-    for (uint32_t k = 0; k < PCA_K; ++k) {
+    for (uint32_t k = 0; k < K; ++k) {
         for (uint32_t j = 0; j < dp; ++j)
             v[j] = 1.0 + 0.001 * j + (j == k ? 1.0 : 0.0);
         double lam = 0.0;
-        for (int it = 0; it < 100; ++it) {
+        for (int it = 0; it < 32; ++it) {
             double nn = 0.0;
             for (uint32_t a = 0; a < dp; ++a) {
-                double t = 0.0;
-                for (uint32_t b = 0; b < dp; ++b)
-                    t += C[(size_t)a * dp + b] * v[b];
+                const double *c = &C[(size_t)a * dp];
+                double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
+                uint32_t b = 0;
+                for (; b + 4 <= dp; b += 4) {
+                    t0 += c[b] * v[b];
+                    t1 += c[b + 1] * v[b + 1];
+                    t2 += c[b + 2] * v[b + 2];
+                    t3 += c[b + 3] * v[b + 3];
+                }
+                for (; b < dp; ++b)
+                    t0 += c[b] * v[b];
+                const double t = (t0 + t1) + (t2 + t3);
                 w[a] = t;
                 nn += t * t;
             }
@@ -628,7 +639,8 @@ void JoinRunner::order(const Index &ix, const void *xs, bool f32, uint32_t n, ui
         MSJ_HIP(hipMemcpyAsync(hC.data(), pca.p + oC, sizeof(double) * hC.size(),
                                hipMemcpyDeviceToHost, s));
         MSJ_HIP(hipStreamSynchronize(s));
-        if (top_directions(hC, dp, hV)) {
+        const uint32_t K = d > 128 ? 6u : 3u;
+        if (top_directions(hC, dp, K, hV)) {
             MSJ_HIP(hipMemcpyAsync(pca.p + oV, hV.data(), sizeof(double) * hV.size(),
                                    hipMemcpyHostToDevice, s));
             if (f32)
@@ -637,7 +649,6 @@ void JoinRunner::order(const Index &ix, const void *xs, bool f32, uint32_t n, ui
             else
                 LAUNCH(k_pca_proj<double>, grid_for(n, 256), 256, (const double *)xs, n, dp,
                        pca.p + oMu, pca.p + oV, pca.p + oP);
-            const uint32_t K = d > 128 ? 6u : 3u;
             hipLaunchKernelGGL(k_col_range<double>, dim3(crb, K), dim3(256), 0, s,
                                (const double *)(pca.p + oP), n, mm.p + 2 * PCA_K);
             MSJ_HIP(hipGetLastError());

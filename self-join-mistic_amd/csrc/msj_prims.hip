@@ -274,38 +274,6 @@ __global__ __launch_bounds__(256) void k_reduce_sum_u64(const uint64_t *__restri
         atomicAdd(dst, (unsigned long long)s);
 }
 
-// Maximum of n values into *dst (zeroed first).  One atomic per workgroup at
-// most, and none when the workgroup's maximum does not exceed what *dst
-// already holds: a value read from *dst is never above its final value, so
-// skipping then loses nothing, and the single address sees few atomics.
-__global__ __launch_bounds__(256) void k_reduce_max_u32(const uint32_t *__restrict__ in, size_t n,
-                                                        uint32_t *dst)
-{
-    __shared__ uint32_t part[256 / WAVE];
-    uint32_t m = 0;
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    const size_t n4 = ((uintptr_t)in & 15) ? 0 : n / 4;
-    const uint4 *in4 = reinterpret_cast<const uint4 *>(in);
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
-        const uint4 v = in4[i];
-        m = max(m, max(max(v.x, v.y), max(v.z, v.w)));
-    }
-    for (size_t i = 4 * n4 + (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-        m = max(m, in[i]);
-#pragma unroll
-    for (int o = WAVE / 2; o > 0; o >>= 1)
-        m = max(m, (uint32_t)__shfl_down(m, o, WAVE));
-    if ((threadIdx.x & (WAVE - 1)) == 0)
-        part[threadIdx.x / WAVE] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int w = 1; w < 256 / WAVE; ++w)
-            m = max(m, part[w]);
-        if (m > __hip_atomic_load(dst, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-            atomicMax(dst, m);
-    }
-}
-
 }  // namespace
 
 void Scanner::excl_u32_to_u32(const uint32_t *in, uint32_t *out, size_t n, hipStream_t s)
@@ -362,16 +330,6 @@ void reduce_sum_u64(const uint64_t *in, size_t n, unsigned long long *dst, hipSt
     if (g > 2048)
         g = 2048;
     hipLaunchKernelGGL(k_reduce_sum_u64, dim3(g), dim3(256), 0, s, in, n, dst);
-    MSJ_HIP(hipGetLastError());
-}
-
-void reduce_max_u32(const uint32_t *in, size_t n, uint32_t *dst, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_fill<uint32_t>, dim3(1), dim3(64), 0, s, dst, (size_t)1, 0u);
-    unsigned g = grid_for(n, 256);
-    if (g > 2048)
-        g = 2048;
-    hipLaunchKernelGGL(k_reduce_max_u32, dim3(g), dim3(256), 0, s, in, n, dst);
     MSJ_HIP(hipGetLastError());
 }
 
