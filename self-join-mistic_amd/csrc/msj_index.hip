@@ -258,12 +258,18 @@ __global__ __launch_bounds__(256) void k_col_max(const T *__restrict__ xs, uint3
     }
 }
 
-// kernel.cu:880-909 (32 RP keys) + nodes.cu:128-135 (axis keys).  One thread
-// per point reads its column once; RP coordinates are wave-uniform (scalar
-// loads of rpT[j][0..31]).  With kmax given (zeroed before), the largest key
-// is folded into it: one atomic per workgroup at most, none when the
-// workgroup's maximum does not exceed what kmax already holds.
+// kernel.cu:880-909 (RP keys) + nodes.cu:128-135 (axis keys).  Thread
+// (i, blockIdx.y) keeps the distances of point i to reference points
+// [KEY_RPT * y, KEY_RPT * y + KEY_RPT) in registers and reads the point's
+// column once; RP coordinates are wave-uniform (scalar loads of rpT[j][..]).
+// Half of the 32 accumulators per thread keeps enough waves resident to
+// hide the loads.  The y = 0 threads also write the axis keys.  With kmax
+// given (zeroed before), the largest key is folded into it: one atomic per
+// workgroup at most, none when the workgroup's maximum does not exceed what
+// kmax already holds.
 // COSS: the tree's axis keys instead (tree.cu:284).
+constexpr int KEY_RPT = 16;  // reference points per thread; grid.y = KEY_RP / KEY_RPT
+
 template <class T, bool COSS = false>
 __global__ __launch_bounds__(256) void k_keys(const T *__restrict__ xs, uint32_t n, uint32_t d,
                                               const double *__restrict__ rpT, uint32_t nrp,
@@ -273,35 +279,40 @@ __global__ __launch_bounds__(256) void k_keys(const T *__restrict__ xs, uint32_t
 {
     __shared__ uint32_t part[256 / WAVE];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t t0 = blockIdx.y * KEY_RPT;
     uint32_t m = 0;
-    if (i < n) {
-        double acc[KEY_RP];
+    if (i < n && t0 < nrp) {
+        double acc[KEY_RPT];
 #pragma unroll
-        for (int t = 0; t < KEY_RP; ++t)
+        for (int t = 0; t < KEY_RPT; ++t)
             acc[t] = 0.0;
+        double xn = (double)xs[i];  // d >= 1; column j + 1 is read during column j
         for (uint32_t j = 0; j < d; ++j) {
-            const double x = (double)xs[(size_t)j * n + i];
-            const double *r = rpT + (size_t)j * KEY_RP;
+            const double x = xn;
+            if (j + 1 < d)
+                xn = (double)xs[(size_t)(j + 1) * n + i];
+            const double *r = rpT + (size_t)j * KEY_RP + t0;
 #pragma unroll
-            for (int t = 0; t < KEY_RP; ++t) {
+            for (int t = 0; t < KEY_RPT; ++t) {
                 double df = x - r[t];
                 acc[t] = fma(df, df, acc[t]);
             }
         }
 #pragma unroll
-        for (int t = 0; t < KEY_RP; ++t)
-            if ((uint32_t)t < nrp) {
+        for (int t = 0; t < KEY_RPT; ++t)
+            if (t0 + t < nrp) {
                 const uint32_t k = rp_key(acc[t], eps);
-                keys[(size_t)t * n + i] = k;
+                keys[(size_t)(t0 + t) * n + i] = k;
                 m = max(m, k);
             }
+    }
+    if (i < n && blockIdx.y == 0)
         for (uint32_t a = 0; a < naxis; ++a) {
             const uint32_t k = COSS ? coss_axis_key((double)xs[(size_t)a * n + i], eps)
                                     : axis_key((double)xs[(size_t)a * n + i], eps);
             keys[(size_t)(nrp + a) * n + i] = k;
             m = max(m, k);
         }
-    }
     if (!kmax)
         return;
 #pragma unroll
@@ -316,6 +327,18 @@ __global__ __launch_bounds__(256) void k_keys(const T *__restrict__ xs, uint32_t
         if (m > __hip_atomic_load(kmax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
             atomicMax(kmax, m);
     }
+}
+
+// Launch of k_keys over n points and every reference-point group.
+template <class T, bool COSS>
+void launch_keys(const T *xs, uint32_t n, uint32_t d, const double *rpT, uint32_t nrp,
+                 uint32_t naxis, double eps, uint32_t *keys, uint32_t *kmax, hipStream_t s)
+{
+    const dim3 g(grid_for(n, 256), (nrp + KEY_RPT - 1) / KEY_RPT);  // nrp in 1..KEY_RP
+    check_grid((size_t)g.x * g.y, 256);
+    hipLaunchKernelGGL((k_keys<T, COSS>), g, dim3(256), 0, s, xs, n, d, rpT, nrp, naxis, eps,
+                       keys, kmax);
+    MSJ_HIP(hipGetLastError());
 }
 
 // Composite (candidate | parent node | key) for candidates [t0, t0+tcount).
@@ -593,81 +616,114 @@ __device__ inline uint32_t lower_bound_u32(const uint32_t *a, uint32_t lo, uint3
     return lo;
 }
 
-// Neighbours of child c in the reference's order (nodes.cu:557-578 siblings,
-// then nodes.cu:719-747 for every other parent neighbour, in list order).
+// Child c of parent P (key k; s: P split) and one entry Q of P's
+// neighbour list: the children of Q that are c's neighbours, in child order
+// (nodes.cu:557-578 siblings when Q == P, nodes.cu:719-747 otherwise).
 template <class F>
-__device__ inline void visit_nbrs(const AdjView &A, uint32_t c, F &&emit)
+__device__ inline void visit_entry(const AdjView &A, uint32_t c, uint32_t P, uint32_t k, bool s,
+                                   uint32_t pf, uint32_t pl, uint32_t Q, F &&emit)
 {
-    const uint32_t P = A.cparent[c];
-    const uint32_t k = A.ckey[c];
-    const bool s = A.osplit[P] != 0;
-    const uint32_t pf = A.cfirst[P], pl = A.cfirst[P + 1];
-    for (uint32_t a = A.oaoff[P]; a < A.oaoff[P + 1]; ++a) {
-        const uint32_t Q = A.oadj[a];
-        if (Q == P) {
-            emit(c);
-            if (s) {
-                if (c > pf && A.ckey[c - 1] + 1u == k)
-                    emit(c - 1);
-                if (c + 1 < pl && A.ckey[c + 1] - 1u == k)
-                    emit(c + 1);
-            }
-            continue;
+    if (Q == P) {
+        emit(c);
+        if (s) {
+            if (c > pf && A.ckey[c - 1] + 1u == k)
+                emit(c - 1);
+            if (c + 1 < pl && A.ckey[c + 1] - 1u == k)
+                emit(c + 1);
         }
-        const uint32_t q0 = A.cfirst[Q], q1 = A.cfirst[Q + 1];
-        if (!s || !A.osplit[Q]) {
-            for (uint32_t v = q0; v < q1; ++v)
-                emit(v);
-            continue;
-        }
-        // keys k-1, k, k+1 modulo 2^32; Q's child keys ascend
-        uint32_t hit[3];
-        int nh = 0;
-        const uint32_t want[3] = {k - 1u, k, k + 1u};
-        for (int w = 0; w < 3; ++w) {
-            uint32_t pos = lower_bound_u32(A.ckey, q0, q1, want[w]);
-            if (pos < q1 && A.ckey[pos] == want[w])
-                hit[nh++] = pos;
-        }
-        for (int x = 1; x < nh; ++x)  // emit in child order
-            for (int y = x; y > 0 && hit[y - 1] > hit[y]; --y) {
-                uint32_t tmp = hit[y];
-                hit[y] = hit[y - 1];
-                hit[y - 1] = tmp;
-            }
-        for (int x = 0; x < nh; ++x)
-            emit(hit[x]);
+        return;
     }
+    const uint32_t q0 = A.cfirst[Q], q1 = A.cfirst[Q + 1];
+    if (!s || !A.osplit[Q]) {
+        for (uint32_t v = q0; v < q1; ++v)
+            emit(v);
+        return;
+    }
+    // keys k-1, k, k+1 modulo 2^32; Q's child keys ascend
+    uint32_t hit[3];
+    int nh = 0;
+    const uint32_t want[3] = {k - 1u, k, k + 1u};
+    for (int w = 0; w < 3; ++w) {
+        uint32_t pos = lower_bound_u32(A.ckey, q0, q1, want[w]);
+        if (pos < q1 && A.ckey[pos] == want[w])
+            hit[nh++] = pos;
+    }
+    for (int x = 1; x < nh; ++x)  // emit in child order
+        for (int y = x; y > 0 && hit[y - 1] > hit[y]; --y) {
+            uint32_t tmp = hit[y];
+            hit[y] = hit[y - 1];
+            hit[y - 1] = tmp;
+        }
+    for (int x = 0; x < nh; ++x)
+        emit(hit[x]);
 }
 
+// One wave per child c (wave-uniform): its lanes take the entries of the
+// parent's neighbour list in turn, so the per-entry searches run side by
+// side; the neighbours come out in the reference's order, entry by entry in
+// list order.
 __global__ __launch_bounds__(256) void k_adj_count(AdjView A, uint32_t mnew,
                                                    uint32_t *__restrict__ deg,
                                                    uint64_t *__restrict__ sumsz,
                                                    uint64_t *__restrict__ ncalcs)
 {
-    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t c = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    const uint32_t lane = threadIdx.x & (WAVE - 1);
     if (c >= mnew)
         return;
+    const uint32_t P = A.cparent[c], k = A.ckey[c];
+    const bool s = A.osplit[P] != 0;
+    const uint32_t pf = A.cfirst[P], pl = A.cfirst[P + 1];
     uint32_t cnt = 0;
     uint64_t sz = 0;
-    visit_nbrs(A, c, [&](uint32_t v) {
-        ++cnt;
-        sz += A.cstart[v + 1] - A.cstart[v];
-    });
-    deg[c] = cnt;
-    sumsz[c] = sz;
-    ncalcs[c] = sz * (uint64_t)(A.cstart[c + 1] - A.cstart[c]);  // nodes.cu:691
+    for (uint32_t a = A.oaoff[P] + lane; a < A.oaoff[P + 1]; a += WAVE)
+        visit_entry(A, c, P, k, s, pf, pl, A.oadj[a], [&](uint32_t v) {
+            ++cnt;
+            sz += A.cstart[v + 1] - A.cstart[v];
+        });
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) {
+        cnt += (uint32_t)__shfl_xor(cnt, o, WAVE);
+        sz += (uint64_t)__shfl_xor((unsigned long long)sz, o, WAVE);
+    }
+    if (lane == 0) {
+        deg[c] = cnt;
+        sumsz[c] = sz;
+        ncalcs[c] = sz * (uint64_t)(A.cstart[c + 1] - A.cstart[c]);  // nodes.cu:691
+    }
 }
 
 __global__ __launch_bounds__(256) void k_adj_write(AdjView A, uint32_t mnew,
                                                    const uint32_t *__restrict__ aoff,
                                                    uint32_t *__restrict__ adj)
 {
-    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t c = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+    const uint32_t lane = threadIdx.x & (WAVE - 1);
     if (c >= mnew)
         return;
+    const uint32_t P = A.cparent[c], k = A.ckey[c];
+    const bool s = A.osplit[P] != 0;
+    const uint32_t pf = A.cfirst[P], pl = A.cfirst[P + 1];
+    const uint32_t a0 = A.oaoff[P], a1 = A.oaoff[P + 1];
     uint32_t w = aoff[c];
-    visit_nbrs(A, c, [&](uint32_t v) { adj[w++] = v; });
+    for (uint32_t base = a0; base < a1; base += WAVE) {
+        const uint32_t a = base + lane;
+        const uint32_t Q = a < a1 ? A.oadj[a] : 0u;
+        uint32_t cnt = 0;
+        if (a < a1)
+            visit_entry(A, c, P, k, s, pf, pl, Q, [&](uint32_t) { ++cnt; });
+        uint32_t incl = cnt;  // inclusive prefix over the lanes
+#pragma unroll
+        for (int o = 1; o < WAVE; o <<= 1) {
+            const uint32_t up = (uint32_t)__shfl_up(incl, o, WAVE);
+            if (lane >= (uint32_t)o)
+                incl += up;
+        }
+        uint32_t pos = w + incl - cnt;
+        if (a < a1)
+            visit_entry(A, c, P, k, s, pf, pl, Q, [&](uint32_t v) { adj[pos++] = v; });
+        w += (uint32_t)__shfl(incl, WAVE - 1, WAVE);
+    }
 }
 
 __global__ __launch_bounds__(256) void k_nsplit(const uint64_t *__restrict__ ncalcs, uint32_t m,
@@ -875,8 +931,7 @@ void build_impl(IndexBuilder &B, const T *xs, uint32_t n, uint32_t d, double eps
     LAUNCH(k_fill<uint32_t>, 1, 64, B.u32scalar.p, (size_t)std::max<uint32_t>(layers, 1), 0u);
     auto layer_keys = [&](uint32_t l, uint32_t *dst) {
         const double *rpT = layer_rps(B, xs, n, d, S, nrp, l, layers, rng, rh, s);
-        LAUNCH(k_keys<T>, grid_for(n, 256), 256, xs, n, d, rpT, nrp, naxis, eps, dst,
-               B.u32scalar.p + l);
+        launch_keys<T, false>(xs, n, d, rpT, nrp, naxis, eps, dst, B.u32scalar.p + l, s);
         if (B.probe && B.probe->keys && B.probe->layer == l) {
             MSJ_HIP(hipMemcpyAsync(B.probe->keys, dst, sizeof(uint32_t) * ncand * n,
                                    hipMemcpyDeviceToHost, s));
@@ -972,7 +1027,8 @@ void build_impl(IndexBuilder &B, const T *xs, uint32_t n, uint32_t d, double eps
         B.ncalcs2.ensure(mnew);
         B.aoff2.ensure((size_t)mnew + 1);
         AdjView A{B.cparent.p, B.ckey.p, B.cstart.p, B.cfirst.p, B.nsplit.p, B.aoff1.p, B.adj1.p};
-        LAUNCH(k_adj_count, grid_for(mnew, 256), 256, A, mnew, B.deg.p, B.sumsz2.p, B.ncalcs2.p);
+        LAUNCH(k_adj_count, grid_for((size_t)mnew * WAVE, 256), 256, A, mnew, B.deg.p, B.sumsz2.p,
+               B.ncalcs2.p);
         {
             // degree prefix in 64 bits first, to detect > 2^32 adjacency entries
             B.tmp64.ensure((size_t)mnew + 1);
@@ -990,7 +1046,8 @@ void build_impl(IndexBuilder &B, const T *xs, uint32_t n, uint32_t d, double eps
                 throw Error(MSJ_E_OVERFLOW, "node adjacency exceeds 2^32 entries");
             B.scan.excl_u32_to_u32(B.deg.p, B.aoff2.p, mnew, s);
             B.adj2.ensure(std::max<uint64_t>(edges, 1));
-            LAUNCH(k_adj_write, grid_for(mnew, 256), 256, A, mnew, B.aoff2.p, B.adj2.p);
+            LAUNCH(k_adj_write, grid_for((size_t)mnew * WAVE, 256), 256, A, mnew, B.aoff2.p,
+                   B.adj2.p);
             out.edges = (uint32_t)edges;
         }
         B.nsplit2.ensure(mnew);
@@ -1369,8 +1426,7 @@ void build_coss_impl(IndexBuilder &B, const T *xs, uint32_t n, uint32_t d, doubl
     for (uint32_t cl = 0; cl < L; ++cl) {
         // ---- candidates and their keys (createRPArray, binningKernel, tree.cu:284)
         const double *rpT = layer_rps(B, xs, n, d, S, nrp, cl, L, rng, rh, s);
-        LAUNCH((k_keys<T, true>), grid_for(n, 256), 256, xs, n, d, rpT, nrp, naxis, eps,
-               B.keys.p, (uint32_t *)nullptr);
+        launch_keys<T, true>(xs, n, d, rpT, nrp, naxis, eps, B.keys.p, nullptr, s);
         if (B.probe && B.probe->keys && B.probe->layer == cl) {
             MSJ_HIP(hipMemcpyAsync(B.probe->keys, B.keys.p, sizeof(uint32_t) * ncand * n,
                                    hipMemcpyDeviceToHost, s));
