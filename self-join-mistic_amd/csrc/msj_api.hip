@@ -89,7 +89,7 @@ struct msj_ctx {
     PinnedBuf pin[2];                   // double buffer of result pages
     uint32_t n = 0, d = 0;
     bool f32 = false;
-    msj::DevBuf<unsigned char> staging;  // row-major upload
+    msj::DevBuf<unsigned char> staging;  // row-major upload (empty after an unordered load)
     msj::DevBuf<unsigned char> xs;       // SoA, row order
     msj::JoinCore core;                  // index, join and their device state
     msj::DevBuf<uint32_t> r_counts;
@@ -292,10 +292,11 @@ void do_load(msj_ctx *c, const void *data, uint32_t n, uint32_t d)
     c->f32 = c->prm.storage == MSJ_STORAGE_F32;
     c->n = n;
     c->d = d;
+    c->staging.release();
     if (n == 0)
         return;
+    // the row-major upload stays: joins gather whole rows from it
     msj::load_soa(data, n, d, c->f32, c->xs, c->staging, c->stream);
-    c->staging.release();
 }
 
 // main.cu:42-50 + 106-122 on the device: n rows of d FP64 values in file
@@ -318,6 +319,7 @@ void do_load_unordered(msj_ctx *c, const double *rows, uint32_t n, uint32_t d,
     c->f32 = c->prm.storage == MSJ_STORAGE_F32;
     c->n = 0;
     c->d = d;
+    c->staging.release();  // the rows arrive in file order, not in column order: no copy kept
     std::vector<uint32_t> order(d);
     std::vector<double> sig(d, 0.0);
     if (n == 0) {
@@ -450,7 +452,8 @@ void do_join(msj_ctx *c, double eps, msj_result *out)
         uint64_t budget = (uint64_t)(freeb / 4) / sizeof(uint64_t);
         cap = std::max<uint64_t>(std::min(want, budget), 1);
     }
-    const uint64_t R = c->core.run(c->xs.p, c->f32, n, d, eps, c->prm, cap, s, c->ev, st);
+    const uint64_t R =
+        c->core.run(c->xs.p, c->staging.p, c->f32, n, d, eps, c->prm, cap, s, c->ev, st);
     c->last_R = R;
     c->r_counts.ensure(n);
     c->r_offsets.ensure((size_t)n + 1);
@@ -492,7 +495,8 @@ void do_join(msj_ctx *c, double eps, msj_result *out)
 
 }  // namespace
 
-uint64_t msj::JoinCore::run(const void *xs, bool f32, uint32_t n, uint32_t d, double eps,
+uint64_t msj::JoinCore::run(const void *xs, const void *rows, bool f32, uint32_t n, uint32_t d,
+                            double eps,
                             const msj_params &prm, uint64_t cap, hipStream_t s, hipEvent_t *ev,
                             msj_stats &st)
 {
@@ -505,7 +509,7 @@ uint64_t msj::JoinCore::run(const void *xs, bool f32, uint32_t n, uint32_t d, do
         builder.build(xs, f32, n, d, eps, prm, ix, ls, s);
     join.order(ix, xs, f32, n, d, s);
     xr.ensure((size_t)n * padded_dim(d));
-    gather_node_rows(xs, xr.p, f32, join.jperm.p, n, d, s);
+    gather_node_rows(xs, rows, xr.p, f32, join.jperm.p, n, d, s);
     const bool use_mfma = prm.form != MSJ_FORM_PLAIN && mfma_filter_applies(eps, n, d);
     join.plan(ix, n, d, s, !use_mfma);
     MSJ_HIP(hipEventRecord(ev[1], s));

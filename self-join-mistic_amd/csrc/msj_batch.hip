@@ -731,7 +731,8 @@ void batch_run(BatchEngine *b, msj_result *outs, msj_batch_stats *st)
                     W.builder.build(B.xs.p, f32, B.n, B.d, B.eps, b->prm, B.ix, B.ls, W.s);
                     B.join.order(B.ix, B.xs.p, f32, B.n, B.d, W.s);
                     B.xr.ensure((size_t)B.n * padded_dim(B.d));
-                    gather_node_rows(B.xs.p, B.xr.p, f32, B.join.jperm.p, B.n, B.d, W.s);
+                    gather_node_rows(B.xs.p, nullptr, B.xr.p, f32, B.join.jperm.p, B.n, B.d,
+                                     W.s);
                     B.solo = B.ix.num_calcs >= MSJ_SOLO_CALCS;
                     B.mfma_form = B.solo && b->prm.form != MSJ_FORM_PLAIN &&
                                   mfma_filter_applies(B.eps, B.n, B.d);

@@ -451,10 +451,11 @@ struct JoinCore {
     MfmaRunner mfma;
     DevBuf<double> xr;           // FP64 rows in join order
     // Candidate generation and distance kernels on xs (SoA, row order);
-    // returns the pair count R (keys in join.pairs).  cap: initial result-key
-    // capacity (0 under an arena: half of what it has left).  Events
-    // ev[0..7] on stream s time the stages; st receives the statistics.
-    uint64_t run(const void *xs, bool f32, uint32_t n, uint32_t d, double eps,
+    // rows: the same data row-major, or null.  Returns the pair count R
+    // (keys in join.pairs).  cap: initial result-key capacity (0 under an
+    // arena: half of what it has left).  Events ev[0..7] on stream s time
+    // the stages; st receives the statistics.
+    uint64_t run(const void *xs, const void *rows, bool f32, uint32_t n, uint32_t d, double eps,
                  const msj_params &prm, uint64_t cap, hipStream_t s, hipEvent_t *ev,
                  msj_stats &st);
     // Sorts the pairs into counts[n], offsets[n+1], nbrs[R] and synchronises.
@@ -487,8 +488,9 @@ void batch_run(BatchEngine *b, msj_result *outs, msj_batch_stats *st);
 inline uint32_t padded_dim(uint32_t d) { return (d + 3u) & ~3u; }
 
 // xr[i*ds + j] = FP64 value of column j of node-order row i (row perm[i]),
-// ds = padded_dim(d), zero padding.
-void gather_node_rows(const void *xs, double *xr, bool f32, const uint32_t *perm, uint32_t n,
-                      uint32_t d, hipStream_t s);
+// ds = padded_dim(d), zero padding; from `rows` (row-major, storage type)
+// when given, else from the columns `xs`.
+void gather_node_rows(const void *xs, const void *rows, double *xr, bool f32,
+                      const uint32_t *perm, uint32_t n, uint32_t d, hipStream_t s);
 
 }  // namespace msj

@@ -251,6 +251,24 @@ __global__ __launch_bounds__(256) void k_gather_rows(const T *__restrict__ xs,
     }
 }
 
+// The same from rows still in the uploaded row-major layout (rows[r*d + j]):
+// one wave copies one row per step, reading it whole.
+template <class T>
+__global__ __launch_bounds__(256) void k_gather_rows_aos(const T *__restrict__ rows,
+                                                         double *__restrict__ xr,
+                                                         const uint32_t *__restrict__ perm,
+                                                         uint32_t n, uint32_t d, uint32_t ds)
+{
+    const uint32_t lane = threadIdx.x & (WAVE - 1);
+    const uint32_t wstride = gridDim.x * (blockDim.x / WAVE);
+    for (uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE; i < n; i += wstride) {
+        const T *src = rows + (size_t)perm[i] * d;
+        double *dst = xr + (size_t)i * ds;
+        for (uint32_t j = lane; j < ds; j += WAVE)
+            dst[j] = j < d ? (double)src[j] : 0.0;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_rows_count(const uint64_t *__restrict__ pairs, size_t R,
                                                     int qbits, uint32_t *__restrict__ counts)
 {
@@ -591,10 +609,18 @@ void dispatch_plain(const JobView &V, unsigned long long *work, hipStream_t s)
 
 }  // namespace
 
-void gather_node_rows(const void *xs, double *xr, bool f32, const uint32_t *perm, uint32_t n,
-                      uint32_t d, hipStream_t s)
+void gather_node_rows(const void *xs, const void *rows, double *xr, bool f32,
+                      const uint32_t *perm, uint32_t n, uint32_t d, hipStream_t s)
 {
     const uint32_t ds = padded_dim(d);
+    if (rows) {
+        const unsigned g = grid_stride((size_t)n * WAVE, 256);
+        if (f32)
+            LAUNCH(k_gather_rows_aos<float>, g, 256, (const float *)rows, xr, perm, n, d, ds);
+        else
+            LAUNCH(k_gather_rows_aos<double>, g, 256, (const double *)rows, xr, perm, n, d, ds);
+        return;
+    }
     size_t m = (size_t)n * ds;
     if (f32)
         LAUNCH(k_gather_rows<float>, grid_stride(m, 256), 256, (const float *)xs, xr, perm, n, d,

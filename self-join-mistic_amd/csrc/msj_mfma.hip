@@ -153,37 +153,72 @@ __global__ __launch_bounds__(256) void k_pad_rows(const uint32_t *__restrict__ n
     porig[r] = jperm[i];
 }
 
-// Column sums of xr over row blocks (deterministic order), then the means.
+// Column sums of xr over row blocks, then the means; every sum in a fixed
+// order, so the centre is the same from run to run.  Workgroup b sums rows
+// [b * rows_per, ...): thread t takes column t % ds and every Q-th row from
+// row t / ds (Q = threads per column), and the Q partial sums are added in
+// phase order.
 __global__ __launch_bounds__(256) void k_col_partial(const double *__restrict__ xr, uint32_t n,
                                                      uint32_t ds, uint32_t rows_per,
                                                      double *__restrict__ part)
 {
-    const uint32_t b = blockIdx.x;
+    __shared__ double ph[256];
+    const uint32_t b = blockIdx.x, tid = threadIdx.x;
     const uint32_t r0 = b * rows_per, r1 = min(n, r0 + rows_per);
-    for (uint32_t j = threadIdx.x; j < ds; j += blockDim.x) {
-        double acc = 0.0;
-        for (uint32_t r = r0; r < r1; ++r)
-            acc += xr[(size_t)r * ds + j];
-        part[(size_t)b * ds + j] = acc;
+    const uint32_t Q = ds < 256 ? 256 / ds : 1u;
+    for (uint32_t j0 = 0; j0 < ds; j0 += 256) {
+        const uint32_t q = tid / (Q > 1 ? ds : 256u), j = j0 + tid % (Q > 1 ? ds : 256u);
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        if (q < Q && j < ds) {
+            uint32_t r = r0 + q;
+            for (; r + 3 * Q < r1; r += 4 * Q) {
+                a0 += xr[(size_t)r * ds + j];
+                a1 += xr[(size_t)(r + Q) * ds + j];
+                a2 += xr[(size_t)(r + 2 * Q) * ds + j];
+                a3 += xr[(size_t)(r + 3 * Q) * ds + j];
+            }
+            for (; r < r1; r += Q)
+                a0 += xr[(size_t)r * ds + j];
+        }
+        ph[tid] = (a0 + a1) + (a2 + a3);
+        __syncthreads();
+        if (tid < min(256u, ds - j0) && (Q > 1 || j0 + tid < ds)) {
+            double acc = ph[tid];
+            for (uint32_t k = 1; k < Q; ++k)
+                acc += ph[k * ds + tid];
+            part[(size_t)b * ds + j0 + tid] = acc;
+        }
+        __syncthreads();
     }
 }
 
+// mu[j] = (sum of the nb partial sums of column j = blockIdx.x) / n: each
+// thread adds every 256th partial, then a fixed tree adds the threads' sums.
 __global__ __launch_bounds__(256) void k_col_final(const double *__restrict__ part, uint32_t nb,
                                                    uint32_t ds, uint32_t n,
                                                    double *__restrict__ mu)
 {
-    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= ds)
-        return;
+    __shared__ double ph[256];
+    const uint32_t j = blockIdx.x, tid = threadIdx.x;
     double acc = 0.0;
-    for (uint32_t b = 0; b < nb; ++b)
+    for (uint32_t b = tid; b < nb; b += 256)
         acc += part[(size_t)b * ds + j];
-    mu[j] = acc / (double)n;
+    ph[tid] = acc;
+    __syncthreads();
+    for (uint32_t o = 128; o > 0; o >>= 1) {
+        if (tid < o)
+            ph[tid] += ph[tid + o];
+        __syncthreads();
+    }
+    if (tid == 0)
+        mu[j] = ph[0] / (double)n;
 }
 
-// One wave per padded row: centred BF16 (and low BF16) fragments, and the
-// row's threshold terms (alpha, alpha', |a'|^2 as FP32, 0).
-template <bool SPLIT>
+// RPW padded rows per wave (64 / RPW lanes each, one 8-dimension group per
+// lane): centred BF16 (and low BF16) fragments, and the row's threshold
+// terms (alpha, alpha', |a'|^2 as FP32, 0).  The squared norm's rounding
+// (any summation order) stays far inside the 2^-40 slack of the terms.
+template <bool SPLIT, uint32_t RPW>
 __global__ __launch_bounds__(256) void k_frag_prep(const double *__restrict__ xr, uint32_t ds,
                                                    uint32_t KS, const double *__restrict__ mu,
                                                    const uint32_t *__restrict__ pjoin,
@@ -192,15 +227,16 @@ __global__ __launch_bounds__(256) void k_frag_prep(const double *__restrict__ xr
                                                    float4 *__restrict__ thr, double Cb,
                                                    float2 *__restrict__ thrb)
 {
-    const uint32_t r = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    constexpr uint32_t LPR = WAVE / RPW;  // lanes per row
     const uint32_t lane = threadIdx.x & (WAVE - 1);
-    if (r >= Ptot)
-        return;
-    const uint32_t j = pjoin[r];
+    const uint32_t r = (blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE) * RPW + lane / LPR;
+    const uint32_t sl = lane % LPR;
+    const bool live = r < Ptot;  // whole waves stay for the shuffles
+    const uint32_t j = live ? pjoin[r] : ~0u;
     const bool valid = j != ~0u;
     const uint32_t groups = KS * 2;  // 8-dim groups
     double na = 0.0;
-    for (uint32_t g = lane; g < groups; g += WAVE) {
+    for (uint32_t g = sl; live && g < groups; g += LPR) {
         uint32_t hw[4] = {0, 0, 0, 0}, lw[4] = {0, 0, 0, 0};
         if (valid) {
 #pragma unroll
@@ -223,9 +259,9 @@ __global__ __launch_bounds__(256) void k_frag_prep(const double *__restrict__ xr
             fl[idx] = make_uint4(lw[0], lw[1], lw[2], lw[3]);
     }
 #pragma unroll
-    for (int o = WAVE / 2; o > 0; o >>= 1)
+    for (uint32_t o = LPR / 2; o > 0; o >>= 1)
         na += __shfl_xor(na, o, WAVE);
-    if (lane == 0) {
+    if (live && sl == 0) {
         float4 t = make_float4(0.f, 0.f, 0.f, 0.f);
         float2 tb = make_float2(0.f, 0.f);  // the same terms under the BF16 bound
         if (valid) {
@@ -1601,7 +1637,7 @@ void MfmaRunner::prepare(const Index &ix, const JoinRunner &J, const double *xr,
     colpart.ensure((size_t)nb * ds);
     mu.ensure(ds);
     LAUNCH(k_col_partial, nb, 256, xr, n, ds, rows_per, colpart.p);
-    LAUNCH(k_col_final, grid_for(ds, 256), 256, colpart.p, nb, ds, n, mu.p);
+    LAUNCH(k_col_final, ds, 256, colpart.p, nb, ds, n, mu.p);
 
     // fragments and threshold terms
     const double C = bound_c(d, split);
@@ -1616,12 +1652,28 @@ void MfmaRunner::prepare(const Index &ix, const JoinRunner &J, const double *xr,
     if (refine)
         thrb.ensure(Ptot);
     const double Cb = bound_c(d, false, 3.0);
-    if (split)
-        LAUNCH(k_frag_prep<true>, grid_for(Ptot, 4), 256, xr, ds, KS, mu.p, pjoin.p, Ptot, C,
-               fh.p, fl.p, thr.p, Cb, refine ? thrb.p : nullptr);
-    else
-        LAUNCH(k_frag_prep<false>, grid_for(Ptot, 4), 256, xr, ds, KS, mu.p, pjoin.p, Ptot, C,
-               fh.p, fh.p, thr.p, Cb, nullptr);
+    // rows per wave: as many as leave a lane per 8-dimension group
+    const uint32_t rpw = 2 * KS <= 16 ? 4u : (2 * KS <= 32 ? 2u : 1u);
+    const unsigned fg = grid_for(Ptot, 4 * rpw);
+#define MSJ_FRAG_PREP(SPL, R)                                                                    \
+    LAUNCH((k_frag_prep<SPL, R>), fg, 256, xr, ds, KS, mu.p, pjoin.p, Ptot, C, fh.p,           \
+           SPL ? fl.p : fh.p, thr.p, Cb, (SPL && refine) ? thrb.p : nullptr)
+    if (split) {
+        if (rpw == 4)
+            MSJ_FRAG_PREP(true, 4);
+        else if (rpw == 2)
+            MSJ_FRAG_PREP(true, 2);
+        else
+            MSJ_FRAG_PREP(true, 1);
+    } else {
+        if (rpw == 4)
+            MSJ_FRAG_PREP(false, 4);
+        else if (rpw == 2)
+            MSJ_FRAG_PREP(false, 2);
+        else
+            MSJ_FRAG_PREP(false, 1);
+    }
+#undef MSJ_FRAG_PREP
 
     // work items: super-blocks over the upper neighbour lists
     uint32_t E = 0;

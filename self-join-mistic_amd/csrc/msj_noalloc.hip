@@ -172,7 +172,7 @@ void run(void *a_)
     msj::JoinCore core;
     // cap 0: the result-key capacity is taken from what the arena has left
     // once the index is built
-    const uint64_t R = core.run(xs.p, f32, a.n, a.d, a.eps, prm, 0, s, sl.ev, st);
+    const uint64_t R = core.run(xs.p, a.rows, f32, a.n, a.d, a.eps, prm, 0, s, sl.ev, st);
     st.total_pairs = R;
     if (R > a.nbrs_cap) {
         if (a.stats)
