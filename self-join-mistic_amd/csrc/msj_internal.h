@@ -461,6 +461,15 @@ struct JoinCore {
     // Sorts the pairs into counts[n], offsets[n+1], nbrs[R] and synchronises.
     void finish(uint32_t n, uint64_t R, uint32_t *counts, uint64_t *offsets, uint32_t *nbrs,
                 hipStream_t s, hipEvent_t *ev, msj_stats &st);
+    // The reference's batch record (launcher.cu:3203-3233) of the last run
+    // into st, once: finish computes it while the pair sort runs.
+    void batch_record(msj_stats &st);
+    std::vector<uint32_t> hstart;  // node starts and numCalcs of the last run
+    std::vector<uint64_t> hcalcs;
+    uint32_t rec_n = 0;
+    double rec_eps = 0.0;
+    uint64_t rec_calcs = 0;
+    bool rec_pending = false;
 };
 
 // The filter's bounds hold for 2^-50 <= eps^2 <= 2^60, n < 2^30, d <= 4096.

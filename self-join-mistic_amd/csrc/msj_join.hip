@@ -269,33 +269,34 @@ __global__ __launch_bounds__(256) void k_gather_rows_aos(const T *__restrict__ r
     }
 }
 
-__global__ __launch_bounds__(256) void k_rows_count(const uint64_t *__restrict__ pairs, size_t R,
-                                                    int qbits, uint32_t *__restrict__ counts)
+// Sorted keys (row << qbits | neighbour): every key's neighbour id into
+// nbrs, and at the start of each row's run its position into first[row].
+__global__ __launch_bounds__(256) void k_rows_first(const uint64_t *__restrict__ pairs, size_t R,
+                                                    int qbits, uint64_t qmask,
+                                                    uint64_t *__restrict__ first,
+                                                    uint32_t *__restrict__ nbrs)
 {
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < R; i += stride) {
-        uint32_t row = (uint32_t)(pairs[i] >> qbits);
-        // sorted keys: count each row once at the end of its run
-        if (i + 1 == R || (uint32_t)(pairs[i + 1] >> qbits) != row) {
-            size_t lo = 0, hi = i + 1;  // first index of this row
-            while (lo < hi) {
-                size_t mid = lo + (hi - lo) / 2;
-                if ((uint32_t)(pairs[mid] >> qbits) < row)
-                    lo = mid + 1;
-                else
-                    hi = mid;
-            }
-            counts[row] = (uint32_t)(i + 1 - lo);
-        }
+        const uint64_t v = pairs[i];
+        const uint64_t row = v >> qbits;
+        nbrs[i] = (uint32_t)(v & qmask);
+        if (i == 0 || (pairs[i - 1] >> qbits) != row)
+            first[row] = i;
     }
 }
 
-__global__ __launch_bounds__(256) void k_rows_nbrs(const uint64_t *__restrict__ pairs, size_t R,
-                                                   uint64_t qmask, uint32_t *__restrict__ nbrs)
+// At the end of each row's run: its length.
+__global__ __launch_bounds__(256) void k_rows_count(const uint64_t *__restrict__ pairs, size_t R,
+                                                    int qbits, const uint64_t *__restrict__ first,
+                                                    uint32_t *__restrict__ counts)
 {
     const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < R; i += stride)
-        nbrs[i] = (uint32_t)(pairs[i] & qmask);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < R; i += stride) {
+        const uint64_t row = pairs[i] >> qbits;
+        if (i + 1 == R || (pairs[i + 1] >> qbits) != row)
+            counts[row] = (uint32_t)(i + 1 - first[row]);
+    }
 }
 
 // Per-column minimum and maximum of the first columns: blockIdx.y is the
@@ -868,9 +869,11 @@ void JoinRunner::finish(uint32_t n, uint64_t R, uint32_t *counts, uint64_t *offs
     sorter.sort(pairs.p, nullptr, R, qbits + rbits, s);
     LAUNCH(k_zero<uint32_t>, grid_for(n, 256), 256, counts, (size_t)n);
     if (R) {
-        LAUNCH(k_rows_count, grid_stride(R, 256), 256, pairs.p, (size_t)R, qbits, counts);
-        LAUNCH(k_rows_nbrs, grid_stride(R, 256), 256, pairs.p, (size_t)R, (1ull << qbits) - 1,
-               nbrs);
+        // offsets[] holds the runs' first positions until the scan below
+        LAUNCH(k_rows_first, grid_stride(R, 256), 256, pairs.p, (size_t)R, qbits,
+               (1ull << qbits) - 1, offsets, nbrs);
+        LAUNCH(k_rows_count, grid_stride(R, 256), 256, pairs.p, (size_t)R, qbits, offsets,
+               counts);
     }
     scan.excl_u32_to_u64(counts, offsets, n, s);
 }

@@ -175,6 +175,7 @@ void run(void *a_)
     const uint64_t R = core.run(xs.p, a.rows, f32, a.n, a.d, a.eps, prm, 0, s, sl.ev, st);
     st.total_pairs = R;
     if (R > a.nbrs_cap) {
+        core.batch_record(st);
         if (a.stats)
             *a.stats = st;  // total_pairs tells the caller how much room to give
         throw Error(MSJ_E_OVERFLOW, std::to_string(R) + " result pairs exceed nbrs_cap " +
