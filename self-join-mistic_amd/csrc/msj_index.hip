@@ -387,7 +387,8 @@ __global__ void k_runfirst(const uint32_t *__restrict__ runidx, uint32_t n, uint
 }
 
 // nodes.cu:194-199: sum over nodes, in node order, of (size - N/#nodes)^2.
-// Sequential in node order (loads issued eight runs ahead).
+// Sequential in node order; the next U + 1 run starts are read while the
+// current U runs are added.
 __global__ void k_cand_sse(const uint32_t *__restrict__ runfirst,
                            const uint32_t *__restrict__ starts, uint32_t tcount, uint32_t n,
                            double *__restrict__ sse, uint32_t *__restrict__ cnt)
@@ -395,22 +396,32 @@ __global__ void k_cand_sse(const uint32_t *__restrict__ runfirst,
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= tcount)
         return;
-    constexpr uint32_t U = 8;
+    constexpr uint32_t U = 16;
     const uint32_t r0 = runfirst[t], r1 = runfirst[t + 1];
     const double avg = (double)n * 1.0 / (double)(r1 - r0);
     double s = 0;
     uint32_t r = r0;
-    for (; r + U <= r1; r += U) {
-        uint32_t st[U + 1];
+    uint32_t st[U + 1];
+    if (r + U <= r1) {
 #pragma unroll
         for (uint32_t k = 0; k <= U; ++k)
-            st[k] = starts[r + k];
+            st[k] = starts[r + k];  // starts holds r1 + 1 entries from r0
+    }
+    for (; r + U <= r1; r += U) {
+        uint32_t nx[U + 1];
+        const bool more = r + 2 * U <= r1;
+#pragma unroll
+        for (uint32_t k = 0; k <= U; ++k)
+            nx[k] = more ? starts[r + U + k] : 0u;
 #pragma unroll
         for (uint32_t k = 0; k < U; ++k) {
             double df = (double)(st[k + 1] - st[k]) - avg;
             double sq = df * df;
             s = s + sq;
         }
+#pragma unroll
+        for (uint32_t k = 0; k <= U; ++k)
+            st[k] = nx[k];
     }
     for (; r < r1; ++r) {
         double df = (double)(starts[r + 1] - starts[r]) - avg;
